@@ -1,0 +1,218 @@
+#!/usr/bin/env python3
+"""Benchmark of the GPP self-energy kernel on one MI355X.
+
+Workload: the GPP benchmark configuration (number_bands=512, nvband=2,
+ncouls=32768, nodes_per_group=20 -> ngpown=1638, 3 frequencies, band-dependent
+wx), seeded synthetic inputs in the reference layout, resident in HBM before the
+timed region.  One step = one bgw_gpp_solver_device call producing achtemp and
+acht_n1_loc (the noflagOCC_solver sum, GPP/gpp.cpp:339-401).  value = algorithmic
+TFLOP/s at 15 FLOP per (n1, my_igp, ig, iw) term.
+
+Also reported: the same step in the FP32-bulk form (time, speed-up, error vs the
+FP64 result), the full-output call (+achstemp, +asxtemp), the dominant kernel's
+roofline from HIP events on its own stream, and the reference's CPU path timed on
+a bounded sample of the same workload (rank 0, N=1).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT / "berkeleygw-kernels-cpp_amd" / "python"))
+
+FP64_PEAK_TFLOPS = 78.6  # MI355X FP64 vector (= FP64 matrix) peak
+TRAFFIC_FILE = ROOT / "profiles" / "gpp_main_traffic.txt"
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def read_traffic():
+    """HBM bytes per gpp_main launch from the committed rocprofv3 --pmc summary."""
+    if not TRAFFIC_FILE.exists():
+        return None
+    for line in TRAFFIC_FILE.read_text().splitlines():
+        if line.startswith("traffic_bytes_per_launch:"):
+            return float(line.split(":")[1])
+    return None
+
+
+def cpu_baseline(d_full, bands: int):
+    """Times the reference's CPU path (oracle/_ref, built from the reference
+    sources with its GCC flags + OpenMP) on `bands` bands of the workload; falls
+    back to the oracle port when the reference build is absent."""
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import oracle as O  # checker / baseline only
+
+    nc = d_full["ncouls"]
+    sub = dict(d_full, nb=bands, aqsm=d_full["aqsm"][: bands * nc].copy(), aqsn=d_full["aqsn"][: bands * nc].copy(),
+               wx=d_full["wx"][: 3 * bands].copy())
+    cores = os.cpu_count() or 1
+    try:
+        nthr = int(os.environ.get("OMP_NUM_THREADS", cores))
+    except ValueError:
+        nthr = cores
+    kind = "reference" if O.REF_OMP.exists() else "port"
+    t0 = time.perf_counter()
+    if kind == "reference":
+        O.ref_gpp_per_band(sub, kind="omp")
+    else:
+        O.gpp_forward(sub, want=("achtemp", "acht_n1_loc"), nthreads=nthr)
+    dt = time.perf_counter() - t0
+    flops = 15.0 * bands * sub["ngpown"] * nc * 3
+    return {"value": flops / dt / 1e12, "unit": "TFLOP/s", "cores": nthr, "kind": kind,
+            "sample": f"{bands} of 512 bands of the benchmark workload (ngpown 1638 x ncouls 32768 x 3 iw), "
+                      f"{dt:.2f} s wall, reference noflagOCC_solver called per band"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--nb", type=int, default=512, help="number_bands (default: the benchmark's 512)")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-bands", type=int, default=12)
+    ap.add_argument("--no-extras", action="store_true", help="skip the FP32-bulk and full-output legs")
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU: the GPP/FF kernels are single-GPU here (no collectives); "
+              "run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    import bgw
+
+    if not torch.cuda.is_available():
+        raise SystemExit("no GPU visible")
+    dev = torch.device("cuda:0")
+
+    log("generating inputs")
+    d = bgw.gpp_config_inputs(1, nb=args.nb)
+    nb, nvband, ngpown, ncouls = d["nb"], d["nvband"], d["ngpown"], d["ncouls"]
+    T = {k: torch.from_numpy(np.ascontiguousarray(d[k])).to(dev) for k in
+         ("inv", "indinv", "wx", "wtilde", "aqsm", "aqsn", "I_eps", "vcoul")}
+    out = {"achtemp": torch.zeros(3, dtype=torch.complex128, device=dev),
+           "acht_n1_loc": torch.zeros(nb * 3, dtype=torch.complex128, device=dev),
+           "achstemp": torch.zeros(1, dtype=torch.complex128, device=dev),
+           "asxtemp": torch.zeros(3, dtype=torch.complex128, device=dev)}
+    P = {k: v.data_ptr() for k, v in T.items()}
+    stream = torch.cuda.current_stream()
+    ctx = bgw.Context(0)
+    ctx.set_stream(stream.cuda_stream)
+
+    def step(full=False):
+        ctx.gpp_solver_device(nb, nvband, ngpown, ncouls, P["inv"], P["indinv"], P["wx"], P["wtilde"], P["aqsm"],
+                              P["aqsn"], P["I_eps"], P["vcoul"], 1.0, out["achtemp"].data_ptr(),
+                              out["acht_n1_loc"].data_ptr(), out["achstemp"].data_ptr() if full else None,
+                              out["asxtemp"].data_ptr() if full else None)
+
+    def timed(k, full=False):
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(k):
+            step(full)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        ctx.synchronize()
+        return e0.elapsed_time(e1) / k
+
+    flops = bgw.gpp_flops(nb, ngpown, ncouls)
+    res = {}
+    for prec, name in ((bgw.BGW_FP64, "fp64"), (bgw.BGW_FP32_BULK, "fp32_bulk")):
+        if name == "fp32_bulk" and args.no_extras:
+            continue
+        ctx.set_precision(prec)
+        log(f"[{name}] warmup {args.warmup}")
+        for _ in range(args.warmup):
+            step()
+        torch.cuda.synchronize()
+        ctx.synchronize()
+        ctx.reset_timing()
+        ctx.set_timing(True)
+        log(f"[{name}] timing {args.steps} steps")
+        ms = timed(args.steps)
+        ctx.set_timing(False)
+        kms, kn = ctx.kernel_time("gpp_main")
+        res[name] = dict(ms=ms, kernel_ms=kms / max(kn, 1), launches=kn,
+                         achtemp=out["achtemp"].cpu().numpy().copy(),
+                         acht_n1_loc=out["acht_n1_loc"].cpu().numpy().reshape(nb, 3).copy())
+        log(f"[{name}] {ms:.3f} ms/step, gpp_main {res[name]['kernel_ms']:.3f} ms")
+    ctx.set_precision(bgw.BGW_FP64)
+
+    full_ms = None
+    if not args.no_extras:
+        step(True)
+        full_ms = timed(max(2, args.steps // 2), full=True)
+        log(f"[full outputs] {full_ms:.3f} ms/step")
+
+    f64 = res["fp64"]
+    value = flops / (f64["ms"] * 1e-3) / 1e12
+    kern_tflops = flops / (f64["kernel_ms"] * 1e-3) / 1e12
+    extras = {"fp64_gpp_main_ms": f64["kernel_ms"], "fp64_step_ms": f64["ms"], "full_outputs_step_ms": full_ms}
+    if "fp32_bulk" in res:
+        r32 = res["fp32_bulk"]
+        err_loc = float(np.max(np.abs(r32["acht_n1_loc"] - f64["acht_n1_loc"]) / np.abs(f64["acht_n1_loc"])))
+        err_ach = float(np.max(np.abs(r32["achtemp"] - f64["achtemp"]) / np.abs(f64["achtemp"])))
+        extras.update({"fp32_bulk_step_ms": r32["ms"], "fp32_bulk_speedup": f64["ms"] / r32["ms"],
+                       "fp32_bulk_rel_err_achtemp": err_ach, "fp32_bulk_rel_err_acht_n1_loc": err_loc,
+                       "fp32_bulk_tflops": flops / (r32["ms"] * 1e-3) / 1e12})
+
+    # parity spot check of the timed result: 2 bands against the oracle
+    try:
+        sys.path.insert(0, str(ROOT / "oracle"))
+        import oracle as O
+
+        sub = dict(d, nb=2, aqsm=d["aqsm"][: 2 * ncouls].copy(), aqsn=d["aqsn"][: 2 * ncouls].copy(),
+                   wx=d["wx"][:6].copy())
+        o = O.gpp_forward(sub, want=("achtemp", "acht_n1_loc"))
+        extras["parity_bands_checked"] = 2
+        extras["parity_max_rel_err"] = float(np.max(np.abs(f64["acht_n1_loc"][:2] - o["acht_n1_loc"]) /
+                                                    np.abs(o["acht_n1_loc"])))
+    except Exception as e:  # the checker is optional for the measurement itself
+        extras["parity_error"] = repr(e)
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        log("cpu baseline")
+        cpu = cpu_baseline(d, args.cpu_bands)
+
+    line = {
+        "metric": "GPP Σ-kernel s & FP64 TFLOP/s (1 GPU): plain FP64 vs FP32-bulk at ≤1e-6 rel err",
+        "value": value,
+        "unit": "TFLOP/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": f64["ms"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded SplitMix64, benchmark distributions)",
+        "config": {"workload": "GPP noflagOCC_solver sum, achtemp + acht_n1_loc",
+                   "number_bands": nb, "nvband": nvband, "ncouls": ncouls, "nodes_per_group": 20,
+                   "ngpown": ngpown, "frequencies": 3, "flop_per_term": 15, "parallelism": "single GPU"},
+        "roofline": {"bound": "valu", "achieved": kern_tflops, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": kern_tflops / FP64_PEAK_TFLOPS, "traffic": read_traffic(),
+                     "kernel": "k_gpp_main", "flop_per_launch": flops},
+        "cpu_baseline": cpu,
+        "extras": extras,
+    }
+    print(json.dumps(line), flush=True)
+    ctx.close()
+
+
+if __name__ == "__main__":
+    main()

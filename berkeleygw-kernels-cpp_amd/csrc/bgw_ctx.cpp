@@ -1,0 +1,411 @@
+// bgw_ctx.cpp -- context lifecycle, errors, pools, timing and the host-pointer
+// (synchronous, reference-like) forms of the C ABI.  The host forms validate
+// their arguments on the host, stage arrays into context-owned device buffers,
+// run the `_device` path and copy results back, so a caller sees the
+// reference's blocking semantics (GPP/gpp.cpp:283-285, FF/ff.cpp:700-728).
+#include <cstdio>
+#include <cstring>
+
+#include "bgw_internal.h"
+
+namespace bgw {
+
+int fail(bgw_ctx* ctx, int code, const std::string& msg) {
+  if (ctx) ctx->err = msg;
+  return code;
+}
+
+int hip_fail(bgw_ctx* ctx, hipError_t e, const char* what) {
+  std::string m = std::string(what) + ": " + hipGetErrorString(e);
+  return fail(ctx, e == hipErrorOutOfMemory ? BGW_ENOMEM : BGW_EHIP, m);
+}
+
+int pool_get(bgw_ctx* ctx, const char* name, size_t bytes, void** out) {
+  DevBuf& b = ctx->pool[name];
+  if (bytes == 0) bytes = 16;
+  if (b.bytes < bytes) {
+    if (b.ptr) {
+      hipError_t e = hipStreamSynchronize(ctx->stream);
+      if (e != hipSuccess) return hip_fail(ctx, e, "hipStreamSynchronize");
+      (void)hipFree(b.ptr);
+      b.ptr = nullptr;
+      b.bytes = 0;
+    }
+    size_t want = (bytes + 255) & ~size_t(255);
+    hipError_t e = hipMalloc(&b.ptr, want);
+    if (e != hipSuccess) {
+      b.ptr = nullptr;
+      return hip_fail(ctx, e, (std::string("hipMalloc ") + name).c_str());
+    }
+    b.bytes = want;
+  }
+  *out = b.ptr;
+  return BGW_OK;
+}
+
+KernelTimer::KernelTimer(bgw_ctx* c, const char* name) : ctx(c), on(c->timing) {
+  if (!on) return;
+  tl.name = name;
+  for (hipEvent_t* ev : {&tl.start, &tl.stop}) {
+    if (!ctx->free_events.empty()) {
+      *ev = ctx->free_events.back();
+      ctx->free_events.pop_back();
+    } else if (hipEventCreate(ev) != hipSuccess) {
+      on = false;
+      return;
+    }
+  }
+  (void)hipEventRecord(tl.start, ctx->stream);
+}
+
+KernelTimer::~KernelTimer() {
+  if (!on) return;
+  (void)hipEventRecord(tl.stop, ctx->stream);
+  ctx->pending.push_back(tl);
+}
+
+int check_launch(bgw_ctx* ctx, const char* what) {
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return hip_fail(ctx, e, what);
+  return BGW_OK;
+}
+
+bool mul_ok(size_t a, size_t b, size_t* out) {
+  if (a != 0 && b > SIZE_MAX / a) return false;
+  *out = a * b;
+  return true;
+}
+
+}  // namespace bgw
+
+using namespace bgw;
+
+extern "C" {
+
+int bgw_create(bgw_ctx** out, int device) {
+  if (!out) return BGW_EINVAL;
+  *out = nullptr;
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return BGW_ENODEV;
+  if (device < 0 || device >= n) return BGW_EINVAL;
+  bgw_ctx* ctx = new bgw_ctx();
+  ctx->device = device;
+  hipError_t e = hipSetDevice(device);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&ctx->own_stream, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipDeviceGetAttribute(&ctx->cus, hipDeviceAttributeMultiprocessorCount, device);
+  if (e == hipSuccess) e = hipMalloc(&ctx->d_flag, 256);
+  if (e != hipSuccess) {
+    delete ctx;
+    return e == hipErrorOutOfMemory ? BGW_ENOMEM : BGW_ENODEV;
+  }
+  ctx->stream = ctx->own_stream;
+  reset_flag(ctx->d_flag, ctx->stream);
+  if (hipStreamSynchronize(ctx->stream) != hipSuccess) {
+    delete ctx;
+    return BGW_ENODEV;
+  }
+  *out = ctx;
+  return BGW_OK;
+}
+
+void bgw_destroy(bgw_ctx* ctx) {
+  if (!ctx) return;
+  (void)hipSetDevice(ctx->device);
+  (void)hipStreamSynchronize(ctx->stream);
+  for (auto& kv : ctx->pool)
+    if (kv.second.ptr) (void)hipFree(kv.second.ptr);
+  for (auto& tl : ctx->pending) {
+    (void)hipEventDestroy(tl.start);
+    (void)hipEventDestroy(tl.stop);
+  }
+  for (hipEvent_t ev : ctx->free_events) (void)hipEventDestroy(ev);
+  if (ctx->d_flag) (void)hipFree(ctx->d_flag);
+  if (ctx->own_stream) (void)hipStreamDestroy(ctx->own_stream);
+  delete ctx;
+}
+
+const char* bgw_last_error(const bgw_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int bgw_set_stream(bgw_ctx* ctx, void* s) {
+  if (!ctx) return BGW_EINVAL;
+  ctx->stream = s ? static_cast<hipStream_t>(s) : ctx->own_stream;
+  return BGW_OK;
+}
+
+void* bgw_get_stream(bgw_ctx* ctx) { return ctx ? static_cast<void*>(ctx->stream) : nullptr; }
+
+int bgw_set_precision(bgw_ctx* ctx, int precision) {
+  if (!ctx) return BGW_EINVAL;
+  if (precision != BGW_FP64 && precision != BGW_FP32_BULK)
+    return fail(ctx, BGW_EINVAL, "unknown precision");
+  ctx->precision = precision;
+  return BGW_OK;
+}
+
+int bgw_synchronize(bgw_ctx* ctx) {
+  if (!ctx) return BGW_EINVAL;
+  hipError_t e = hipSetDevice(ctx->device);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  if (e != hipSuccess) return hip_fail(ctx, e, "hipStreamSynchronize");
+  int flag = 0;
+  e = hipMemcpy(&flag, ctx->d_flag, sizeof(int), hipMemcpyDeviceToHost);
+  if (e != hipSuccess) return hip_fail(ctx, e, "flag readback");
+  if (flag) {
+    reset_flag(ctx->d_flag, ctx->stream);
+    (void)hipStreamSynchronize(ctx->stream);
+    return fail(ctx, BGW_EINVAL, "index out of range in inv_igp_index/indinv (clamped)");
+  }
+  return BGW_OK;
+}
+
+int bgw_set_timing(bgw_ctx* ctx, int on) {
+  if (!ctx) return BGW_EINVAL;
+  ctx->timing = on != 0;
+  return BGW_OK;
+}
+
+static int resolve_timing(bgw_ctx* ctx) {
+  if (ctx->pending.empty()) return BGW_OK;
+  hipError_t e = hipStreamSynchronize(ctx->stream);
+  if (e != hipSuccess) return hip_fail(ctx, e, "hipStreamSynchronize");
+  for (auto& tl : ctx->pending) {
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, tl.start, tl.stop) == hipSuccess) {
+      auto& slot = ctx->ktime[tl.name];
+      slot.first += ms;
+      slot.second += 1;
+    }
+    ctx->free_events.push_back(tl.start);
+    ctx->free_events.push_back(tl.stop);
+  }
+  ctx->pending.clear();
+  return BGW_OK;
+}
+
+int bgw_kernel_time(bgw_ctx* ctx, const char* kernel, double* total_ms, long* launches) {
+  if (!ctx || !kernel) return BGW_EINVAL;
+  int rc = resolve_timing(ctx);
+  if (rc) return rc;
+  auto it = ctx->ktime.find(kernel);
+  if (total_ms) *total_ms = it == ctx->ktime.end() ? 0.0 : it->second.first;
+  if (launches) *launches = it == ctx->ktime.end() ? 0 : it->second.second;
+  return BGW_OK;
+}
+
+int bgw_reset_timing(bgw_ctx* ctx) {
+  if (!ctx) return BGW_EINVAL;
+  int rc = resolve_timing(ctx);
+  ctx->ktime.clear();
+  return rc;
+}
+
+int bgw_device_cus(bgw_ctx* ctx) { return ctx ? ctx->cus : 0; }
+
+}  // extern "C"
+
+// ------------------------------------------------------------------ host forms
+namespace {
+
+struct Stage {
+  bgw_ctx* ctx;
+  int rc = BGW_OK;
+  explicit Stage(bgw_ctx* c) : ctx(c) {}
+  // uploads `bytes` from host `src` into pool slot `name`
+  template <typename T>
+  T* up(const char* name, const T* src, size_t count) {
+    if (rc) return nullptr;
+    size_t bytes;
+    if (!mul_ok(count, sizeof(T), &bytes)) {
+      rc = fail(ctx, BGW_EINVAL, "size overflow");
+      return nullptr;
+    }
+    void* d = nullptr;
+    rc = pool_get(ctx, name, bytes, &d);
+    if (rc) return nullptr;
+    hipError_t e = hipMemcpyAsync(d, src, bytes, hipMemcpyHostToDevice, ctx->stream);
+    if (e != hipSuccess) {
+      rc = hip_fail(ctx, e, "upload");
+      return nullptr;
+    }
+    return static_cast<T*>(d);
+  }
+  double* out(const char* name, size_t count) {
+    if (rc) return nullptr;
+    void* d = nullptr;
+    rc = pool_get(ctx, name, count * sizeof(double), &d);
+    return rc ? nullptr : static_cast<double*>(d);
+  }
+  void down(double* host, const double* dev, size_t count) {
+    if (rc || !host) return;
+    hipError_t e = hipMemcpyAsync(host, dev, count * sizeof(double), hipMemcpyDeviceToHost, ctx->stream);
+    if (e != hipSuccess) rc = hip_fail(ctx, e, "download");
+  }
+};
+
+int check_ptrs(bgw_ctx* ctx, std::initializer_list<const void*> ps) {
+  for (const void* p : ps)
+    if (!p) return fail(ctx, BGW_EINVAL, "null array argument");
+  return BGW_OK;
+}
+
+// GPP index domain: inv_igp_index in [0,ncouls] (the reference's own init reaches
+// ncouls, GPP/gpp.cpp:263-264), indinv[ncouls+1] in [0,ncouls).
+int check_gpp_indices(bgw_ctx* ctx, size_t ngpown, size_t ncouls, const int* inv, const int* indinv) {
+  for (size_t p = 0; p < ngpown; ++p)
+    if (inv[p] < 0 || (size_t)inv[p] > ncouls)
+      return fail(ctx, BGW_EINVAL, "inv_igp_index[" + std::to_string(p) + "] out of [0,ncouls]");
+  for (size_t g = 0; g <= ncouls; ++g)
+    if (indinv[g] < 0 || (size_t)indinv[g] >= ncouls)
+      return fail(ctx, BGW_EINVAL, "indinv[" + std::to_string(g) + "] out of [0,ncouls)");
+  return BGW_OK;
+}
+
+int gpp_sizes(bgw_ctx* ctx, size_t nb, size_t ngpown, size_t ncouls, size_t* n_bg, size_t* n_pg) {
+  if (nb == 0 || ngpown == 0 || ncouls == 0) return fail(ctx, BGW_EINVAL, "zero size");
+  if (ncouls >= (size_t)INT32_MAX || ngpown >= (size_t)INT32_MAX || nb >= (size_t)INT32_MAX)
+    return fail(ctx, BGW_EINVAL, "size exceeds int range");
+  if (!mul_ok(nb, ncouls, n_bg) || !mul_ok(ngpown, ncouls, n_pg) || *n_pg > SIZE_MAX / 16 ||
+      *n_bg > SIZE_MAX / 16)
+    return fail(ctx, BGW_EINVAL, "size overflow");
+  return BGW_OK;
+}
+
+int finish(bgw_ctx* ctx, int rc) {
+  if (rc) {
+    (void)hipStreamSynchronize(ctx->stream);
+    return rc;
+  }
+  return bgw_synchronize(ctx);
+}
+
+int gpp_host(bgw_ctx* ctx, size_t nb, size_t nvband, size_t ngpown, size_t ncouls, const int* inv,
+             const int* indinv, const double* wx, int wx_per_band, const double* wtilde,
+             const double* aqsm, const double* aqsn, const double* I_eps, const double* vcoul,
+             double occ, double* achtemp, double* acht_n1_loc, double* achstemp, double* asxtemp) {
+  if (!ctx) return BGW_EINVAL;
+  size_t n_bg, n_pg;
+  int rc = check_ptrs(ctx, {inv, indinv, wx, wtilde, aqsm, aqsn, I_eps, vcoul, achtemp});
+  if (!rc) rc = gpp_sizes(ctx, nb, ngpown, ncouls, &n_bg, &n_pg);
+  if (!rc && nvband > nb) rc = fail(ctx, BGW_EINVAL, "nvband > number_bands");
+  if (!rc) rc = check_gpp_indices(ctx, ngpown, ncouls, inv, indinv);
+  if (rc) return rc;
+  if (hipSetDevice(ctx->device) != hipSuccess) return fail(ctx, BGW_ENODEV, "hipSetDevice");
+  Stage st(ctx);
+  const int* d_inv = st.up("h_inv", inv, ngpown);
+  const int* d_indinv = st.up("h_indinv", indinv, ncouls + 1);
+  const double* d_wx = st.up("h_wx", wx, wx_per_band ? nb * 3 : 3);
+  const double* d_wt = st.up("h_wtilde", wtilde, 2 * n_pg);
+  const double* d_am = st.up("h_aqsm", aqsm, 2 * n_bg);
+  const double* d_an = st.up("h_aqsn", aqsn, 2 * n_bg);
+  const double* d_I = st.up("h_ieps", I_eps, 2 * n_pg);
+  const double* d_vc = st.up("h_vcoul", vcoul, ncouls);
+  double* d_ach = st.out("h_achtemp", 6);
+  double* d_loc = acht_n1_loc ? st.out("h_acht_n1_loc", 6 * nb) : nullptr;
+  double* d_achs = achstemp ? st.out("h_achstemp", 2) : nullptr;
+  double* d_asx = asxtemp ? st.out("h_asxtemp", 6) : nullptr;
+  if (st.rc) return finish(ctx, st.rc);
+  rc = gpp_forward_device(ctx, nb, nvband, ngpown, ncouls, d_inv, d_indinv, d_wx, wx_per_band, d_wt,
+                          d_am, d_an, d_I, d_vc, occ, d_ach, d_loc, d_achs, d_asx);
+  if (rc) return finish(ctx, rc);
+  st.down(achtemp, d_ach, 6);
+  st.down(acht_n1_loc, d_loc, 6 * nb);
+  st.down(achstemp, d_achs, 2);
+  st.down(asxtemp, d_asx, 6);
+  return finish(ctx, st.rc);
+}
+
+}  // namespace
+
+extern "C" {
+
+int bgw_noflagOCC_solver(bgw_ctx* ctx, size_t number_bands, size_t ngpown, size_t ncouls,
+                         const int* inv_igp_index, const int* indinv, const double* wx_array,
+                         const double* wtilde_array, const double* aqsmtemp,
+                         const double* aqsntemp, const double* I_eps_array, const double* vcoul,
+                         double* achtemp) {
+  return gpp_host(ctx, number_bands, 0, ngpown, ncouls, inv_igp_index, indinv, wx_array, 0,
+                  wtilde_array, aqsmtemp, aqsntemp, I_eps_array, vcoul, 0.0, achtemp, nullptr,
+                  nullptr, nullptr);
+}
+
+int bgw_gpp_solver(bgw_ctx* ctx, size_t number_bands, size_t nvband, size_t ngpown, size_t ncouls,
+                   const int* inv_igp_index, const int* indinv, const double* wx_array,
+                   const double* wtilde_array, const double* aqsmtemp, const double* aqsntemp,
+                   const double* I_eps_array, const double* vcoul, double occ, double* achtemp,
+                   double* acht_n1_loc, double* achstemp, double* asxtemp) {
+  return gpp_host(ctx, number_bands, nvband, ngpown, ncouls, inv_igp_index, indinv, wx_array, 1,
+                  wtilde_array, aqsmtemp, aqsntemp, I_eps_array, vcoul, occ, achtemp, acht_n1_loc,
+                  achstemp, asxtemp);
+}
+
+int bgw_gpp_solver_device(bgw_ctx* ctx, size_t number_bands, size_t nvband, size_t ngpown,
+                          size_t ncouls, const int* inv_igp_index, const int* indinv,
+                          const double* wx_array, const double* wtilde_array,
+                          const double* aqsmtemp, const double* aqsntemp,
+                          const double* I_eps_array, const double* vcoul, double occ,
+                          double* achtemp, double* acht_n1_loc, double* achstemp,
+                          double* asxtemp) {
+  if (!ctx) return BGW_EINVAL;
+  size_t n_bg, n_pg;
+  int rc = check_ptrs(ctx, {inv_igp_index, indinv, wx_array, wtilde_array, aqsmtemp, aqsntemp,
+                            I_eps_array, vcoul, achtemp});
+  if (!rc) rc = gpp_sizes(ctx, number_bands, ngpown, ncouls, &n_bg, &n_pg);
+  if (!rc && nvband > number_bands) rc = fail(ctx, BGW_EINVAL, "nvband > number_bands");
+  if (rc) return rc;
+  if (hipSetDevice(ctx->device) != hipSuccess) return fail(ctx, BGW_ENODEV, "hipSetDevice");
+  return gpp_forward_device(ctx, number_bands, nvband, ngpown, ncouls, inv_igp_index, indinv,
+                            wx_array, 1, wtilde_array, aqsmtemp, aqsntemp, I_eps_array, vcoul, occ,
+                            achtemp, acht_n1_loc, achstemp, asxtemp);
+}
+
+int bgw_gpp_backward(bgw_ctx* ctx, size_t nb, size_t nvband, size_t ngpown, size_t ncouls,
+                     const int* inv, const int* indinv, const double* wx, const double* wtilde,
+                     const double* aqsm, const double* aqsn, const double* I_eps,
+                     const double* vcoul, double occ, double* dsigma, double* dwt, double* dI) {
+  if (!ctx) return BGW_EINVAL;
+  size_t n_bg, n_pg;
+  int rc = check_ptrs(ctx, {inv, indinv, wx, wtilde, aqsm, aqsn, I_eps, vcoul});
+  if (!rc) rc = gpp_sizes(ctx, nb, ngpown, ncouls, &n_bg, &n_pg);
+  if (!rc && nvband > nb) rc = fail(ctx, BGW_EINVAL, "nvband > number_bands");
+  if (!rc) rc = check_gpp_indices(ctx, ngpown, ncouls, inv, indinv);
+  if (rc) return rc;
+  if (hipSetDevice(ctx->device) != hipSuccess) return fail(ctx, BGW_ENODEV, "hipSetDevice");
+  Stage st(ctx);
+  const int* d_inv = st.up("h_inv", inv, ngpown);
+  const int* d_indinv = st.up("h_indinv", indinv, ncouls + 1);
+  const double* d_wx = st.up("h_wx", wx, nb * 3);
+  const double* d_wt = st.up("h_wtilde", wtilde, 2 * n_pg);
+  const double* d_am = st.up("h_aqsm", aqsm, 2 * n_bg);
+  const double* d_an = st.up("h_aqsn", aqsn, 2 * n_bg);
+  const double* d_I = st.up("h_ieps", I_eps, 2 * n_pg);
+  const double* d_vc = st.up("h_vcoul", vcoul, ncouls);
+  double* d_ds = dsigma ? st.out("h_dsigma", 6) : nullptr;
+  double* d_dw = dwt ? st.out("h_dwt", 2 * n_pg) : nullptr;
+  double* d_dI = dI ? st.out("h_dI", 2 * n_pg) : nullptr;
+  if (st.rc) return finish(ctx, st.rc);
+  rc = gpp_backward_device(ctx, nb, nvband, ngpown, ncouls, d_inv, d_indinv, d_wx, d_wt, d_am, d_an,
+                           d_I, d_vc, occ, d_ds, d_dw, d_dI);
+  if (rc) return finish(ctx, rc);
+  st.down(dsigma, d_ds, 6);
+  st.down(dwt, d_dw, 2 * n_pg);
+  st.down(dI, d_dI, 2 * n_pg);
+  return finish(ctx, st.rc);
+}
+
+int bgw_gpp_backward_device(bgw_ctx* ctx, size_t nb, size_t nvband, size_t ngpown, size_t ncouls,
+                            const int* inv, const int* indinv, const double* wx,
+                            const double* wtilde, const double* aqsm, const double* aqsn,
+                            const double* I_eps, const double* vcoul, double occ, double* dsigma,
+                            double* dwt, double* dI) {
+  if (!ctx) return BGW_EINVAL;
+  size_t n_bg, n_pg;
+  int rc = check_ptrs(ctx, {inv, indinv, wx, wtilde, aqsm, aqsn, I_eps, vcoul});
+  if (!rc) rc = gpp_sizes(ctx, nb, ngpown, ncouls, &n_bg, &n_pg);
+  if (!rc && nvband > nb) rc = fail(ctx, BGW_EINVAL, "nvband > number_bands");
+  if (rc) return rc;
+  if (hipSetDevice(ctx->device) != hipSuccess) return fail(ctx, BGW_ENODEV, "hipSetDevice");
+  return gpp_backward_device(ctx, nb, nvband, ngpown, ncouls, inv, indinv, wx, wtilde, aqsm, aqsn,
+                             I_eps, vcoul, occ, dsigma, dwt, dI);
+}
+
+}  // extern "C"

@@ -1,0 +1,99 @@
+// bgw_internal.h -- context, error handling and device-buffer plumbing shared by
+// the host-side translation units of libbgw (not part of the public ABI).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "bgw.h"
+
+namespace bgw {
+
+// Grow-only device allocation owned by a context.
+struct DevBuf {
+  void* ptr = nullptr;
+  size_t bytes = 0;
+};
+
+struct TimedLaunch {
+  std::string name;
+  hipEvent_t start, stop;
+};
+
+}  // namespace bgw
+
+struct bgw_ctx {
+  int device = 0;
+  int cus = 0;
+  hipStream_t own_stream = nullptr;
+  hipStream_t stream = nullptr;
+  int precision = BGW_FP64;
+  std::string err;
+  int* d_flag = nullptr;  // device word: bad-index report of *_device calls
+  // named grow-only pools: workspace and host-API staging of inputs/outputs
+  std::map<std::string, bgw::DevBuf> pool;
+  // timing
+  bool timing = false;
+  std::vector<bgw::TimedLaunch> pending;
+  std::vector<hipEvent_t> free_events;
+  std::map<std::string, std::pair<double, long>> ktime;
+};
+
+namespace bgw {
+
+int fail(bgw_ctx* ctx, int code, const std::string& msg);
+int hip_fail(bgw_ctx* ctx, hipError_t e, const char* what);
+
+// Returns a device buffer of at least `bytes` from pool slot `name`.
+int pool_get(bgw_ctx* ctx, const char* name, size_t bytes, void** out);
+
+// Brackets a launch with HIP events when timing is on.
+struct KernelTimer {
+  bgw_ctx* ctx;
+  TimedLaunch tl;
+  bool on;
+  KernelTimer(bgw_ctx* c, const char* name);
+  ~KernelTimer();
+};
+
+int check_launch(bgw_ctx* ctx, const char* what);
+
+// Overflow-checked product of sizes; returns false on overflow.
+bool mul_ok(size_t a, size_t b, size_t* out);
+
+// Zero-fill kernels usable inside captured graphs (no memset nodes).
+void fill_zero(void* p, size_t bytes, hipStream_t s);
+void reset_flag(int* d_flag, hipStream_t s);
+
+// Device-side index validation: out-of-range indices set *flag (the kernels
+// clamp them so no access leaves its array).
+void launch_check_indices(const int* inv_igp_index, size_t ngpown, const int* indinv,
+                          size_t n_indinv, int inv_max, int ncouls, int* flag, hipStream_t s);
+
+}  // namespace bgw
+
+// Internal entry points implemented in the kernel translation units.
+namespace bgw {
+int gpp_forward_device(bgw_ctx* ctx, size_t nb, size_t nvband, size_t ngpown, size_t ncouls,
+                       const int* inv, const int* indinv, const double* wx, int wx_per_band,
+                       const double* wtilde, const double* aqsm, const double* aqsn,
+                       const double* I_eps, const double* vcoul, double occ, double* achtemp,
+                       double* acht_n1_loc, double* achstemp, double* asxtemp);
+int gpp_backward_device(bgw_ctx* ctx, size_t nb, size_t nvband, size_t ngpown, size_t ncouls,
+                        const int* inv, const int* indinv, const double* wx,
+                        const double* wtilde, const double* aqsm, const double* aqsn,
+                        const double* I_eps, const double* vcoul, double occ,
+                        double* dsigma, double* dwt, double* dI);
+
+// FF: op 0 = achsDtemp, 1 = asxDtemp, 2 = achDtemp_cor
+int ff_device(bgw_ctx* ctx, int op, int nb, int nvband, int nfreqeval, int ncouls, int ngpown,
+              int nFreq, double freqevalmin, double freqevalstep, double occ, const double* ekq,
+              const double* dFreqGrid, const int* inv, const int* indinv, const double* aqsm,
+              const double* aqsn, const double* vcoul, const double* I_epsR,
+              const double* I_epsA, double* out);
+}  // namespace bgw

@@ -1,0 +1,51 @@
+// contract.h -- arguments of the MFMA trace contraction and the helpers around it.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "bgw_internal.h"
+
+namespace bgw {
+
+constexpr int kModePlain = 0;  // B = X
+constexpr int kModeDiff = 1;   // B = X - Y   (I_epsR - I_epsA, FF/ff.cpp:399-404)
+constexpr int kModeMask = 2;   // B = X where |X|^2 > thr2, else 0
+
+struct ContractArgs {
+  const double2* A;        // [rows of A][K]   aqsntemp
+  int K;                   // ncouls
+  const double2* Xa;       // slices of X for jobs [0, jobs_a)
+  const double2* Xb;       // slices of X for jobs [jobs_a, njobs) (nullable)
+  const double2* Y;        // subtracted slices (nullable)
+  long slice_stride;       // elements between frequency slices (ngpown*ncouls)
+  int jobs_a;
+  int N;                   // ngpown
+  const int* slice_of_job; // [njobs]
+  const int* rows;         // [njobs][max_rows] band indices
+  const double2* wts;      // [njobs][max_rows] complex row weights
+  const int* cnt;          // [njobs] active rows
+  int max_rows;
+  const int* inv;          // inv_igp_index
+  const int* indinv;
+  int gpp_index;           // 1: indinv has ncouls+1 entries (GPP); 0: FF
+  const double2* aqsm;
+  const double* vcoul;
+  int ncouls;
+  double thr2;             // mask threshold (kModeMask)
+  int ksplit;              // set by contract_launch
+  double4* partial;        // set by contract_launch
+  double2* out;            // c128[1] result (device)
+};
+
+int contract_launch(bgw_ctx* ctx, ContractArgs a, int njobs, int max_row_tiles, const char* tname);
+int contract_row_tile();
+
+// rows[i] = i, wts[i] = w, cnt[0] = n, slice[0] = slice.
+void launch_identity_rows(int* rows, double2* wts, int* cnt, int* slice, int n, double2 w, int slice_idx,
+                          hipStream_t s);
+// out[k] = sum_w partial[w*nout + k] in fixed order (double-double), k < nout.
+void launch_reduce_cdd(const double4* partial, int W, int nout, double2* out, hipStream_t s);
+// out[1..n-1] = out[0]
+void launch_replicate(double2* out, int n, hipStream_t s);
+
+}  // namespace bgw

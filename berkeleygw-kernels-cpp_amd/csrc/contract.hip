@@ -1,0 +1,240 @@
+// contract.hip -- FP64-MFMA "trace contraction" shared by the GPP static-COH term
+// and the three FF kernels:
+//
+//   out = sum_job sum_{r < cnt[job]} sum_p  W(job,r,p) * sum_g A(rows[job][r], g) * B_job(p, g)
+//   W(job,r,p) = wts[job][r] * vcoul(igp(p)) * conj(aqsm(rows[job][r], igp(p)))
+//
+// B_job is one [ngpown x ncouls] frequency slice of X (optionally minus the same
+// slice of Y, or masked to |X| > to1).  This is the common form of
+//   achsDtemp     FF/ff.cpp:191-208   (one job: I_epsR slice 1, w = -1/2)
+//   asxDtemp      FF/ff.cpp:242-281   (jobs = slices of I_epsR / I_epsA, w = occ*fact1|fact2)
+//   achDtemp_cor  FF/ff.cpp:332-372 + :381-451 (jobs = slices of R - A, w = fact1|fact2, *-1/2 in Kernel2)
+//   achstemp      static COH term of the GPP kernel (one job: I_eps masked, w = -1/2)
+// The inner sum_g A*B^T is a complex NT-GEMM tile on v_mfma_f64_16x16x4f64 (four
+// real products per complex block); the weights are applied in the epilogue and
+// every workgroup writes one double-double partial, summed in fixed order.
+#include "bgw_dd.h"
+#include "bgw_fact.h"
+#include "bgw_internal.h"
+#include "contract.h"
+#include "gpp_common.h"
+
+namespace bgw {
+namespace {
+
+typedef double f64x4 __attribute__((ext_vector_type(4)));
+
+constexpr int BM = 64;   // rows (bands) per workgroup tile
+constexpr int BN = 64;   // columns (G') per workgroup tile
+constexpr int BK = 16;   // G-vectors per LDS stage
+constexpr int NT = 256;  // 4 waves, each a 32x32 sub-tile (2x2 MFMA blocks)
+constexpr int LDP = BM + 2;  // padded LDS row
+
+template <int MODE>
+__device__ inline double2 load_b(const double2* __restrict__ X, const double2* __restrict__ Y, size_t idx,
+                                 double thr2) {
+  double2 v = X[idx];
+  if (MODE == kModeDiff) {
+    const double2 y = Y[idx];
+    v.x -= y.x;
+    v.y -= y.y;
+  } else if (MODE == kModeMask) {
+    if (!(v.x * v.x + v.y * v.y > thr2)) v = make_double2(0.0, 0.0);
+  }
+  return v;
+}
+
+template <int MODE>
+__global__ __launch_bounds__(NT) void k_contract(ContractArgs a) {
+  __shared__ double As[2][BK][LDP];  // [re/im][k][row]
+  __shared__ double Bs[2][BK][LDP];  // [re/im][k][col]
+  __shared__ cdd red[NT / 64];
+
+  const int job = blockIdx.z;
+  const int ntn = (a.N + BN - 1) / BN;
+  const int ct = blockIdx.x % ntn;
+  const int ks = blockIdx.x / ntn;
+  const int rt = blockIdx.y;
+  const int nrows = a.cnt[job];
+  const size_t pid = ((size_t)job * gridDim.y + rt) * gridDim.x + blockIdx.x;
+  if (rt * BM >= nrows) {  // empty row tile: a zero partial keeps the reduction order fixed
+    if (threadIdx.x == 0) cdd_store(&a.partial[pid], cdd_zero());
+    return;
+  }
+  const int slice = a.slice_of_job[job];
+  const double2* __restrict__ X = (a.Xb && job >= a.jobs_a ? a.Xb : a.Xa) + (size_t)slice * a.slice_stride;
+  const double2* __restrict__ Y = a.Y ? a.Y + (size_t)slice * a.slice_stride : nullptr;
+  const int* __restrict__ rows = a.rows + (size_t)job * a.max_rows;
+  const int K = a.K;
+  const int kper = ((K + a.ksplit - 1) / a.ksplit + BK - 1) / BK * BK;
+  const int k0 = ks * kper, k1 = min(K, k0 + kper);
+
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wr = (wave >> 1) * 32, wc = (wave & 1) * 32;
+
+  f64x4 dr[2][2], di[2][2];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      dr[i][j] = (f64x4){0.0, 0.0, 0.0, 0.0};
+      di[i][j] = (f64x4){0.0, 0.0, 0.0, 0.0};
+    }
+
+  // each thread stages 4 A and 4 B elements per stage: idx = tid + 256*i -> (r, k)
+  int arow[4];
+  bool aok[4];
+  size_t bbase[4];
+  bool bok[4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const int idx = tid + NT * i;
+    const int r = idx / BK;
+    const int gr = rt * BM + r;
+    aok[i] = gr < nrows;
+    arow[i] = aok[i] ? rows[gr] : 0;
+    const int gc = ct * BN + r;
+    bok[i] = gc < a.N;
+    bbase[i] = (size_t)(bok[i] ? gc : 0) * K;
+  }
+
+  for (int kb = k0; kb < k1; kb += BK) {
+    double2 av[4], bv[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int idx = tid + NT * i;
+      const int kl = idx % BK;
+      const int k = kb + kl;
+      const bool kin = k < k1;
+      av[i] = (aok[i] && kin) ? a.A[(size_t)arow[i] * K + k] : make_double2(0.0, 0.0);
+      bv[i] = (bok[i] && kin) ? load_b<MODE>(X, Y, bbase[i] + k, a.thr2) : make_double2(0.0, 0.0);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int idx = tid + NT * i;
+      const int r = idx / BK, kl = idx % BK;
+      As[0][kl][r] = av[i].x;
+      As[1][kl][r] = av[i].y;
+      Bs[0][kl][r] = bv[i].x;
+      Bs[1][kl][r] = bv[i].y;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int kk = 0; kk < BK; kk += 4) {
+      const int kr = kk + (lane >> 4), cl = lane & 15;
+      double ar[2], ai[2], nai[2], br[2], bi[2];
+#pragma unroll
+      for (int i = 0; i < 2; ++i) {
+        ar[i] = As[0][kr][wr + i * 16 + cl];
+        ai[i] = As[1][kr][wr + i * 16 + cl];
+        nai[i] = -ai[i];
+        br[i] = Bs[0][kr][wc + i * 16 + cl];
+        bi[i] = Bs[1][kr][wc + i * 16 + cl];
+      }
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+          dr[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[i], br[j], dr[i][j], 0, 0, 0);
+          dr[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(nai[i], bi[j], dr[i][j], 0, 0, 0);
+          di[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[i], bi[j], di[i][j], 0, 0, 0);
+          di[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai[i], br[j], di[i][j], 0, 0, 0);
+        }
+    }
+  }
+
+  // epilogue: lane holds D[row = 4*(lane/16)+v][col = lane%16] of each 16x16 block
+  cdd acc = cdd_zero();
+#pragma unroll
+  for (int j = 0; j < 2; ++j) {
+    const int p = ct * BN + wc + j * 16 + (lane & 15);
+    if (p >= a.N) continue;
+    const int igp = a.gpp_index ? gpp_igp(a.inv, a.indinv, p, a.ncouls) : ff_igp(a.inv, a.indinv, p, a.ncouls);
+    const double vc = a.vcoul[igp];
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int r = rt * BM + wr + i * 16 + 4 * (lane >> 4) + v;
+        if (r >= nrows) continue;
+        const int n1 = rows[r];
+        const double2 am = a.aqsm[(size_t)n1 * a.ncouls + igp];
+        const double2 w = cmul(a.wts[(size_t)job * a.max_rows + r], make_double2(vc * am.x, -vc * am.y));
+        const double2 t = cmul(w, make_double2(dr[i][j][v], di[i][j][v]));
+        acc = cdd_add(acc, t.x, t.y);
+      }
+    }
+  }
+  cdd tot = cdd_block_reduce(acc, red);
+  if (tid == 0) cdd_store(&a.partial[pid], tot);
+}
+
+}  // namespace
+
+int contract_launch(bgw_ctx* ctx, ContractArgs a, int njobs, int max_row_tiles, const char* tname) {
+  const int ntn = (a.N + BN - 1) / BN;
+  // K split: ~1024 G-vectors per workgroup.  Row counts per job are only known on
+  // the device (plan kernel), so the grid covers the largest possible row count;
+  // workgroups of empty row tiles exit after writing a zero partial.
+  int ksplit = a.K / 1024;
+  if (ksplit < 1) ksplit = 1;
+  if (ksplit > 64) ksplit = 64;
+  a.ksplit = ksplit;
+  dim3 grid(ntn * ksplit, max_row_tiles, njobs);
+  size_t W = (size_t)grid.x * grid.y * grid.z;
+  void* pp;
+  int rc = pool_get(ctx, "ws_contract", W * sizeof(double4), &pp);
+  if (rc) return rc;
+  a.partial = static_cast<double4*>(pp);
+  {
+    KernelTimer kt(ctx, tname);
+    if (a.Y)
+      hipLaunchKernelGGL(k_contract<kModeDiff>, grid, dim3(NT), 0, ctx->stream, a);
+    else if (a.thr2 > 0.0)
+      hipLaunchKernelGGL(k_contract<kModeMask>, grid, dim3(NT), 0, ctx->stream, a);
+    else
+      hipLaunchKernelGGL(k_contract<kModePlain>, grid, dim3(NT), 0, ctx->stream, a);
+  }
+  if ((rc = check_launch(ctx, "k_contract"))) return rc;
+  launch_reduce_cdd(static_cast<const double4*>(pp), (int)W, 1, a.out, ctx->stream);
+  return check_launch(ctx, "reduce");
+}
+
+int contract_achstemp(bgw_ctx* ctx, int nb, int ngpown, int ncouls, const int* inv, const int* indinv,
+                      const double2* aqsn, const double2* aqsm, const double2* Ieps, const double* vcoul,
+                      double2* achstemp) {
+  void *prow, *pw, *pcnt, *pslice;
+  int rc = pool_get(ctx, "ws_rows_gpp", (size_t)nb * sizeof(int), &prow);
+  if (!rc) rc = pool_get(ctx, "ws_wts_gpp", (size_t)nb * 16, &pw);
+  if (!rc) rc = pool_get(ctx, "ws_cnt_gpp", 64, &pcnt);
+  if (!rc) rc = pool_get(ctx, "ws_slice_gpp", 64, &pslice);
+  if (rc) return rc;
+  launch_identity_rows(static_cast<int*>(prow), static_cast<double2*>(pw), static_cast<int*>(pcnt),
+                       static_cast<int*>(pslice), nb, make_double2(-0.5, 0.0), 0, ctx->stream);
+  ContractArgs a{};
+  a.A = aqsn;
+  a.K = ncouls;
+  a.Xa = Ieps;
+  a.slice_stride = 0;
+  a.jobs_a = 1;
+  a.N = ngpown;
+  a.slice_of_job = static_cast<const int*>(pslice);
+  a.rows = static_cast<const int*>(prow);
+  a.wts = static_cast<const double2*>(pw);
+  a.cnt = static_cast<const int*>(pcnt);
+  a.max_rows = nb;
+  a.inv = inv;
+  a.indinv = indinv;
+  a.gpp_index = 1;
+  a.aqsm = aqsm;
+  a.vcoul = vcoul;
+  a.ncouls = ncouls;
+  a.thr2 = kTo1 * kTo1;  // |schs| > to1 cut of the static COH term
+  a.out = achstemp;
+  return contract_launch(ctx, a, 1, (nb + BM - 1) / BM, "contract_achs");
+}
+
+int contract_row_tile() { return BM; }
+
+}  // namespace bgw

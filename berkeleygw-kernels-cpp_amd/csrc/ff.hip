@@ -1,0 +1,313 @@
+// ff.hip -- FF full-frequency kernels (FF/ff.cpp:103-451) on the MFMA contraction.
+//
+// Each reference kernel is a sum over bands of an interpolated frequency slice
+// contracted with aqsntemp over ig and weighted by vcoul*conj(aqsm) over my_igp.
+// A single-thread plan kernel evaluates compute_fact per band on the device
+// (FF/ff.cpp:245-252, :337-342) and lists, for every frequency slice, the bands
+// that use it with their interpolation weight; the MFMA contraction then reads
+// each touched slice once for all bands of its list.  Because wx has no iw
+// dependence in the reference (FF/ff.cpp:245,337) each output is computed once
+// and written to all nfreqeval entries.
+#include <cstring>
+#include <string>
+
+#include "bgw_fact.h"
+#include "bgw_internal.h"
+#include "contract.h"
+#include "gpp_common.h"
+
+namespace bgw {
+namespace {
+
+// op 1 (asxDtemp): bands n1 < nvband, loop 1, jobs [0,nFreq) on I_epsR, [nFreq,2nFreq) on I_epsA.
+// op 2 (achDtemp_cor): all bands, loop 2, jobs [0,nFreq) on I_epsR - I_epsA;
+//   wx>0 & unoccupied -> schDttt_corKernel1 (weights fact1,fact2),
+//   wx<=0 & occupied  -> schDttt_corKernel2 (weights -1/2 fact1, -1/2 fact2).
+__global__ void k_ff_plan(int op, int nb, int nvband, int nFreq, double freqevalmin, double freqevalstep,
+                          double occ, const double* __restrict__ ekq, const double* __restrict__ grid,
+                          int* __restrict__ rows, double2* __restrict__ wts, int* __restrict__ cnt,
+                          int* __restrict__ slice_of_job) {
+  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+  const int njobs = op == 1 ? 2 * nFreq : nFreq;
+  for (int j = 0; j < njobs; ++j) {
+    cnt[j] = 0;
+    slice_of_job[j] = j % nFreq;
+  }
+  const int nbands = op == 1 ? nvband : nb;
+  for (int n1 = 0; n1 < nbands; ++n1) {
+    const double wx = freqevalmin - ekq[n1] + freqevalstep;
+    double f1 = 0.0, f2 = 0.0;
+    int ifreq = 0;
+    double scale;
+    int base = 0;
+    if (op == 1) {
+      compute_fact(wx, nFreq, grid, f1, f2, ifreq, 1, false);
+      scale = occ;
+      base = wx > 0 ? 0 : nFreq;
+    } else {
+      const bool occd = n1 < nvband;
+      compute_fact(wx, nFreq, grid, f1, f2, ifreq, 2, occd);
+      if (wx > 0 && !occd)
+        scale = 1.0;
+      else if (!(wx > 0) && occd)
+        scale = -0.5;
+      else
+        continue;
+    }
+    const int ja = base + ifreq, jb = base + ifreq + 1;
+    rows[(size_t)ja * nb + cnt[ja]] = n1;
+    wts[(size_t)ja * nb + cnt[ja]] = make_double2(scale * f1, 0.0);
+    cnt[ja] += 1;
+    rows[(size_t)jb * nb + cnt[jb]] = n1;
+    wts[(size_t)jb * nb + cnt[jb]] = make_double2(scale * f2, 0.0);
+    cnt[jb] += 1;
+  }
+}
+
+}  // namespace
+
+int ff_device(bgw_ctx* ctx, int op, int nb, int nvband, int nfreqeval, int ncouls, int ngpown, int nFreq,
+              double freqevalmin, double freqevalstep, double occ, const double* ekq, const double* dFreqGrid,
+              const int* inv, const int* indinv, const double* aqsm, const double* aqsn, const double* vcoul,
+              const double* I_epsR, const double* I_epsA, double* out) {
+  hipStream_t s = ctx->stream;
+  launch_check_indices(inv, (size_t)ngpown, indinv, (size_t)ncouls, ncouls - 1, ncouls, ctx->d_flag, s);
+  ContractArgs a{};
+  a.A = reinterpret_cast<const double2*>(aqsn);
+  a.K = ncouls;
+  a.N = ngpown;
+  a.slice_stride = (long)ngpown * ncouls;
+  a.inv = inv;
+  a.indinv = indinv;
+  a.gpp_index = 0;
+  a.aqsm = reinterpret_cast<const double2*>(aqsm);
+  a.vcoul = vcoul;
+  a.ncouls = ncouls;
+  a.thr2 = 0.0;
+  a.out = reinterpret_cast<double2*>(out);
+  const int BMr = contract_row_tile();
+  void *prow, *pw, *pcnt, *pslice;
+  const int njobs = op == 0 ? 1 : (op == 1 ? 2 * nFreq : nFreq);
+  const int maxrows = op == 0 ? nb : (op == 1 ? nvband : nb);
+  int rc = pool_get(ctx, "ws_ff_rows", (size_t)njobs * nb * sizeof(int), &prow);
+  if (!rc) rc = pool_get(ctx, "ws_ff_wts", (size_t)njobs * nb * 16, &pw);
+  if (!rc) rc = pool_get(ctx, "ws_ff_cnt", (size_t)njobs * sizeof(int), &pcnt);
+  if (!rc) rc = pool_get(ctx, "ws_ff_slice", (size_t)njobs * sizeof(int), &pslice);
+  if (rc) return rc;
+  a.rows = static_cast<const int*>(prow);
+  a.wts = static_cast<const double2*>(pw);
+  a.cnt = static_cast<const int*>(pcnt);
+  a.slice_of_job = static_cast<const int*>(pslice);
+  a.max_rows = nb;
+  if (op == 0) {
+    // achsDtemp: slice 1 of I_epsR (FF/ff.cpp:200), every band, weight -1/2
+    launch_identity_rows(static_cast<int*>(prow), static_cast<double2*>(pw), static_cast<int*>(pcnt),
+                         static_cast<int*>(pslice), nb, make_double2(-0.5, 0.0), 1, s);
+    a.Xa = reinterpret_cast<const double2*>(I_epsR);
+    a.jobs_a = 1;
+    return contract_launch(ctx, a, 1, (nb + BMr - 1) / BMr, "contract_ff_achs");
+  }
+  hipLaunchKernelGGL(k_ff_plan, dim3(1), dim3(64), 0, s, op, nb, nvband, nFreq, freqevalmin, freqevalstep, occ,
+                     ekq, dFreqGrid, static_cast<int*>(prow), static_cast<double2*>(pw), static_cast<int*>(pcnt),
+                     static_cast<int*>(pslice));
+  if ((rc = check_launch(ctx, "k_ff_plan"))) return rc;
+  if (op == 1) {
+    a.Xa = reinterpret_cast<const double2*>(I_epsR);
+    a.Xb = reinterpret_cast<const double2*>(I_epsA);
+    a.jobs_a = nFreq;
+    rc = contract_launch(ctx, a, njobs, (maxrows + BMr - 1) / BMr, "contract_ff_asx");
+  } else {
+    a.Xa = reinterpret_cast<const double2*>(I_epsR);
+    a.Y = reinterpret_cast<const double2*>(I_epsA);
+    a.jobs_a = nFreq;
+    rc = contract_launch(ctx, a, njobs, (maxrows + BMr - 1) / BMr, "contract_ff_cor");
+  }
+  if (rc) return rc;
+  launch_replicate(a.out, nfreqeval, s);
+  return check_launch(ctx, "replicate");
+}
+
+}  // namespace bgw
+
+// ===================================================================== C ABI
+using namespace bgw;
+
+namespace {
+
+int ff_validate(bgw_ctx* ctx, int op, int nb, int nvband, int nfreqeval, int ncouls, int ngpown, int nFreq,
+                std::initializer_list<const void*> ptrs) {
+  if (!ctx) return BGW_EINVAL;
+  for (const void* p : ptrs)
+    if (!p) return fail(ctx, BGW_EINVAL, "null array argument");
+  if (nb <= 0 || ncouls <= 0 || ngpown <= 0) return fail(ctx, BGW_EINVAL, "zero or negative size");
+  if (ngpown > ncouls) return fail(ctx, BGW_EINVAL, "ngpown cannot be greater than ncouls");  // FF/ff.cpp:492-499
+  if (op != 0 && (nFreq < 2 || nfreqeval <= 0)) return fail(ctx, BGW_EINVAL, "nFreq < 2 or nfreqeval <= 0");
+  if (op != 0 && (nvband < 0 || nvband > nb)) return fail(ctx, BGW_EINVAL, "nvband out of [0, number_bands]");
+  size_t t;
+  if (!mul_ok((size_t)ngpown, (size_t)ncouls, &t) || !mul_ok(t, (size_t)(nFreq > 2 ? nFreq : 2), &t) ||
+      t > SIZE_MAX / 16)
+    return fail(ctx, BGW_EINVAL, "size overflow");
+  if (hipSetDevice(ctx->device) != hipSuccess) return fail(ctx, BGW_ENODEV, "hipSetDevice");
+  return BGW_OK;
+}
+
+int ff_host_indices(bgw_ctx* ctx, int ngpown, int ncouls, const int* inv, const int* indinv) {
+  for (int p = 0; p < ngpown; ++p)
+    if (inv[p] < 0 || inv[p] >= ncouls) return fail(ctx, BGW_EINVAL, "inv_igp_index out of [0,ncouls)");
+  for (int g = 0; g < ncouls; ++g)
+    if (indinv[g] < 0 || indinv[g] >= ncouls) return fail(ctx, BGW_EINVAL, "indinv out of [0,ncouls)");
+  return BGW_OK;
+}
+
+template <typename T>
+int upload(bgw_ctx* ctx, const char* slot, const T* src, size_t count, const T** dst) {
+  void* d;
+  int rc = pool_get(ctx, slot, count * sizeof(T), &d);
+  if (rc) return rc;
+  hipError_t e = hipMemcpyAsync(d, src, count * sizeof(T), hipMemcpyHostToDevice, ctx->stream);
+  if (e != hipSuccess) return hip_fail(ctx, e, "upload");
+  *dst = static_cast<const T*>(d);
+  return BGW_OK;
+}
+
+int ff_host(bgw_ctx* ctx, int op, int nb, int nvband, int nfreqeval, int ncouls, int ngpown, int nFreq,
+            double fmin, double fstep, double occ, const double* ekq, const double* grid, const int* inv,
+            const int* indinv, const double* aqsm, const double* aqsn, const double* vcoul,
+            const double* R, const double* A, double* out) {
+  int rc = ff_host_indices(ctx, ngpown, ncouls, inv, indinv);
+  if (rc) return rc;
+  const size_t nbg = (size_t)nb * ncouls, npg = (size_t)ngpown * ncouls;
+  const int *d_inv, *d_indinv;
+  const double *d_ekq = nullptr, *d_grid = nullptr, *d_am, *d_an, *d_vc, *d_R = nullptr, *d_A = nullptr;
+  if ((rc = upload(ctx, "h_inv", inv, (size_t)ngpown, &d_inv))) return rc;
+  if ((rc = upload(ctx, "h_indinv", indinv, (size_t)ncouls, &d_indinv))) return rc;
+  if ((rc = upload(ctx, "h_aqsm", aqsm, 2 * nbg, &d_am))) return rc;
+  if ((rc = upload(ctx, "h_aqsn", aqsn, 2 * nbg, &d_an))) return rc;
+  if ((rc = upload(ctx, "h_vcoul", vcoul, (size_t)ncouls, &d_vc))) return rc;
+  if (op == 0) {
+    // only slice 1 of I_epsR is read (FF/ff.cpp:200): stage it as slice 1 of a 2-slice array
+    void* d;
+    if ((rc = pool_get(ctx, "h_epsR", 2 * npg * 16, &d))) return rc;
+    hipError_t e = hipMemcpyAsync(static_cast<char*>(d) + npg * 16, R + 2 * npg, npg * 16,
+                                  hipMemcpyHostToDevice, ctx->stream);
+    if (e != hipSuccess) return hip_fail(ctx, e, "upload");
+    d_R = static_cast<const double*>(d);
+  } else {
+    if ((rc = upload(ctx, "h_ekq", ekq, (size_t)(op == 1 ? nvband : nb), &d_ekq))) return rc;
+    if ((rc = upload(ctx, "h_grid", grid, (size_t)nFreq, &d_grid))) return rc;
+    if ((rc = upload(ctx, "h_epsR", R, 2 * npg * nFreq, &d_R))) return rc;
+    if ((rc = upload(ctx, "h_epsA", A, 2 * npg * nFreq, &d_A))) return rc;
+  }
+  void* d_out;
+  if ((rc = pool_get(ctx, "h_ffout", (size_t)(nfreqeval > 1 ? nfreqeval : 1) * 16, &d_out))) return rc;
+  rc = ff_device(ctx, op, nb, nvband, nfreqeval, ncouls, ngpown, nFreq, fmin, fstep, occ, d_ekq, d_grid,
+                 d_inv, d_indinv, d_am, d_an, d_vc, d_R, d_A, static_cast<double*>(d_out));
+  if (rc) {
+    (void)hipStreamSynchronize(ctx->stream);
+    return rc;
+  }
+  const size_t nout = op == 0 ? 1 : (size_t)nfreqeval;
+  hipError_t e = hipMemcpyAsync(out, d_out, nout * 16, hipMemcpyDeviceToHost, ctx->stream);
+  if (e != hipSuccess) return hip_fail(ctx, e, "download");
+  return bgw_synchronize(ctx);
+}
+
+}  // namespace
+
+extern "C" {
+
+int bgw_compute_fact(double wx, int nFreq, const double* dFreqGrid, double* fact1, double* fact2, int* ifreq,
+                     int loop, int flag_occ) {
+  if (!dFreqGrid || !fact1 || !fact2 || !ifreq || nFreq < 2) return BGW_EINVAL;
+  compute_fact(wx, nFreq, dFreqGrid, *fact1, *fact2, *ifreq, loop, flag_occ != 0);
+  return BGW_OK;
+}
+
+int bgw_achsDtemp_Kernel(bgw_ctx* ctx, int number_bands, int ngpown, int ncouls, const int* inv_igp_index,
+                         const int* indinv, const double* aqsntemp, const double* aqsmtemp,
+                         const double* I_epsR_array, const double* vcoul, double* achsDtemp) {
+  int rc = ff_validate(ctx, 0, number_bands, 0, 1, ncouls, ngpown, 2,
+                       {inv_igp_index, indinv, aqsntemp, aqsmtemp, I_epsR_array, vcoul, achsDtemp});
+  if (rc) return rc;
+  return ff_host(ctx, 0, number_bands, 0, 1, ncouls, ngpown, 2, 0.0, 0.0, 0.0, nullptr, nullptr, inv_igp_index,
+                 indinv, aqsmtemp, aqsntemp, vcoul, I_epsR_array, nullptr, achsDtemp);
+}
+
+int bgw_asxDtemp_Kernel(bgw_ctx* ctx, int nvband, int nfreqeval, int ncouls, int ngpown, int nFreq,
+                        double freqevalmin, double freqevalstep, double occ, const double* ekq,
+                        const double* dFreqGrid, const int* inv_igp_index, const int* indinv,
+                        const double* aqsmtemp, const double* aqsntemp, const double* vcoul,
+                        const double* I_epsR_array, const double* I_epsA_array, double* asxDtemp) {
+  // asxDtemp_Kernel reads bands n1 < nvband only (FF/ff.cpp:242): the band arrays
+  // passed here need nvband rows.
+  int rc = ff_validate(ctx, 1, nvband > 0 ? nvband : 1, nvband, nfreqeval, ncouls, ngpown, nFreq,
+                       {ekq, dFreqGrid, inv_igp_index, indinv, aqsmtemp, aqsntemp, vcoul, I_epsR_array,
+                        I_epsA_array, asxDtemp});
+  if (rc) return rc;
+  if (nvband == 0) {
+    std::memset(asxDtemp, 0, sizeof(double) * 2 * nfreqeval);
+    return BGW_OK;
+  }
+  return ff_host(ctx, 1, nvband, nvband, nfreqeval, ncouls, ngpown, nFreq, freqevalmin, freqevalstep, occ, ekq,
+                 dFreqGrid, inv_igp_index, indinv, aqsmtemp, aqsntemp, vcoul, I_epsR_array, I_epsA_array,
+                 asxDtemp);
+}
+
+int bgw_achDtemp_cor_Kernel(bgw_ctx* ctx, int number_bands, int nvband, int nfreqeval, int ncouls, int ngpown,
+                            int nFreq, double freqevalmin, double freqevalstep, const double* ekq,
+                            const double* dFreqGrid, const int* inv_igp_index, const int* indinv,
+                            const double* aqsmtemp, const double* aqsntemp, const double* vcoul,
+                            const double* I_epsR_array, const double* I_epsA_array, double* achDtemp_cor) {
+  int rc = ff_validate(ctx, 2, number_bands, nvband, nfreqeval, ncouls, ngpown, nFreq,
+                       {ekq, dFreqGrid, inv_igp_index, indinv, aqsmtemp, aqsntemp, vcoul, I_epsR_array,
+                        I_epsA_array, achDtemp_cor});
+  if (rc) return rc;
+  return ff_host(ctx, 2, number_bands, nvband, nfreqeval, ncouls, ngpown, nFreq, freqevalmin, freqevalstep, 0.0,
+                 ekq, dFreqGrid, inv_igp_index, indinv, aqsmtemp, aqsntemp, vcoul, I_epsR_array, I_epsA_array,
+                 achDtemp_cor);
+}
+
+int bgw_achsDtemp_Kernel_device(bgw_ctx* ctx, int number_bands, int ngpown, int ncouls, const int* inv_igp_index,
+                                const int* indinv, const double* aqsntemp, const double* aqsmtemp,
+                                const double* I_epsR_array, const double* vcoul, double* achsDtemp) {
+  int rc = ff_validate(ctx, 0, number_bands, 0, 1, ncouls, ngpown, 2,
+                       {inv_igp_index, indinv, aqsntemp, aqsmtemp, I_epsR_array, vcoul, achsDtemp});
+  if (rc) return rc;
+  return ff_device(ctx, 0, number_bands, 0, 1, ncouls, ngpown, 2, 0.0, 0.0, 0.0, nullptr, nullptr, inv_igp_index,
+                   indinv, aqsmtemp, aqsntemp, vcoul, I_epsR_array, nullptr, achsDtemp);
+}
+
+int bgw_asxDtemp_Kernel_device(bgw_ctx* ctx, int nvband, int nfreqeval, int ncouls, int ngpown, int nFreq,
+                               double freqevalmin, double freqevalstep, double occ, const double* ekq,
+                               const double* dFreqGrid, const int* inv_igp_index, const int* indinv,
+                               const double* aqsmtemp, const double* aqsntemp, const double* vcoul,
+                               const double* I_epsR_array, const double* I_epsA_array, double* asxDtemp) {
+  int rc = ff_validate(ctx, 1, nvband > 0 ? nvband : 1, nvband, nfreqeval, ncouls, ngpown, nFreq,
+                       {ekq, dFreqGrid, inv_igp_index, indinv, aqsmtemp, aqsntemp, vcoul, I_epsR_array,
+                        I_epsA_array, asxDtemp});
+  if (rc) return rc;
+  if (nvband == 0) {
+    fill_zero(asxDtemp, sizeof(double) * 2 * nfreqeval, ctx->stream);
+    return check_launch(ctx, "fill");
+  }
+  return ff_device(ctx, 1, nvband, nvband, nfreqeval, ncouls, ngpown, nFreq, freqevalmin, freqevalstep, occ, ekq,
+                   dFreqGrid, inv_igp_index, indinv, aqsmtemp, aqsntemp, vcoul, I_epsR_array, I_epsA_array,
+                   asxDtemp);
+}
+
+int bgw_achDtemp_cor_Kernel_device(bgw_ctx* ctx, int number_bands, int nvband, int nfreqeval, int ncouls,
+                                   int ngpown, int nFreq, double freqevalmin, double freqevalstep,
+                                   const double* ekq, const double* dFreqGrid, const int* inv_igp_index,
+                                   const int* indinv, const double* aqsmtemp, const double* aqsntemp,
+                                   const double* vcoul, const double* I_epsR_array, const double* I_epsA_array,
+                                   double* achDtemp_cor) {
+  int rc = ff_validate(ctx, 2, number_bands, nvband, nfreqeval, ncouls, ngpown, nFreq,
+                       {ekq, dFreqGrid, inv_igp_index, indinv, aqsmtemp, aqsntemp, vcoul, I_epsR_array,
+                        I_epsA_array, achDtemp_cor});
+  if (rc) return rc;
+  return ff_device(ctx, 2, number_bands, nvband, nfreqeval, ncouls, ngpown, nFreq, freqevalmin, freqevalstep, 0.0,
+                   ekq, dFreqGrid, inv_igp_index, indinv, aqsmtemp, aqsntemp, vcoul, I_epsR_array, I_epsA_array,
+                   achDtemp_cor);
+}
+
+}  // extern "C"

@@ -1,0 +1,453 @@
+// gpp.hip -- GPP self-energy forward on gfx950 (CDNA4).
+//
+// Reference: noflagOCC_solver, GPP/gpp.cpp:307-402.  The quadruple loop
+// n1 x my_igp x ig x iw (:339-391) is mapped as
+//   * lanes   = bands n1 (one or two bands per lane), so the band factor
+//               sch_store1(n1,p) (:369-371) leaves the G-vector sum and is applied
+//               once per (n1, p, G-chunk) instead of once per term;
+//   * workgroup = one (p, G-chunk) work item at a time; the (p, g) data of a
+//               G-tile is read once from HBM, coalesced along ig, turned into the
+//               per-g constants of the pole term and staged in LDS, from where all
+//               band-waves read it as a broadcast;
+//   * iw (3 frequencies) unrolled in registers, the three reciprocals of a
+//               (p,g,n1) triple formed from one v_rcp_f64 (batched inversion).
+// Partial sums stay in registers per lane and per work item, then go to
+// per-workgroup partials [W][nb][3]; a fixed-order double-double pass produces
+// acht_n1_loc and achtemp (run-to-run bitwise reproducible).
+#include <cmath>
+
+#include "bgw_dd.h"
+#include "bgw_internal.h"
+#include "contract.h"
+#include "gpp_common.h"
+
+namespace bgw {
+namespace {
+
+// ------------------------------------------------------------------ prep kernels
+// S[p][n1] = conj(aqsm(n1,igp))*aqsn(n1,igp)*0.5*vcoul(igp)*wtilde(p,igp)  (GPP/gpp.cpp:369-371)
+// wxb[n1][iw] = wx(n1,iw) (per-band form) or wx[iw] (reference form, GPP/gpp.cpp:374-375).
+__global__ void k_gpp_prep(const int* __restrict__ inv, const int* __restrict__ indinv,
+                           const double2* __restrict__ wt, const double2* __restrict__ aqsm,
+                           const double2* __restrict__ aqsn, const double* __restrict__ vcoul,
+                           const double* __restrict__ wx, int wx_per_band, int nb, int ngpown,
+                           int ncouls, double2* __restrict__ S, double2* __restrict__ C,
+                           double* __restrict__ wxb) {
+  const int n1 = blockIdx.x * blockDim.x + threadIdx.x;
+  const int p = blockIdx.y;
+  if (n1 >= nb) return;
+  const int igp = gpp_igp(inv, indinv, p, ncouls);
+  const double2 am = aqsm[(size_t)n1 * ncouls + igp];
+  const double2 an = aqsn[(size_t)n1 * ncouls + igp];
+  const double vc = vcoul[igp];
+  const double2 w = wt[(size_t)p * ncouls + igp];
+  // conj(am) * an, then * 0.5, * vcoul, * wtilde(p,igp), in the reference's order
+  double2 t = cmul(make_double2(am.x, -am.y), an);
+  t.x *= 0.5;
+  t.y *= 0.5;
+  t.x *= vc;
+  t.y *= vc;
+  C[(size_t)p * nb + n1] = t;
+  S[(size_t)p * nb + n1] = cmul(t, w);
+  if (p == 0) {
+#pragma unroll
+    for (int iw = 0; iw < 3; ++iw) wxb[n1 * 3 + iw] = wx_per_band ? wx[n1 * 3 + iw] : wx[iw];
+  }
+}
+
+// [min, max] of all wx(n1,iw): bounds every pole denominator of a G-tile at once.
+__global__ void k_wx_range(const double* __restrict__ wxb, int n, double* __restrict__ range) {
+  __shared__ double smin[256], smax[256];
+  double lo = INFINITY, hi = -INFINITY;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) {
+    lo = fmin(lo, wxb[i]);
+    hi = fmax(hi, wxb[i]);
+  }
+  smin[threadIdx.x] = lo;
+  smax[threadIdx.x] = hi;
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) {
+      smin[threadIdx.x] = fmin(smin[threadIdx.x], smin[threadIdx.x + s]);
+      smax[threadIdx.x] = fmax(smax[threadIdx.x], smax[threadIdx.x + s]);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    // NaN anywhere in wx makes the range test fail, so every tile takes the
+    // reference-arithmetic path and propagates the NaN like the reference.
+    bool bad = false;
+    for (int i = 0; i < n; ++i) bad |= !(wxb[i] == wxb[i]);
+    range[0] = bad ? NAN : smin[0];
+    range[1] = bad ? NAN : smax[0];
+  }
+}
+
+// ------------------------------------------------------------------ main kernel
+// One LDS entry per G-vector of a tile: the per-(p,g) constants of the term
+//   delw*I = (u + i wi)(Ir + i Ii)/d,  u = wx - wr,  d = u^2 + wi^2
+//          = (fma(Ir,u,a) + i fma(Ii,u,b)) / d,  a = -wi*Ii,  b = wi*Ir.
+struct __align__(16) TileEntry {
+  double wr, wi2, Ir, Ii, a, b;
+};
+
+// Bounds of the pole denominators over all bands: every d in [2^-300, 2^300]
+// keeps the batched product d0*d1*d2 and its reciprocal normal.
+constexpr double kDMin = 0x1p-300;
+constexpr double kDMax = 0x1p+300;
+
+__device__ inline TileEntry make_entry(double2 w, double2 I, double wxmin, double wxmax, bool& ok) {
+  TileEntry e;
+  e.wr = w.x;
+  e.wi2 = w.y * w.y;
+  e.Ir = I.x;
+  e.Ii = I.y;
+  e.a = -w.y * I.y;
+  e.b = w.y * I.x;
+  const double lo = w.x < wxmin ? wxmin - w.x : (w.x > wxmax ? w.x - wxmax : 0.0);
+  const double hi = fmax(fabs(wxmin - w.x), fabs(wxmax - w.x));
+  const double dlo = fma(lo, lo, e.wi2), dhi = fma(hi, hi, e.wi2);
+  ok = (dlo >= kDMin) && (dhi <= kDMax);  // false for NaN
+  return e;
+}
+
+// Fast path: three reciprocals from one v_rcp_f64 plus one Newton step on the
+// product (batched inversion); all in FP64.
+template <int BPL>
+__device__ inline void tile_fast(const TileEntry* __restrict__ tile, int n, const double (&x)[BPL][3],
+                                 double (&acc)[BPL][3][2]) {
+#pragma unroll 2
+  for (int j = 0; j < n; ++j) {
+    const TileEntry e = tile[j];
+#pragma unroll
+    for (int bb = 0; bb < BPL; ++bb) {
+      const double u0 = x[bb][0] - e.wr, u1 = x[bb][1] - e.wr, u2 = x[bb][2] - e.wr;
+      const double d0 = fma(u0, u0, e.wi2), d1 = fma(u1, u1, e.wi2), d2 = fma(u2, u2, e.wi2);
+      const double m01 = d0 * d1;
+      const double m = m01 * d2;
+      double r = __builtin_amdgcn_rcp(m);
+      double t = fma(-m, r, 1.0);
+      r = fma(r, t, r);
+      t = fma(-m, r, 1.0);
+      r = fma(r, t, r);
+      const double r2 = r * m01;
+      const double q = r * d2;
+      const double r0 = q * d1, r1 = q * d0;
+      acc[bb][0][0] = fma(fma(e.Ir, u0, e.a), r0, acc[bb][0][0]);
+      acc[bb][0][1] = fma(fma(e.Ii, u0, e.b), r0, acc[bb][0][1]);
+      acc[bb][1][0] = fma(fma(e.Ir, u1, e.a), r1, acc[bb][1][0]);
+      acc[bb][1][1] = fma(fma(e.Ii, u1, e.b), r1, acc[bb][1][1]);
+      acc[bb][2][0] = fma(fma(e.Ir, u2, e.a), r2, acc[bb][2][0]);
+      acc[bb][2][1] = fma(fma(e.Ii, u2, e.b), r2, acc[bb][2][1]);
+    }
+  }
+}
+
+// Reference-arithmetic path (GPP/gpp.cpp:374-383): IEEE division per term.  Used
+// for a tile whose denominators are not all safely inside [2^-300, 2^300]
+// (poles, zeros, NaN/Inf), so such inputs give what the reference gives.
+template <int BPL>
+__device__ inline void tile_exact(const TileEntry* __restrict__ tile, int n, const double (&x)[BPL][3],
+                                  double (&acc)[BPL][3][2]) {
+  for (int j = 0; j < n; ++j) {
+    const TileEntry e = tile[j];
+#pragma unroll
+    for (int bb = 0; bb < BPL; ++bb) {
+#pragma unroll
+      for (int iw = 0; iw < 3; ++iw) {
+        const double u = x[bb][iw] - e.wr;
+        const double d = u * u + e.wi2;
+        const double rden = 1.0 / d;
+        // delw*I with delw = (u, wi_signed)*rden; a = -wi*Ii, b = wi*Ir carry the sign
+        const double pr = e.Ir * u + e.a;
+        const double pi = e.Ii * u + e.b;
+        acc[bb][iw][0] += pr * rden;
+        acc[bb][iw][1] += pi * rden;
+      }
+    }
+  }
+}
+
+template <int NT, int BPL>
+__global__ __launch_bounds__(NT) void k_gpp_main(const double2* __restrict__ wt,
+                                                 const double2* __restrict__ Ieps,
+                                                 const double2* __restrict__ S,
+                                                 const double* __restrict__ wxb,
+                                                 const double* __restrict__ wxrange, int nb,
+                                                 int ngpown, int ncouls, int chunk, int nchunks,
+                                                 double2* __restrict__ partial) {
+  __shared__ TileEntry tile[NT];
+  const int tid = threadIdx.x;
+  const int band0 = blockIdx.y * NT * BPL;
+  int n1[BPL];
+  double x[BPL][3];
+  double tot[BPL][3][2];
+#pragma unroll
+  for (int bb = 0; bb < BPL; ++bb) {
+    n1[bb] = band0 + tid + bb * NT;
+    const int nc = n1[bb] < nb ? n1[bb] : nb - 1;  // idle lanes mirror the last band
+#pragma unroll
+    for (int iw = 0; iw < 3; ++iw) {
+      x[bb][iw] = wxb[nc * 3 + iw];
+      tot[bb][iw][0] = tot[bb][iw][1] = 0.0;
+    }
+  }
+  const double wxmin = wxrange[0], wxmax = wxrange[1];
+  const long nitems = (long)ngpown * nchunks;
+
+  for (long item = blockIdx.x; item < nitems; item += gridDim.x) {
+    const int p = (int)(item / nchunks);
+    const int c = (int)(item - (long)p * nchunks);
+    const int gb = c * chunk;
+    const int ge = min(ncouls, gb + chunk);
+    double acc[BPL][3][2];
+#pragma unroll
+    for (int bb = 0; bb < BPL; ++bb)
+#pragma unroll
+      for (int iw = 0; iw < 3; ++iw) acc[bb][iw][0] = acc[bb][iw][1] = 0.0;
+
+    const size_t rowoff = (size_t)p * ncouls;
+    for (int g0 = gb; g0 < ge; g0 += NT) {
+      const int g = g0 + tid;
+      bool ok;
+      TileEntry e;
+      if (g < ge) {
+        e = make_entry(wt[rowoff + g], Ieps[rowoff + g], wxmin, wxmax, ok);
+      } else {  // padding: contributes exactly zero, denominators wx^2+1
+        e = make_entry(make_double2(0.0, 1.0), make_double2(0.0, 0.0), wxmin, wxmax, ok);
+      }
+      __syncthreads();  // previous tile fully consumed
+      tile[tid] = e;
+      const int unsafe = __syncthreads_or(!ok);
+      const int n = min(NT, ge - g0);
+      if (!unsafe)
+        tile_fast<BPL>(tile, n, x, acc);
+      else
+        tile_exact<BPL>(tile, n, x, acc);
+    }
+    // fold the G-chunk sums with sch_store1(n1,p)
+#pragma unroll
+    for (int bb = 0; bb < BPL; ++bb) {
+      if (n1[bb] < nb) {
+        const double2 s = S[(size_t)p * nb + n1[bb]];
+#pragma unroll
+        for (int iw = 0; iw < 3; ++iw) {
+          tot[bb][iw][0] = fma(s.x, acc[bb][iw][0], fma(-s.y, acc[bb][iw][1], tot[bb][iw][0]));
+          tot[bb][iw][1] = fma(s.x, acc[bb][iw][1], fma(s.y, acc[bb][iw][0], tot[bb][iw][1]));
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int bb = 0; bb < BPL; ++bb) {
+    if (n1[bb] < nb) {
+      double2* dst = partial + (size_t)blockIdx.x * nb * 3 + (size_t)n1[bb] * 3;
+#pragma unroll
+      for (int iw = 0; iw < 3; ++iw) dst[iw] = make_double2(tot[bb][iw][0], tot[bb][iw][1]);
+    }
+  }
+}
+
+// ------------------------------------------------------------------ reductions
+// acht_n1_loc[n1][iw] = sum_w partial[w][n1][iw] in fixed w order (double-double).
+__global__ void k_reduce_bands(const double2* __restrict__ partial, int W, int nb,
+                               double2* __restrict__ acht_n1_loc, double4* __restrict__ band_dd) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nb * 3) return;
+  cdd a = cdd_zero();
+  for (int w = 0; w < W; ++w) {
+    const double2 v = partial[(size_t)w * nb * 3 + k];
+    a = cdd_add(a, v.x, v.y);
+  }
+  if (acht_n1_loc) acht_n1_loc[k] = make_double2(a.re.hi + a.re.lo, a.im.hi + a.im.lo);
+  cdd_store(&band_dd[k], a);
+}
+
+// achtemp[iw] = sum_n1 band_dd[n1][iw], fixed order.
+__global__ void k_reduce_achtemp(const double4* __restrict__ band_dd, int nb,
+                                 double2* __restrict__ achtemp) {
+  __shared__ cdd sh[16];
+  for (int iw = 0; iw < 3; ++iw) {
+    cdd a = cdd_zero();
+    for (int n1 = threadIdx.x; n1 < nb; n1 += blockDim.x) a = cdd_add(a, cdd_load(&band_dd[n1 * 3 + iw]));
+    cdd r = cdd_block_reduce(a, sh);
+    if (threadIdx.x == 0) achtemp[iw] = make_double2(r.re.hi + r.re.lo, r.im.hi + r.im.lo);
+    __syncthreads();
+  }
+}
+
+// ------------------------------------------------------------- screened exchange
+// asxtemp[iw] = sum_{n1<nvband,p} occ vcoul(igp) sum_g aqsn(n1,g) conj(aqsm(n1,igp)) ssx(n1,p,g,iw)
+template <int NT>
+__global__ __launch_bounds__(NT) void k_gpp_asx(const int* __restrict__ inv, const int* __restrict__ indinv,
+                                                const double2* __restrict__ wt,
+                                                const double2* __restrict__ Ieps,
+                                                const double2* __restrict__ aqsm,
+                                                const double2* __restrict__ aqsn,
+                                                const double* __restrict__ vcoul,
+                                                const double* __restrict__ wxb, int nvband, int ngpown,
+                                                int ncouls, int chunk, int nchunks, double occ,
+                                                double4* __restrict__ partial /*[W][3]*/) {
+  __shared__ cdd sh[NT / 64];
+  cdd tot[3] = {cdd_zero(), cdd_zero(), cdd_zero()};
+  const long nitems = (long)ngpown * nchunks;
+  for (long item = blockIdx.x; item < nitems; item += gridDim.x) {
+    const int p = (int)(item / nchunks);
+    const int c = (int)(item - (long)p * nchunks);
+    const int gb = c * chunk, ge = min(ncouls, gb + chunk);
+    const int igp = gpp_igp(inv, indinv, p, ncouls);
+    const double vc = vcoul[igp];
+    for (int n1 = 0; n1 < nvband; ++n1) {
+      const double2 am = aqsm[(size_t)n1 * ncouls + igp];
+      // m = conj(aqsm) * occ * vcoul
+      const double2 m = make_double2(am.x * occ * vc, -am.y * occ * vc);
+      double s[3][2] = {{0, 0}, {0, 0}, {0, 0}};
+      for (int g = gb + threadIdx.x; g < ge; g += NT) {
+        const double2 w = wt[(size_t)p * ncouls + g];
+        const double2 I = Ieps[(size_t)p * ncouls + g];
+        const double2 an = aqsn[(size_t)n1 * ncouls + g];
+#pragma unroll
+        for (int iw = 0; iw < 3; ++iw) {
+          const double2 ss = gpp_ssx(wxb[n1 * 3 + iw], w, I);
+          const double2 t = cmul(an, ss);
+          s[iw][0] += t.x;
+          s[iw][1] += t.y;
+        }
+      }
+#pragma unroll
+      for (int iw = 0; iw < 3; ++iw) {
+        const double2 t = cmul(m, make_double2(s[iw][0], s[iw][1]));
+        tot[iw] = cdd_add(tot[iw], t.x, t.y);
+      }
+    }
+  }
+#pragma unroll
+  for (int iw = 0; iw < 3; ++iw) {
+    cdd r = cdd_block_reduce(tot[iw], sh);
+    if (threadIdx.x == 0) cdd_store(&partial[(size_t)blockIdx.x * 3 + iw], r);
+    __syncthreads();
+  }
+}
+
+template <int NT, int BPL>
+int launch_main(bgw_ctx* ctx, const double2* wt, const double2* I, const double2* S,
+                const double* wxb, const double* range, int nb, int ngpown, int ncouls,
+                double2* partial_ws, int* W_out, int chunk) {
+  const int nchunks = (ncouls + chunk - 1) / chunk;
+  const int btiles = (nb + NT * BPL - 1) / (NT * BPL);
+  int per_cu = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_gpp_main<NT, BPL>, NT, 0) != hipSuccess ||
+      per_cu < 1)
+    per_cu = 1;
+  long items = (long)ngpown * nchunks;
+  long W = (long)ctx->cus * per_cu / btiles;
+  if (W < 1) W = 1;
+  if (W > items) W = items;
+  *W_out = (int)W;
+  if (!partial_ws) return BGW_OK;  // sizing query
+  KernelTimer kt(ctx, "gpp_main");
+  hipLaunchKernelGGL((k_gpp_main<NT, BPL>), dim3((unsigned)W, btiles), dim3(NT), 0, ctx->stream, wt, I, S,
+                     wxb, range, nb, ngpown, ncouls, chunk, nchunks, partial_ws);
+  return check_launch(ctx, "k_gpp_main");
+}
+
+struct MainCfg {
+  int nt, bpl;
+};
+
+MainCfg pick_main(int nb) {
+  if (nb > 256) return {256, 2};
+  if (nb > 128) return {128, 2};
+  if (nb > 64) return {128, 1};
+  return {64, 1};
+}
+
+int run_main(bgw_ctx* ctx, MainCfg cfg, const double2* wt, const double2* I, const double2* S,
+             const double* wxb, const double* range, int nb, int ngpown, int ncouls, double2* ws,
+             int* W) {
+  const int chunk = 2048;
+  if (cfg.nt == 256 && cfg.bpl == 2)
+    return launch_main<256, 2>(ctx, wt, I, S, wxb, range, nb, ngpown, ncouls, ws, W, chunk);
+  if (cfg.nt == 128 && cfg.bpl == 2)
+    return launch_main<128, 2>(ctx, wt, I, S, wxb, range, nb, ngpown, ncouls, ws, W, chunk);
+  if (cfg.nt == 128)
+    return launch_main<128, 1>(ctx, wt, I, S, wxb, range, nb, ngpown, ncouls, ws, W, chunk);
+  return launch_main<64, 1>(ctx, wt, I, S, wxb, range, nb, ngpown, ncouls, ws, W, chunk);
+}
+
+}  // namespace
+
+// Forward driver on device pointers; enqueues on ctx->stream.
+int gpp_forward_device(bgw_ctx* ctx, size_t nb_, size_t nvband_, size_t ngpown_, size_t ncouls_,
+                       const int* inv, const int* indinv, const double* wx, int wx_per_band,
+                       const double* wtilde, const double* aqsm, const double* aqsn,
+                       const double* I_eps, const double* vcoul, double occ, double* achtemp,
+                       double* acht_n1_loc, double* achstemp, double* asxtemp) {
+  const int nb = (int)nb_, nvband = (int)nvband_, ngpown = (int)ngpown_, ncouls = (int)ncouls_;
+  hipStream_t s = ctx->stream;
+  const double2* wt = reinterpret_cast<const double2*>(wtilde);
+  const double2* I = reinterpret_cast<const double2*>(I_eps);
+  const double2* am = reinterpret_cast<const double2*>(aqsm);
+  const double2* an = reinterpret_cast<const double2*>(aqsn);
+
+  // workspace
+  const MainCfg cfg = pick_main(nb);
+  int W = 0;
+  run_main(ctx, cfg, nullptr, nullptr, nullptr, nullptr, nullptr, nb, ngpown, ncouls, nullptr, &W);
+  void *pS, *pC, *pwx, *prange, *ppart, *pband;
+  int rc = pool_get(ctx, "ws_S", (size_t)nb * ngpown * 16, &pS);
+  if (!rc) rc = pool_get(ctx, "ws_C", (size_t)nb * ngpown * 16, &pC);
+  if (!rc) rc = pool_get(ctx, "ws_wxb", (size_t)nb * 3 * 8, &pwx);
+  if (!rc) rc = pool_get(ctx, "ws_range", 64, &prange);
+  if (!rc) rc = pool_get(ctx, "ws_partial", (size_t)W * nb * 3 * 16, &ppart);
+  if (!rc) rc = pool_get(ctx, "ws_band_dd", (size_t)nb * 3 * 32, &pband);
+  if (rc) return rc;
+  double2* S = static_cast<double2*>(pS);
+  double2* C = static_cast<double2*>(pC);
+  double* wxb = static_cast<double*>(pwx);
+  double* range = static_cast<double*>(prange);
+
+  launch_check_indices(inv, ngpown_, indinv, ncouls_ + 1, ncouls, ncouls, ctx->d_flag, s);
+  hipLaunchKernelGGL(k_gpp_prep, dim3((nb + 255) / 256, ngpown), dim3(256), 0, s, inv, indinv, wt, am, an,
+                     vcoul, wx, wx_per_band, nb, ngpown, ncouls, S, C, wxb);
+  hipLaunchKernelGGL(k_wx_range, dim3(1), dim3(256), 0, s, wxb, nb * 3, range);
+  if ((rc = check_launch(ctx, "k_gpp_prep"))) return rc;
+
+  rc = run_main(ctx, cfg, wt, I, S, wxb, range, nb, ngpown, ncouls, static_cast<double2*>(ppart), &W);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_reduce_bands, dim3((nb * 3 + 255) / 256), dim3(256), 0, s,
+                     static_cast<const double2*>(ppart), W, nb, reinterpret_cast<double2*>(acht_n1_loc),
+                     static_cast<double4*>(pband));
+  hipLaunchKernelGGL(k_reduce_achtemp, dim3(1), dim3(256), 0, s, static_cast<const double4*>(pband), nb,
+                     reinterpret_cast<double2*>(achtemp));
+  if ((rc = check_launch(ctx, "k_reduce"))) return rc;
+
+  if (asxtemp) {
+    if (nvband == 0) {
+      fill_zero(asxtemp, 6 * sizeof(double), s);
+    } else {
+      const int chunk = 4096, nchunks = (ncouls + chunk - 1) / chunk;
+      long items = (long)ngpown * nchunks;
+      long Wa = (long)ctx->cus * 4;
+      if (Wa > items) Wa = items;
+      void* pax;
+      if ((rc = pool_get(ctx, "ws_asx", (size_t)Wa * 3 * 32, &pax))) return rc;
+      {
+        KernelTimer kt(ctx, "gpp_asx");
+        hipLaunchKernelGGL(k_gpp_asx<256>, dim3((unsigned)Wa), dim3(256), 0, s, inv, indinv, wt, I, am, an,
+                           vcoul, wxb, nvband, ngpown, ncouls, chunk, nchunks, occ,
+                           static_cast<double4*>(pax));
+      }
+      launch_reduce_cdd(static_cast<const double4*>(pax), (int)Wa, 3, reinterpret_cast<double2*>(asxtemp), s);
+      if ((rc = check_launch(ctx, "k_gpp_asx"))) return rc;
+    }
+  }
+  if (achstemp) {
+    rc = contract_achstemp(ctx, nb, ngpown, ncouls, inv, indinv, an, am, I, vcoul,
+                           reinterpret_cast<double2*>(achstemp));
+    if (rc) return rc;
+  }
+  return BGW_OK;
+}
+
+}  // namespace bgw

@@ -1,0 +1,79 @@
+// gpp_common.h -- device helpers shared by the GPP/FF kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "bgw_internal.h"
+
+namespace bgw {
+
+// GPP constants: to1 and limittwo as declared in GPP/gpp.cpp:167-168, with the
+// companion thresholds of BerkeleyGW's occupied-band branch.
+constexpr double kTo1 = 1e-6;
+constexpr double kLimitTwo = 0.25;
+constexpr double kLimitOne = 1.0 / (kTo1 * 4.0);
+constexpr double kSexcut = 4.0;
+
+__host__ __device__ inline double2 cmul(double2 a, double2 b) {
+  return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__host__ __device__ inline double2 cconj(double2 a) { return make_double2(a.x, -a.y); }
+__host__ __device__ inline double2 cscale(double2 a, double s) { return make_double2(a.x * s, a.y * s); }
+__host__ __device__ inline double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
+__host__ __device__ inline double2 csub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
+__host__ __device__ inline double cnorm2(double2 a) { return a.x * a.x + a.y * a.y; }
+
+// igp = indinv[inv_igp_index[p]] (GPP/gpp.cpp:362-363), clamped into range so a bad
+// index can never address outside the arrays (bad indices are reported separately).
+__device__ inline int gpp_igp(const int* __restrict__ inv, const int* __restrict__ indinv, int p, int ncouls) {
+  int a = inv[p];
+  a = a < 0 ? 0 : (a > ncouls ? ncouls : a);
+  int g = indinv[a];
+  return g < 0 ? 0 : (g >= ncouls ? ncouls - 1 : g);
+}
+// FF form: indinv has ncouls entries (FF/ff.cpp:552), inv_igp_index < ncouls.
+__device__ inline int ff_igp(const int* __restrict__ inv, const int* __restrict__ indinv, int p, int ncouls) {
+  int a = inv[p];
+  a = a < 0 ? 0 : (a >= ncouls ? ncouls - 1 : a);
+  int g = indinv[a];
+  return g < 0 ? 0 : (g >= ncouls ? ncouls - 1 : g);
+}
+
+// ssx of BerkeleyGW's occupied-band GPP branch for one (wx, wtilde, I_eps):
+// regular region ssx = Omega2 conj(cden)/|cden|^2 with cden = wx^2 - wtilde^2,
+// near-pole region ssx = -Omega2 conj(cden)/|cden|^2 delw with cden = 4 wtilde^2 (delw+1/2),
+// zero otherwise, and zero when |ssx| > sexcut |I_eps| for wx < 0.
+__device__ inline double2 gpp_ssx(double wx, double2 wt, double2 I) {
+  const double2 wt2 = cmul(wt, wt);
+  const double2 Omega2 = cmul(wt2, I);
+  const double2 wdiff = make_double2(wx - wt.x, -wt.y);
+  const double wdiffr = cnorm2(wdiff);
+  const double rden = 1.0 / wdiffr;
+  const double2 delw = cscale(cmul(wt, cconj(wdiff)), rden);
+  const double delwr = cnorm2(delw);
+  double2 ssx;
+  if (wdiffr > kLimitTwo && delwr < kLimitOne) {
+    const double2 cden = make_double2(wx * wx - wt2.x, -wt2.y);
+    const double rd = 1.0 / cnorm2(cden);
+    ssx = cscale(cmul(Omega2, cconj(cden)), rd);
+  } else if (delwr > kTo1) {
+    const double2 cden = cscale(cmul(wt2, make_double2(delw.x + 0.5, delw.y)), 4.0);
+    const double rd = 1.0 / cnorm2(cden);
+    const double2 t = cscale(cmul(make_double2(-Omega2.x, -Omega2.y), cconj(cden)), rd);
+    ssx = cmul(t, delw);
+  } else {
+    ssx = make_double2(0.0, 0.0);
+  }
+  const double cutoff = kSexcut * sqrt(cnorm2(I));
+  if (sqrt(cnorm2(ssx)) > cutoff && wx < 0.0) ssx = make_double2(0.0, 0.0);
+  return ssx;
+}
+
+// MFMA trace contraction (contract.hip):
+//   out = sum_{r} sum_p W(r,p) sum_g A(rows[r], g) * B(p, g),
+//   W(r,p) = wts[r] * vcoul(igp(p)) * conj(aqsm(rows[r], igp(p))).
+int contract_achstemp(bgw_ctx* ctx, int nb, int ngpown, int ncouls, const int* inv,
+                      const int* indinv, const double2* aqsn, const double2* aqsm,
+                      const double2* Ieps, const double* vcoul, double2* achstemp);
+
+}  // namespace bgw

@@ -1,0 +1,405 @@
+"""ctypes binding of libbgw (the C ABI declared in include/bgw.h).
+
+Host-pointer calls take numpy arrays in the reference layout (row-major,
+interleaved complex128); ``*_device`` calls take raw device addresses (ints), for
+example ``torch.Tensor.data_ptr()``.  There is no CPU fallback: without the
+built library, or without a GPU, every compute call raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from pathlib import Path
+
+import numpy as np
+
+PKG = Path(__file__).resolve().parents[1]
+LIB_PATH = PKG / "build" / "libbgw.so"
+HEADER = PKG.parent / "include" / "bgw.h"
+
+BGW_OK, BGW_EINVAL, BGW_ENOMEM, BGW_EHIP, BGW_ENODEV = 0, -22, -12, -5, -19
+BGW_FP64, BGW_FP32_BULK = 0, 1
+
+_ERRNAMES = {BGW_EINVAL: "BGW_EINVAL", BGW_ENOMEM: "BGW_ENOMEM", BGW_EHIP: "BGW_EHIP", BGW_ENODEV: "BGW_ENODEV"}
+
+_lib = None
+
+
+class BgwError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"{_ERRNAMES.get(code, code)}: {msg}")
+        self.code = code
+
+
+def lib() -> C.CDLL:
+    """Loads build/libbgw.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise FileNotFoundError(f"{LIB_PATH} not built; run __graft_entry__.build()")
+        _lib = C.CDLL(str(LIB_PATH))
+        _declare(_lib)
+    return _lib
+
+
+_P = C.c_void_p
+_SZ = C.c_size_t
+_D = C.c_double
+_I = C.c_int
+
+
+def _declare(L: C.CDLL) -> None:
+    L.bgw_create.argtypes = [C.POINTER(_P), _I]
+    L.bgw_destroy.argtypes = [_P]
+    L.bgw_destroy.restype = None
+    L.bgw_last_error.argtypes = [_P]
+    L.bgw_last_error.restype = C.c_char_p
+    L.bgw_set_stream.argtypes = [_P, _P]
+    L.bgw_get_stream.argtypes = [_P]
+    L.bgw_get_stream.restype = _P
+    L.bgw_set_precision.argtypes = [_P, _I]
+    L.bgw_synchronize.argtypes = [_P]
+    L.bgw_set_timing.argtypes = [_P, _I]
+    L.bgw_kernel_time.argtypes = [_P, C.c_char_p, C.POINTER(_D), C.POINTER(C.c_long)]
+    L.bgw_reset_timing.argtypes = [_P]
+    L.bgw_device_cus.argtypes = [_P]
+    gpp12 = [_P, _SZ, _SZ, _SZ, _P, _P, _P, _P, _P, _P, _P, _P, _P]
+    L.bgw_noflagOCC_solver.argtypes = gpp12
+    ext = [_P, _SZ, _SZ, _SZ, _SZ, _P, _P, _P, _P, _P, _P, _P, _P, _D, _P, _P, _P, _P]
+    L.bgw_gpp_solver.argtypes = ext
+    L.bgw_gpp_solver_device.argtypes = ext
+    L.bgw_gpp_backward.argtypes = ext[:-1]
+    L.bgw_gpp_backward_device.argtypes = ext[:-1]
+    L.bgw_compute_fact.argtypes = [_D, _I, _P, C.POINTER(_D), C.POINTER(_D), C.POINTER(_I), _I, _I]
+    achs = [_P, _I, _I, _I, _P, _P, _P, _P, _P, _P, _P]
+    L.bgw_achsDtemp_Kernel.argtypes = achs
+    L.bgw_achsDtemp_Kernel_device.argtypes = achs
+    asx = [_P, _I, _I, _I, _I, _I, _D, _D, _D, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P]
+    L.bgw_asxDtemp_Kernel.argtypes = asx
+    L.bgw_asxDtemp_Kernel_device.argtypes = asx
+    cor = [_P, _I, _I, _I, _I, _I, _I, _D, _D, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P]
+    L.bgw_achDtemp_cor_Kernel.argtypes = cor
+    L.bgw_achDtemp_cor_Kernel_device.argtypes = cor
+    L.bgw_synth_uniform.argtypes = [C.c_ulonglong, _D, _D, _P, _SZ]
+    L.bgw_synth_uniform.restype = None
+    L.bgw_synth_complex.argtypes = [C.c_ulonglong, _D, _D, _D, _D, _P, _SZ]
+    L.bgw_synth_complex.restype = None
+    L.bgw_synth_index.argtypes = [C.c_ulonglong, _I, _P, _SZ]
+    L.bgw_synth_index.restype = None
+    L.bgw_synth_permutation.argtypes = [C.c_ulonglong, _P, _SZ]
+    L.bgw_synth_permutation.restype = None
+
+
+def header_symbols() -> list[str]:
+    """Names of every function declared in include/bgw.h."""
+    import re
+
+    text = HEADER.read_text()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(bgw_[a-zA-Z0-9_]+)\s*\(", text)))
+
+
+def _ptr(a):
+    if a is None:
+        return None
+    if isinstance(a, int):
+        return a
+    if not a.flags["C_CONTIGUOUS"]:
+        raise ValueError("array must be C-contiguous")
+    return a.ctypes.data
+
+
+def _cplx(a: np.ndarray) -> np.ndarray:
+    a = np.ascontiguousarray(a)
+    if a.dtype != np.complex128:
+        raise TypeError(f"expected complex128, got {a.dtype}")
+    return a
+
+
+# ----------------------------------------------------------------- synthetic inputs
+def synth_uniform(seed: int, lo: float, hi: float, n: int) -> np.ndarray:
+    out = np.empty(n, np.float64)
+    lib().bgw_synth_uniform(seed, lo, hi, _ptr(out), n)
+    return out
+
+
+def synth_complex(seed: int, re: tuple, im: tuple, n: int) -> np.ndarray:
+    out = np.empty(n, np.complex128)
+    lib().bgw_synth_complex(seed, re[0], re[1], im[0], im[1], _ptr(out), n)
+    return out
+
+
+def synth_index(seed: int, n_max: int, n: int) -> np.ndarray:
+    out = np.empty(n, np.int32)
+    lib().bgw_synth_index(seed, n_max, _ptr(out), n)
+    return out
+
+
+def synth_permutation(seed: int, n: int) -> np.ndarray:
+    out = np.empty(n, np.int32)
+    lib().bgw_synth_permutation(seed, _ptr(out), n)
+    return out
+
+
+# The benchmark configurations (number_bands, nvband, ncouls, nodes_per_group).
+GPP_CONFIGS = {
+    0: dict(nb=32, nvband=2, ncouls=2048, nodes_per_group=1),
+    1: dict(nb=512, nvband=2, ncouls=32768, nodes_per_group=20),
+    2: dict(nb=512, nvband=2, ncouls=32768, nodes_per_group=20),
+    3: dict(nb=256, nvband=2, ncouls=16384, nodes_per_group=20),
+}
+FF_CONFIG4 = dict(nb=512, nvband=2, ncouls=16384, nodes_per_group=20, nFreq=15, nfreqeval=10)
+
+
+def seed_base(config: int) -> int:
+    return 0xB6600000 + 16 * config
+
+
+def gpp_inputs(nb: int, nvband: int, ncouls: int, ngpown: int, seed: int, kind: str = "random") -> dict:
+    """GPP inputs in the reference layout.
+
+    kind="random": the benchmark distributions (aqsm/aqsn re,im U[-1,1]; I_eps re
+    U[0,.5], im U[-.05,.05]; wtilde re U[.5,3], im U[-.1,0]; vcoul U[.1,1];
+    inv_igp_index U{0..ncouls-1}; indinv a permutation with indinv[ncouls] =
+    ncouls-1; ek U[-1,1]; wx(n1,iw) = max(10 - ek[n1] + (iw-1), 1e-6)).
+    kind="reference": the reference driver's constants (GPP/gpp.cpp:246-276):
+    every complex input 0.5+0.5i, vcoul 1, inv_igp_index[p] = (p+1)*ncouls/ngpown,
+    indinv[g] = g, wx = {3,4,5} for every band.
+    """
+    if kind == "reference":
+        c = np.full(nb * ncouls, 0.5 + 0.5j)
+        e = np.full(ngpown * ncouls, 0.5 + 0.5j)
+        inv = ((np.arange(ngpown, dtype=np.int64) + 1) * ncouls // ngpown).astype(np.int32)
+        indinv = np.concatenate([np.arange(ncouls, dtype=np.int32), np.array([ncouls - 1], np.int32)])
+        wx = np.tile(np.array([3.0, 4.0, 5.0]), nb)
+        return dict(nb=nb, nvband=nvband, ncouls=ncouls, ngpown=ngpown, aqsm=c, aqsn=c.copy(), I_eps=e,
+                    wtilde=e.copy(), vcoul=np.ones(ncouls), inv=inv, indinv=indinv, wx=wx, ek=None)
+    aqsm = synth_complex(seed + 1, (-1, 1), (-1, 1), nb * ncouls)
+    aqsn = synth_complex(seed + 2, (-1, 1), (-1, 1), nb * ncouls)
+    I_eps = synth_complex(seed + 3, (0, 0.5), (-0.05, 0.05), ngpown * ncouls)
+    wtilde = synth_complex(seed + 4, (0.5, 3.0), (-0.1, 0.0), ngpown * ncouls)
+    vcoul = synth_uniform(seed + 5, 0.1, 1.0, ncouls)
+    inv = synth_index(seed + 6, ncouls, ngpown)
+    indinv = np.concatenate([synth_permutation(seed + 7, ncouls), np.array([ncouls - 1], np.int32)])
+    ek = synth_uniform(seed + 8, -1.0, 1.0, nb)
+    e_lk, dw, to1 = 10.0, 1.0, 1e-6
+    wx = e_lk - ek[:, None] + dw * (np.arange(3)[None, :] - 1.0)
+    wx = np.where(wx < to1, to1, wx).reshape(-1).copy()
+    return dict(nb=nb, nvband=nvband, ncouls=ncouls, ngpown=ngpown, aqsm=aqsm, aqsn=aqsn, I_eps=I_eps,
+                wtilde=wtilde, vcoul=vcoul, inv=inv, indinv=indinv, wx=wx, ek=ek)
+
+
+def gpp_config_inputs(config: int, nb: int | None = None) -> dict:
+    c = GPP_CONFIGS[config]
+    ngpown = c["ncouls"] // c["nodes_per_group"]
+    return gpp_inputs(nb or c["nb"], c["nvband"], c["ncouls"], ngpown, seed_base(config))
+
+
+def ff_inputs(nb: int, nvband: int, ncouls: int, ngpown: int, nFreq: int, nfreqeval: int, seed: int,
+              kind: str = "random") -> dict:
+    """FF inputs.  kind="reference" reproduces FF/ff.cpp:584-633 (inv/indinv identity,
+    ekq = -10 + n1, aqs 0.5+0.5i, vcoul 1, I_epsR 0.5+0.5i, I_epsA 0.5-0.5i,
+    dFreqGrid = 2f, freqevalmin 0, freqevalstep 0.5, occ 1).  kind="random": aqs as
+    the GPP inputs, I_eps{R,A} re U[0,.5], im U[-.1,.1], ekq U[-1,1], dFreqGrid
+    uniform on [0,2], same freqevalmin/step/occ."""
+    npg = nFreq * ngpown * ncouls
+    if kind == "reference":
+        return dict(nb=nb, nvband=nvband, ncouls=ncouls, ngpown=ngpown, nFreq=nFreq, nfreqeval=nfreqeval,
+                    inv=np.arange(ngpown, dtype=np.int32), indinv=np.arange(ncouls, dtype=np.int32),
+                    ekq=-10.0 + np.arange(nb, dtype=np.float64), aqsm=np.full(nb * ncouls, 0.5 + 0.5j),
+                    aqsn=np.full(nb * ncouls, 0.5 + 0.5j), vcoul=np.ones(ncouls),
+                    I_epsR=np.full(npg, 0.5 + 0.5j), I_epsA=np.full(npg, 0.5 - 0.5j),
+                    dFreqGrid=2.0 * np.arange(nFreq, dtype=np.float64), freqevalmin=0.0, freqevalstep=0.5,
+                    occ=1.0)
+    return dict(nb=nb, nvband=nvband, ncouls=ncouls, ngpown=ngpown, nFreq=nFreq, nfreqeval=nfreqeval,
+                inv=synth_index(seed + 6, ncouls, ngpown), indinv=synth_permutation(seed + 7, ncouls),
+                ekq=synth_uniform(seed + 8, -1.0, 1.0, nb),
+                aqsm=synth_complex(seed + 1, (-1, 1), (-1, 1), nb * ncouls),
+                aqsn=synth_complex(seed + 2, (-1, 1), (-1, 1), nb * ncouls),
+                vcoul=synth_uniform(seed + 5, 0.1, 1.0, ncouls),
+                I_epsR=synth_complex(seed + 9, (0, 0.5), (-0.1, 0.1), npg),
+                I_epsA=synth_complex(seed + 10, (0, 0.5), (-0.1, 0.1), npg),
+                dFreqGrid=2.0 * np.arange(nFreq, dtype=np.float64) / (nFreq - 1), freqevalmin=0.0,
+                freqevalstep=0.5, occ=1.0)
+
+
+def ff_config4_inputs(nb: int | None = None, ngpown: int | None = None, ncouls: int | None = None) -> dict:
+    c = FF_CONFIG4
+    nc = ncouls or c["ncouls"]
+    return ff_inputs(nb or c["nb"], c["nvband"], nc, ngpown or c["ncouls"] // c["nodes_per_group"], c["nFreq"],
+                     c["nfreqeval"], seed_base(4))
+
+
+# ----------------------------------------------------------------------- context
+class Context:
+    """One GPU + one stream (bgw_ctx)."""
+
+    def __init__(self, device: int = 0):
+        self._h = _P()
+        rc = lib().bgw_create(C.byref(self._h), device)
+        if rc != BGW_OK:
+            raise BgwError(rc, "bgw_create failed (no usable GPU?)")
+
+    def close(self) -> None:
+        if self._h:
+            lib().bgw_destroy(self._h)
+            self._h = _P()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def handle(self):
+        return self._h
+
+    def _check(self, rc: int) -> None:
+        if rc != BGW_OK:
+            raise BgwError(rc, lib().bgw_last_error(self._h).decode())
+
+    def set_stream(self, stream_ptr: int | None) -> None:
+        self._check(lib().bgw_set_stream(self._h, stream_ptr))
+
+    def set_precision(self, precision: int) -> None:
+        self._check(lib().bgw_set_precision(self._h, precision))
+
+    def synchronize(self) -> None:
+        self._check(lib().bgw_synchronize(self._h))
+
+    def set_timing(self, on: bool) -> None:
+        self._check(lib().bgw_set_timing(self._h, int(on)))
+
+    def kernel_time(self, name: str) -> tuple[float, int]:
+        ms, n = _D(), C.c_long()
+        self._check(lib().bgw_kernel_time(self._h, name.encode(), C.byref(ms), C.byref(n)))
+        return ms.value, n.value
+
+    def reset_timing(self) -> None:
+        self._check(lib().bgw_reset_timing(self._h))
+
+    @property
+    def cus(self) -> int:
+        return lib().bgw_device_cus(self._h)
+
+    # ------------------------------------------------------------ GPP (host)
+    def noflagOCC_solver(self, nb, ngpown, ncouls, inv, indinv, wx3, wtilde, aqsm, aqsn, I_eps, vcoul):
+        ach = np.zeros(3, np.complex128)
+        self._check(lib().bgw_noflagOCC_solver(
+            self._h, nb, ngpown, ncouls, _ptr(np.ascontiguousarray(inv, np.int32)),
+            _ptr(np.ascontiguousarray(indinv, np.int32)), _ptr(np.ascontiguousarray(wx3, np.float64)),
+            _ptr(_cplx(wtilde)), _ptr(_cplx(aqsm)), _ptr(_cplx(aqsn)), _ptr(_cplx(I_eps)),
+            _ptr(np.ascontiguousarray(vcoul, np.float64)), _ptr(ach)))
+        return ach
+
+    def gpp_solver(self, d: dict, occ: float = 1.0, outputs=("achtemp", "acht_n1_loc", "achstemp", "asxtemp")):
+        nb = d["nb"]
+        res = {"achtemp": np.zeros(3, np.complex128)}
+        if "acht_n1_loc" in outputs:
+            res["acht_n1_loc"] = np.zeros(nb * 3, np.complex128)
+        if "achstemp" in outputs:
+            res["achstemp"] = np.zeros(1, np.complex128)
+        if "asxtemp" in outputs:
+            res["asxtemp"] = np.zeros(3, np.complex128)
+        self._check(lib().bgw_gpp_solver(
+            self._h, nb, d["nvband"], d["ngpown"], d["ncouls"], _ptr(d["inv"]), _ptr(d["indinv"]), _ptr(d["wx"]),
+            _ptr(_cplx(d["wtilde"])), _ptr(_cplx(d["aqsm"])), _ptr(_cplx(d["aqsn"])), _ptr(_cplx(d["I_eps"])),
+            _ptr(d["vcoul"]), occ, _ptr(res["achtemp"]), _ptr(res.get("acht_n1_loc")), _ptr(res.get("achstemp")),
+            _ptr(res.get("asxtemp"))))
+        if "acht_n1_loc" in res:
+            res["acht_n1_loc"] = res["acht_n1_loc"].reshape(nb, 3)
+        return res
+
+    def gpp_backward(self, d: dict, occ: float = 1.0):
+        npg = d["ngpown"] * d["ncouls"]
+        ds = np.zeros(3, np.complex128)
+        dw = np.zeros(npg, np.complex128)
+        dI = np.zeros(npg, np.complex128)
+        self._check(lib().bgw_gpp_backward(
+            self._h, d["nb"], d["nvband"], d["ngpown"], d["ncouls"], _ptr(d["inv"]), _ptr(d["indinv"]),
+            _ptr(d["wx"]), _ptr(_cplx(d["wtilde"])), _ptr(_cplx(d["aqsm"])), _ptr(_cplx(d["aqsn"])),
+            _ptr(_cplx(d["I_eps"])), _ptr(d["vcoul"]), occ, _ptr(ds), _ptr(dw), _ptr(dI)))
+        return {"dsigma_domega": ds, "dL_dwtilde": dw, "dL_dI_eps": dI}
+
+    # ------------------------------------------------------------- GPP (device)
+    def gpp_solver_device(self, nb, nvband, ngpown, ncouls, inv, indinv, wx, wtilde, aqsm, aqsn, I_eps, vcoul,
+                          occ, achtemp, acht_n1_loc=None, achstemp=None, asxtemp=None):
+        self._check(lib().bgw_gpp_solver_device(self._h, nb, nvband, ngpown, ncouls, inv, indinv, wx, wtilde, aqsm,
+                                                aqsn, I_eps, vcoul, occ, achtemp, acht_n1_loc, achstemp, asxtemp))
+
+    def gpp_backward_device(self, nb, nvband, ngpown, ncouls, inv, indinv, wx, wtilde, aqsm, aqsn, I_eps, vcoul,
+                            occ, dsigma=None, dwt=None, dI=None):
+        self._check(lib().bgw_gpp_backward_device(self._h, nb, nvband, ngpown, ncouls, inv, indinv, wx, wtilde,
+                                                  aqsm, aqsn, I_eps, vcoul, occ, dsigma, dwt, dI))
+
+    # -------------------------------------------------------------------- FF
+    def achsDtemp_Kernel(self, f: dict):
+        out = np.zeros(1, np.complex128)
+        self._check(lib().bgw_achsDtemp_Kernel(
+            self._h, f["nb"], f["ngpown"], f["ncouls"], _ptr(f["inv"]), _ptr(f["indinv"]), _ptr(_cplx(f["aqsn"])),
+            _ptr(_cplx(f["aqsm"])), _ptr(_cplx(f["I_epsR"])), _ptr(f["vcoul"]), _ptr(out)))
+        return out[0]
+
+    def asxDtemp_Kernel(self, f: dict):
+        out = np.zeros(f["nfreqeval"], np.complex128)
+        self._check(lib().bgw_asxDtemp_Kernel(
+            self._h, f["nvband"], f["nfreqeval"], f["ncouls"], f["ngpown"], f["nFreq"], f["freqevalmin"],
+            f["freqevalstep"], f["occ"], _ptr(f["ekq"]), _ptr(f["dFreqGrid"]), _ptr(f["inv"]), _ptr(f["indinv"]),
+            _ptr(_cplx(f["aqsm"])), _ptr(_cplx(f["aqsn"])), _ptr(f["vcoul"]), _ptr(_cplx(f["I_epsR"])),
+            _ptr(_cplx(f["I_epsA"])), _ptr(out)))
+        return out
+
+    def achDtemp_cor_Kernel(self, f: dict):
+        out = np.zeros(f["nfreqeval"], np.complex128)
+        self._check(lib().bgw_achDtemp_cor_Kernel(
+            self._h, f["nb"], f["nvband"], f["nfreqeval"], f["ncouls"], f["ngpown"], f["nFreq"], f["freqevalmin"],
+            f["freqevalstep"], _ptr(f["ekq"]), _ptr(f["dFreqGrid"]), _ptr(f["inv"]), _ptr(f["indinv"]),
+            _ptr(_cplx(f["aqsm"])), _ptr(_cplx(f["aqsn"])), _ptr(f["vcoul"]), _ptr(_cplx(f["I_epsR"])),
+            _ptr(_cplx(f["I_epsA"])), _ptr(out)))
+        return out
+
+    def ff_device(self, op: str, f: dict, ptrs: dict, out_ptr: int):
+        L = lib()
+        if op == "achsDtemp":
+            rc = L.bgw_achsDtemp_Kernel_device(self._h, f["nb"], f["ngpown"], f["ncouls"], ptrs["inv"],
+                                               ptrs["indinv"], ptrs["aqsn"], ptrs["aqsm"], ptrs["I_epsR"],
+                                               ptrs["vcoul"], out_ptr)
+        elif op == "asxDtemp":
+            rc = L.bgw_asxDtemp_Kernel_device(self._h, f["nvband"], f["nfreqeval"], f["ncouls"], f["ngpown"],
+                                              f["nFreq"], f["freqevalmin"], f["freqevalstep"], f["occ"],
+                                              ptrs["ekq"], ptrs["dFreqGrid"], ptrs["inv"], ptrs["indinv"],
+                                              ptrs["aqsm"], ptrs["aqsn"], ptrs["vcoul"], ptrs["I_epsR"],
+                                              ptrs["I_epsA"], out_ptr)
+        else:
+            rc = L.bgw_achDtemp_cor_Kernel_device(self._h, f["nb"], f["nvband"], f["nfreqeval"], f["ncouls"],
+                                                  f["ngpown"], f["nFreq"], f["freqevalmin"], f["freqevalstep"],
+                                                  ptrs["ekq"], ptrs["dFreqGrid"], ptrs["inv"], ptrs["indinv"],
+                                                  ptrs["aqsm"], ptrs["aqsn"], ptrs["vcoul"], ptrs["I_epsR"],
+                                                  ptrs["I_epsA"], out_ptr)
+        self._check(rc)
+
+
+def compute_fact(wx: float, dFreqGrid: np.ndarray, loop: int, flag_occ: bool, ifreq: int = 0,
+                 fact1: float = 0.0, fact2: float = 0.0):
+    g = np.ascontiguousarray(dFreqGrid, np.float64)
+    f1, f2, i = _D(fact1), _D(fact2), _I(ifreq)
+    rc = lib().bgw_compute_fact(wx, len(g), _ptr(g), C.byref(f1), C.byref(f2), C.byref(i), loop, int(flag_occ))
+    if rc != BGW_OK:
+        raise BgwError(rc, "bgw_compute_fact")
+    return f1.value, f2.value, i.value
+
+
+def gpp_flops(nb: int, ngpown: int, ncouls: int) -> float:
+    """Algorithmic FP64 work of achtemp: 15 FLOP per (n1, p, g, iw) term."""
+    return 15.0 * nb * ngpown * ncouls * 3
+
+
+def ff_flops(n_cmacs: float) -> float:
+    return 8.0 * n_cmacs

@@ -1,0 +1,205 @@
+/*
+ * bgw.h -- C ABI of the MI355X (gfx950) BerkeleyGW self-energy kernels.
+ *
+ * Drop-in boundary for the reference mini-apps' kernel API:
+ *   GPP  noflagOCC_solver            GPP/gpp.cpp:307-402   (decl common/Defines.h:50-62)
+ *   FF   achsDtemp_Kernel            FF/ff.cpp:171-209
+ *        asxDtemp_Kernel             FF/ff.cpp:211-293
+ *        achDtemp_cor_Kernel         FF/ff.cpp:295-379     (+ schDttt_corKernel1/2 :381-451)
+ *        compute_fact                FF/ff.cpp:103-152
+ *
+ * Conventions
+ *  - Every reference `ArrayMD&` argument becomes a plain pointer to its `.dptr`
+ *    (row-major, last index fastest; arrayMD/arrayMD.h:58-81).  Complex arrays are
+ *    `double*` to interleaved (re,im) pairs, layout-compatible with
+ *    `std::complex<double>[]` (common/Defines.h:21-22 with -Dstd_complex).
+ *  - Argument order is the reference's, with a leading `bgw_ctx*`; `void` becomes
+ *    an `int` status (BGW_OK or a negative BGW_E* code).  Nothing throws across
+ *    the ABI; there is no CPU fallback: without a usable GPU every compute entry
+ *    point returns BGW_ENODEV.
+ *  - Functions without suffix take HOST pointers and are synchronous: results are
+ *    valid on return, exactly like the reference.  `_device` variants take DEVICE
+ *    pointers for every array, enqueue on the context's stream and return at once;
+ *    call bgw_synchronize() before reading results.  Index arrays passed to a
+ *    `_device` variant are checked on the GPU; a bad index is clamped (no fault)
+ *    and reported as BGW_EINVAL by the next bgw_synchronize().
+ *  - One context = one GPU + one stream.  Distinct contexts may be used from
+ *    distinct threads; one context must not be used concurrently.
+ */
+#ifndef BGW_H
+#define BGW_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+#define BGW_OK      0
+#define BGW_EINVAL  (-22) /* null pointer, zero size, bad index, ngpown>ncouls, overflow */
+#define BGW_ENOMEM  (-12) /* device allocation failed                                   */
+#define BGW_EHIP    (-5)  /* any other HIP runtime error (text in bgw_last_error)       */
+#define BGW_ENODEV  (-19) /* no GPU / HIP runtime not usable                            */
+
+/* ---- arithmetic forms (bgw_set_precision) ------------------------------- */
+#define BGW_FP64       0  /* plain FP64 throughout (default)                           */
+#define BGW_FP32_BULK  1  /* bulk complex products in FP32 (GPP) / split-FP32 MFMA (FF),
+                             FP64 pole denominators and FP64 compensated reductions   */
+
+typedef struct bgw_ctx bgw_ctx;
+
+/* ---- context ------------------------------------------------------------ */
+/* Creates a context on HIP device `device`.  No reference counterpart (the
+ * reference maps data per call inside the solver, GPP/gpp.cpp:320-334). */
+int         bgw_create(bgw_ctx** ctx, int device);
+void        bgw_destroy(bgw_ctx* ctx);
+const char* bgw_last_error(const bgw_ctx* ctx);
+/* Launch on a caller-owned hipStream_t (NULL = the context's own stream). */
+int         bgw_set_stream(bgw_ctx* ctx, void* hip_stream);
+void*       bgw_get_stream(bgw_ctx* ctx);
+int         bgw_set_precision(bgw_ctx* ctx, int precision);
+int         bgw_synchronize(bgw_ctx* ctx);
+/* Per-kernel HIP-event timing of the dominant kernels (off by default). */
+int         bgw_set_timing(bgw_ctx* ctx, int on);
+int         bgw_kernel_time(bgw_ctx* ctx, const char* kernel, double* total_ms, long* launches);
+int         bgw_reset_timing(bgw_ctx* ctx);
+/* Number of compute units of the context's device (for launch sizing reports). */
+int         bgw_device_cus(bgw_ctx* ctx);
+
+/* ---- GPP ---------------------------------------------------------------- */
+/* Replaces noflagOCC_solver (GPP/gpp.cpp:307-312, common/Defines.h:50-62):
+ *   achtemp[iw] = sum_{n1<number_bands} sum_{p<ngpown} sum_{g<ncouls}
+ *                 sch_store1(n1,p) * I_eps(p,g) * conj(wdiff)/|wdiff|^2,
+ *   wdiff = wx_array[iw] - wtilde(p,g),  iw = 0..2  (nstart=0, nend=3, common/Defines.h:16-17)
+ *   sch_store1 = conj(aqsm(n1,igp))*aqsn(n1,igp)*0.5*vcoul(igp)*wtilde(p,igp),
+ *   igp = indinv[inv_igp_index[p]]                               (GPP/gpp.cpp:362-371)
+ * Sizes: inv_igp_index[ngpown] in [0,ncouls]; indinv[ncouls+1] in [0,ncouls);
+ * wx_array[3]; wtilde_array, I_eps_array c128[ngpown*ncouls]; aqsmtemp,
+ * aqsntemp c128[number_bands*ncouls]; vcoul[ncouls]; achtemp c128[3] (out). */
+int bgw_noflagOCC_solver(bgw_ctx* ctx, size_t number_bands, size_t ngpown, size_t ncouls,
+                         const int* inv_igp_index, const int* indinv,
+                         const double* wx_array, const double* wtilde_array,
+                         const double* aqsmtemp, const double* aqsntemp,
+                         const double* I_eps_array, const double* vcoul,
+                         double* achtemp);
+
+/* Extended GPP solver (same loop nest, GPP/gpp.cpp:339-391) with band-dependent
+ * frequencies wx_array[number_bands*3] (wx(n1,iw) = e_lk - ek[n1] + dw*(iw-1),
+ * the reference's GPP/gpp.cpp:271-275 with e_n1kq -> ek[n1]) and the optional
+ * outputs of the full BerkeleyGW GPP kernel (NULL = not computed):
+ *   achtemp      c128[3]                 required
+ *   acht_n1_loc  c128[number_bands*3]    per-band split: sum_n1 acht_n1_loc[n1][iw] = achtemp[iw]
+ *   achstemp     c128[1]                 static COH term
+ *                  -1/2 sum_{n1,p} vcoul(igp) conj(aqsm(n1,igp)) sum_g aqsn(n1,g) I_eps(p,g) [|I_eps|>to1]
+ *   asxtemp      c128[3]                 screened exchange over occupied bands n1<nvband:
+ *                  sum occ vcoul(igp) sum_g aqsn(n1,g) conj(aqsm(n1,igp)) ssx(p,g,iw),
+ *                  ssx = Omega2/(wx^2 - wtilde^2) with Omega2 = wtilde^2 I_eps in the
+ *                  regular region; near-pole and cut-off branches as in BerkeleyGW
+ *                  (limittwo = 0.25 of GPP/gpp.cpp:168, limitone = 1/(4 to1), sexcut = 4).
+ * No reference counterpart beyond noflagOCC_solver (the reference keeps only achtemp). */
+int bgw_gpp_solver(bgw_ctx* ctx, size_t number_bands, size_t nvband, size_t ngpown, size_t ncouls,
+                   const int* inv_igp_index, const int* indinv,
+                   const double* wx_array, const double* wtilde_array,
+                   const double* aqsmtemp, const double* aqsntemp,
+                   const double* I_eps_array, const double* vcoul, double occ,
+                   double* achtemp, double* acht_n1_loc, double* achstemp, double* asxtemp);
+int bgw_gpp_solver_device(bgw_ctx* ctx, size_t number_bands, size_t nvband, size_t ngpown, size_t ncouls,
+                          const int* inv_igp_index, const int* indinv,
+                          const double* wx_array, const double* wtilde_array,
+                          const double* aqsmtemp, const double* aqsntemp,
+                          const double* I_eps_array, const double* vcoul, double occ,
+                          double* achtemp, double* acht_n1_loc, double* achstemp, double* asxtemp);
+
+/* Adjoint of bgw_gpp_solver for L = Re sum_iw (asxtemp[iw] + sum_n1 acht_n1_loc[n1][iw]).
+ * Outputs (NULL = not computed):
+ *   dsigma_domega c128[3]  d(achtemp[iw]+asxtemp[iw])/d omega, omega shifting every wx(n1,iw)
+ *   dL_dwtilde    c128[ngpown*ncouls]   dL/dRe + i dL/dIm of wtilde_array
+ *   dL_dI_eps     c128[ngpown*ncouls]   dL/dRe + i dL/dIm of I_eps_array
+ * Pole terms are recomputed, not stored.  No reference counterpart. */
+int bgw_gpp_backward(bgw_ctx* ctx, size_t number_bands, size_t nvband, size_t ngpown, size_t ncouls,
+                     const int* inv_igp_index, const int* indinv,
+                     const double* wx_array, const double* wtilde_array,
+                     const double* aqsmtemp, const double* aqsntemp,
+                     const double* I_eps_array, const double* vcoul, double occ,
+                     double* dsigma_domega, double* dL_dwtilde, double* dL_dI_eps);
+int bgw_gpp_backward_device(bgw_ctx* ctx, size_t number_bands, size_t nvband, size_t ngpown, size_t ncouls,
+                            const int* inv_igp_index, const int* indinv,
+                            const double* wx_array, const double* wtilde_array,
+                            const double* aqsmtemp, const double* aqsntemp,
+                            const double* I_eps_array, const double* vcoul, double occ,
+                            double* dsigma_domega, double* dL_dwtilde, double* dL_dI_eps);
+
+/* ---- FF ----------------------------------------------------------------- */
+/* Replaces compute_fact (FF/ff.cpp:103-152), including its quirks: strict '<'
+ * bracketing, ifreq==0 -> nFreq-2 remap for loop 1 and loop-2 occupied, no remap
+ * for loop 2 with wx>0, nothing set for loop 2 unoccupied with wx<=0.  `ifreq`,
+ * `fact1`, `fact2` are in/out exactly like the reference's references.  Host only. */
+int bgw_compute_fact(double wx, int nFreq, const double* dFreqGrid, double* fact1,
+                     double* fact2, int* ifreq, int loop, int flag_occ);
+
+/* Replaces achsDtemp_Kernel (FF/ff.cpp:171-175; note aqsntemp before aqsmtemp):
+ *   achsDtemp = sum_{n1,p} 0.5 vcoul(igp) * (-sum_g aqsn(n1,g) conj(aqsm(n1,igp)) I_epsR(1,p,g))
+ * I_epsR_array c128[nFreq*ngpown*ncouls] (slice 1 is read; nFreq>=2), indinv[ncouls]. */
+int bgw_achsDtemp_Kernel(bgw_ctx* ctx, int number_bands, int ngpown, int ncouls,
+                         const int* inv_igp_index, const int* indinv,
+                         const double* aqsntemp, const double* aqsmtemp,
+                         const double* I_epsR_array, const double* vcoul,
+                         double* achsDtemp);
+/* Replaces asxDtemp_Kernel (FF/ff.cpp:211-217).  asxDtemp c128[nfreqeval];
+ * ekq[nvband..], dFreqGrid[nFreq], I_eps{R,A}_array c128[nFreq*ngpown*ncouls]. */
+int bgw_asxDtemp_Kernel(bgw_ctx* ctx, int nvband, int nfreqeval, int ncouls, int ngpown,
+                        int nFreq, double freqevalmin, double freqevalstep, double occ,
+                        const double* ekq, const double* dFreqGrid,
+                        const int* inv_igp_index, const int* indinv,
+                        const double* aqsmtemp, const double* aqsntemp,
+                        const double* vcoul, const double* I_epsR_array,
+                        const double* I_epsA_array, double* asxDtemp);
+/* Replaces achDtemp_cor_Kernel (FF/ff.cpp:295-302) with schDttt_corKernel1/2
+ * (FF/ff.cpp:381-451) folded in.  achDtemp_cor c128[nfreqeval]; ekq[number_bands]. */
+int bgw_achDtemp_cor_Kernel(bgw_ctx* ctx, int number_bands, int nvband, int nfreqeval,
+                            int ncouls, int ngpown, int nFreq, double freqevalmin,
+                            double freqevalstep, const double* ekq, const double* dFreqGrid,
+                            const int* inv_igp_index, const int* indinv,
+                            const double* aqsmtemp, const double* aqsntemp,
+                            const double* vcoul, const double* I_epsR_array,
+                            const double* I_epsA_array, double* achDtemp_cor);
+
+/* Device-pointer forms of the three FF kernels (all arrays on the device). */
+int bgw_achsDtemp_Kernel_device(bgw_ctx* ctx, int number_bands, int ngpown, int ncouls,
+                                const int* inv_igp_index, const int* indinv,
+                                const double* aqsntemp, const double* aqsmtemp,
+                                const double* I_epsR_array, const double* vcoul,
+                                double* achsDtemp);
+int bgw_asxDtemp_Kernel_device(bgw_ctx* ctx, int nvband, int nfreqeval, int ncouls, int ngpown,
+                               int nFreq, double freqevalmin, double freqevalstep, double occ,
+                               const double* ekq, const double* dFreqGrid,
+                               const int* inv_igp_index, const int* indinv,
+                               const double* aqsmtemp, const double* aqsntemp,
+                               const double* vcoul, const double* I_epsR_array,
+                               const double* I_epsA_array, double* asxDtemp);
+int bgw_achDtemp_cor_Kernel_device(bgw_ctx* ctx, int number_bands, int nvband, int nfreqeval,
+                                   int ncouls, int ngpown, int nFreq, double freqevalmin,
+                                   double freqevalstep, const double* ekq, const double* dFreqGrid,
+                                   const int* inv_igp_index, const int* indinv,
+                                   const double* aqsmtemp, const double* aqsntemp,
+                                   const double* vcoul, const double* I_epsR_array,
+                                   const double* I_epsA_array, double* achDtemp_cor);
+
+/* ---- seeded synthetic inputs (host only) ---------------------------------
+ * SplitMix64 counter stream: x_i = mix64(seed + (i+1)*0x9E3779B97F4A7C15),
+ * double = lo + (hi-lo) * (x>>11) * 2^-53, int in [0,n) = ((x>>32)*n)>>32,
+ * permutation = Fisher-Yates driven by the same stream.  Replaces the
+ * reference drivers' constant initialisation (GPP/gpp.cpp:246-276,
+ * FF/ff.cpp:584-633) by the distributions of the benchmark configurations. */
+void bgw_synth_uniform(unsigned long long seed, double lo, double hi, double* out, size_t n);
+void bgw_synth_complex(unsigned long long seed, double re_lo, double re_hi,
+                       double im_lo, double im_hi, double* out, size_t n);
+void bgw_synth_index(unsigned long long seed, int n_max, int* out, size_t n);
+void bgw_synth_permutation(unsigned long long seed, int* out, size_t n);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* BGW_H */

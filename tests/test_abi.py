@@ -1,0 +1,79 @@
+"""C-ABI library checks that need no GPU: it loads, exports every symbol that
+include/bgw.h declares, and its host-only logic (seeded generator, compute_fact)
+behaves; without a GPU every context creation fails loudly."""
+import ctypes as C
+import os
+
+import numpy as np
+import pytest
+
+import bgw
+import oracle as O
+from test_oracle import FACT_CASES
+
+
+def test_library_exports_every_header_symbol():
+    L = bgw.lib()
+    syms = bgw.header_symbols()
+    assert len(syms) >= 25
+    missing = [s for s in syms if not hasattr(L, s)]
+    assert not missing, missing
+
+
+def test_no_gpu_means_enodev():
+    if os.path.exists("/dev/kfd"):
+        pytest.skip("a GPU is present")
+    h = C.c_void_p()
+    assert bgw.lib().bgw_create(C.byref(h), 0) == bgw.BGW_ENODEV
+    with pytest.raises(bgw.BgwError):
+        bgw.Context(0)
+
+
+def test_null_context_is_einval():
+    L = bgw.lib()
+    assert L.bgw_set_precision(None, 0) == bgw.BGW_EINVAL
+    assert L.bgw_synchronize(None) == bgw.BGW_EINVAL
+    assert L.bgw_noflagOCC_solver(None, 1, 1, 1, *([None] * 9)) == bgw.BGW_EINVAL
+    assert L.bgw_gpp_solver(None, 1, 0, 1, 1, *([None] * 8), 1.0, None, None, None, None) == bgw.BGW_EINVAL
+
+
+def test_synth_is_deterministic_and_in_range():
+    a = bgw.synth_complex(17, (0.5, 3.0), (-0.1, 0.0), 10000)
+    b = bgw.synth_complex(17, (0.5, 3.0), (-0.1, 0.0), 10000)
+    assert np.array_equal(a, b)
+    assert a.real.min() >= 0.5 and a.real.max() < 3.0 and a.imag.min() >= -0.1 and a.imag.max() < 0.0
+    # prefix property of the counter-based stream
+    assert np.array_equal(bgw.synth_uniform(3, 0, 1, 50), bgw.synth_uniform(3, 0, 1, 100)[:50])
+    p = bgw.synth_permutation(9, 1000)
+    assert sorted(p.tolist()) == list(range(1000))
+    idx = bgw.synth_index(4, 37, 5000)
+    assert idx.min() >= 0 and idx.max() < 37
+    # uniformity sanity
+    u = bgw.synth_uniform(5, 0, 1, 200000)
+    assert abs(u.mean() - 0.5) < 5e-3
+
+
+def test_gpp_inputs_match_benchmark_distributions():
+    d = bgw.gpp_inputs(8, 2, 512, 256, 1)
+    assert d["indinv"][-1] == 511 and sorted(d["indinv"][:-1].tolist()) == list(range(512))
+    assert np.all(d["inv"] < 512)
+    wx = d["wx"].reshape(8, 3)
+    assert np.allclose(wx[:, 1] - wx[:, 0], 1.0) and np.allclose(wx[:, 1], 10 - d["ek"])
+    assert d["I_eps"].real.min() >= 0 and d["I_eps"].real.max() < 0.5
+
+
+@pytest.mark.parametrize("wx,loop,occ", FACT_CASES)
+def test_product_compute_fact_matches_oracle(wx, loop, occ):
+    grid = np.array([0.0, 0.5, 1.0, 1.5, 2.0])
+    for init in ((0, 0.0, 0.0), (7, 3.0, 4.0)):
+        assert bgw.compute_fact(wx, grid, loop, occ, *init) == O.compute_fact(wx, grid, loop, occ, *init)
+
+
+def test_product_compute_fact_sweep_matches_oracle():
+    rng = np.random.default_rng(0)
+    grid = np.sort(rng.uniform(0, 3, 9))
+    grid[0] = 0.0
+    for wx in np.concatenate([rng.uniform(-4, 4, 300), grid, -grid]):
+        for loop in (1, 2):
+            for occ in (False, True):
+                assert bgw.compute_fact(wx, grid, loop, occ) == O.compute_fact(wx, grid, loop, occ)
