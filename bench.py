@@ -73,6 +73,104 @@ def cpu_baseline(d_full, bands: int):
                       f"{dt:.2f} s wall, reference noflagOCC_solver called per band"}
 
 
+def _timed_calls(torch, stream, fn, k):
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(k):
+        fn()
+    e1.record(stream)
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / k
+
+
+def ff_leg(bgw, torch, dev, stream, ctx, reps=3):
+    """FF config 4 (nb 512, nvband 2, ncouls 16384, ngpown 819, nFreq 15, nfreqeval 10):
+    the three FF kernels in FP64 MFMA and split-FP32 MFMA; FLOP = 8 per distinct
+    complex MAC (achsDtemp nb*ngpown*ncouls, achDtemp_cor 2*n_active*ngpown*ncouls,
+    asxDtemp 2*nvband*ngpown*ncouls)."""
+    log("[ff] generating config 4 inputs")
+    f = bgw.ff_config4_inputs()
+    keys = ("inv", "indinv", "ekq", "aqsm", "aqsn", "vcoul", "I_epsR", "I_epsA", "dFreqGrid")
+    T = {k: torch.from_numpy(np.ascontiguousarray(f[k])).to(dev) for k in keys}
+    ptrs = {k: v.data_ptr() for k, v in T.items()}
+    out = torch.zeros(f["nfreqeval"], dtype=torch.complex128, device=dev)
+    # active bands of achDtemp_cor (FF/ff.cpp:344-352), from the same compute_fact rules
+    n_active = 0
+    for n1 in range(f["nb"]):
+        wx = f["freqevalmin"] - f["ekq"][n1] + f["freqevalstep"]
+        occ = n1 < f["nvband"]
+        n_active += int((wx > 0 and not occ) or (not wx > 0 and occ))
+    npg = f["ngpown"] * f["ncouls"]
+    work = {"achsDtemp": 8.0 * f["nb"] * npg, "asxDtemp": 8.0 * 2 * f["nvband"] * npg,
+            "achDtemp_cor": 8.0 * 2 * n_active * npg}
+    names = {"achsDtemp": "contract_ff_achs", "asxDtemp": "contract_ff_asx", "achDtemp_cor": "contract_ff_cor"}
+    res = {"n_active_bands": n_active}
+    vals = {}
+    for prec, pname, peak in ((bgw.BGW_FP64, "fp64", 78.6), (bgw.BGW_FP32_BULK, "split_fp32", 157.3)):
+        ctx.set_precision(prec)
+        for op in ("achsDtemp", "asxDtemp", "achDtemp_cor"):
+            ctx.ff_device(op, f, ptrs, out.data_ptr())  # warm-up
+            ctx.synchronize()
+            ctx.reset_timing()
+            ctx.set_timing(True)
+            ms = _timed_calls(torch, stream, lambda: ctx.ff_device(op, f, ptrs, out.data_ptr()), reps)
+            ctx.set_timing(False)
+            kms, kn = ctx.kernel_time(names[op])
+            kms /= max(kn, 1)
+            ctx.synchronize()
+            vals[(pname, op)] = out.cpu().numpy().copy()
+            res[f"{op}_{pname}_ms"] = ms
+            res[f"{op}_{pname}_kernel_ms"] = kms
+            res[f"{op}_{pname}_mfma_frac"] = work[op] / (kms * 1e-3) / 1e12 / peak
+    ctx.set_precision(bgw.BGW_FP64)
+    for op in ("achsDtemp", "asxDtemp", "achDtemp_cor"):
+        a, b = vals[("split_fp32", op)], vals[("fp64", op)]
+        res[f"{op}_split_fp32_rel_err"] = float(np.max(np.abs(a - b) / np.abs(b)))
+    res["value_fp64"] = {op: complex(vals[("fp64", op)][0]).__repr__() for op in names}
+    del T
+    torch.cuda.empty_cache()
+    return res
+
+
+def adjoint_leg(bgw, torch, dev, stream, ctx, reps=3):
+    """GPP adjoint, config 3 (nb 256, nvband 2, ncouls 16384, ngpown 819): forward
+    (all outputs) and backward (dSigma/domega, dL/dwtilde, dL/dI_eps) times."""
+    log("[adjoint] config 3")
+    d = bgw.gpp_config_inputs(3)
+    keys = ("inv", "indinv", "wx", "wtilde", "aqsm", "aqsn", "I_eps", "vcoul")
+    T = {k: torch.from_numpy(np.ascontiguousarray(d[k])).to(dev) for k in keys}
+    P = [T[k].data_ptr() for k in keys]
+    npg = d["ngpown"] * d["ncouls"]
+    o = {k: torch.zeros(n, dtype=torch.complex128, device=dev) for k, n in
+         (("ach", 3), ("loc", d["nb"] * 3), ("achs", 1), ("asx", 3), ("ds", 3), ("dw", npg), ("dI", npg))}
+    dims = (d["nb"], d["nvband"], d["ngpown"], d["ncouls"])
+
+    def fwd():
+        ctx.gpp_solver_device(*dims, *P, 1.0, o["ach"].data_ptr(), o["loc"].data_ptr(), o["achs"].data_ptr(),
+                              o["asx"].data_ptr())
+
+    def bwd():
+        ctx.gpp_backward_device(*dims, *P, 1.0, o["ds"].data_ptr(), o["dw"].data_ptr(), o["dI"].data_ptr())
+
+    fwd()
+    bwd()
+    ctx.synchronize()
+    ctx.reset_timing()
+    ctx.set_timing(True)
+    f_ms = _timed_calls(torch, stream, fwd, reps)
+    b_ms = _timed_calls(torch, stream, bwd, reps)
+    ctx.set_timing(False)
+    bk, bn = ctx.kernel_time("gpp_bwd")
+    ctx.synchronize()
+    res = {"forward_all_outputs_ms": f_ms, "backward_ms": b_ms, "gpp_bwd_kernel_ms": bk / max(bn, 1),
+           "backward_over_forward": b_ms / f_ms,
+           "dsigma_domega": [repr(complex(z)) for z in o["ds"].cpu().numpy()]}
+    del T, o
+    torch.cuda.empty_cache()
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -80,8 +178,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--nb", type=int, default=512, help="number_bands (default: the benchmark's 512)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-bands", type=int, default=12)
-    ap.add_argument("--no-extras", action="store_true", help="skip the FP32-bulk and full-output legs")
+    ap.add_argument("--cpu-bands", type=int, default=192)
+    ap.add_argument("--no-extras", action="store_true", help="skip the FP32-bulk, full-output, FF and adjoint legs")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
+                    help="only run a device-to-device copy of BYTES (reference for rocprofv3 FETCH/WRITE_SIZE)")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -96,6 +196,15 @@ def main():
     if not torch.cuda.is_available():
         raise SystemExit("no GPU visible")
     dev = torch.device("cuda:0")
+    if args.copy_calibration:
+        n = args.copy_calibration // 8
+        a = torch.ones(n, dtype=torch.float64, device=dev)
+        b = torch.empty_like(a)
+        torch.cuda.synchronize()
+        b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes": n * 8}), flush=True)
+        return
 
     log("generating inputs")
     d = bgw.gpp_config_inputs(1, nb=args.nb)
@@ -107,7 +216,9 @@ def main():
            "achstemp": torch.zeros(1, dtype=torch.complex128, device=dev),
            "asxtemp": torch.zeros(3, dtype=torch.complex128, device=dev)}
     P = {k: v.data_ptr() for k, v in T.items()}
-    stream = torch.cuda.current_stream()
+    # a dedicated (default-priority) stream: the kernels and the timing events share it
+    stream = torch.cuda.Stream()
+    torch.cuda.set_stream(stream)
     ctx = bgw.Context(0)
     ctx.set_stream(stream.cuda_stream)
 
@@ -182,6 +293,16 @@ def main():
                                                     np.abs(o["acht_n1_loc"])))
     except Exception as e:  # the checker is optional for the measurement itself
         extras["parity_error"] = repr(e)
+
+    if not args.no_extras:
+        try:
+            extras["ff_config4"] = ff_leg(bgw, torch, dev, stream, ctx)
+        except Exception as e:
+            extras["ff_config4_error"] = repr(e)
+        try:
+            extras["adjoint_config3"] = adjoint_leg(bgw, torch, dev, stream, ctx)
+        except Exception as e:
+            extras["adjoint_config3_error"] = repr(e)
 
     cpu = None
     if not args.no_cpu_baseline:

@@ -13,6 +13,8 @@
 // The inner sum_g A*B^T is a complex NT-GEMM tile on v_mfma_f64_16x16x4f64 (four
 // real products per complex block); the weights are applied in the epilogue and
 // every workgroup writes one double-double partial, summed in fixed order.
+#include <type_traits>
+
 #include "bgw_dd.h"
 #include "bgw_fact.h"
 #include "bgw_internal.h"
@@ -23,6 +25,7 @@ namespace bgw {
 namespace {
 
 typedef double f64x4 __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 constexpr int BM = 64;   // rows (bands) per workgroup tile
 constexpr int BN = 64;   // columns (G') per workgroup tile
@@ -44,10 +47,15 @@ __device__ inline double2 load_b(const double2* __restrict__ X, const double2* _
   return v;
 }
 
-template <int MODE>
+// F32 = false: FP64 MFMA throughout.
+// F32 = true (precision form): operands rounded to FP32 in LDS, products on
+// v_mfma_f32_16x16x4f32 with FP32 accumulation over one LDS stage (16 G-vectors),
+// folded into the FP64 accumulators after every stage.
+template <int MODE, bool F32>
 __global__ __launch_bounds__(NT) void k_contract(ContractArgs a) {
-  __shared__ double As[2][BK][LDP];  // [re/im][k][row]
-  __shared__ double Bs[2][BK][LDP];  // [re/im][k][col]
+  typedef typename std::conditional<F32, float, double>::type T;
+  __shared__ T As[2][BK][LDP];  // [re/im][k][row]
+  __shared__ T Bs[2][BK][LDP];  // [re/im][k][col]
   __shared__ cdd red[NT / 64];
 
   const int job = blockIdx.z;
@@ -114,37 +122,82 @@ __global__ __launch_bounds__(NT) void k_contract(ContractArgs a) {
     for (int i = 0; i < 4; ++i) {
       const int idx = tid + NT * i;
       const int r = idx / BK, kl = idx % BK;
-      As[0][kl][r] = av[i].x;
-      As[1][kl][r] = av[i].y;
-      Bs[0][kl][r] = bv[i].x;
-      Bs[1][kl][r] = bv[i].y;
+      As[0][kl][r] = (T)av[i].x;
+      As[1][kl][r] = (T)av[i].y;
+      Bs[0][kl][r] = (T)bv[i].x;
+      Bs[1][kl][r] = (T)bv[i].y;
     }
     __syncthreads();
+    if constexpr (!F32) {
 #pragma unroll
-    for (int kk = 0; kk < BK; kk += 4) {
-      const int kr = kk + (lane >> 4), cl = lane & 15;
-      double ar[2], ai[2], nai[2], br[2], bi[2];
+      for (int kk = 0; kk < BK; kk += 4) {
+        const int kr = kk + (lane >> 4), cl = lane & 15;
+        double ar[2], ai[2], nai[2], br[2], bi[2];
 #pragma unroll
-      for (int i = 0; i < 2; ++i) {
-        ar[i] = As[0][kr][wr + i * 16 + cl];
-        ai[i] = As[1][kr][wr + i * 16 + cl];
-        nai[i] = -ai[i];
-        br[i] = Bs[0][kr][wc + i * 16 + cl];
-        bi[i] = Bs[1][kr][wc + i * 16 + cl];
+        for (int i = 0; i < 2; ++i) {
+          ar[i] = As[0][kr][wr + i * 16 + cl];
+          ai[i] = As[1][kr][wr + i * 16 + cl];
+          nai[i] = -ai[i];
+          br[i] = Bs[0][kr][wc + i * 16 + cl];
+          bi[i] = Bs[1][kr][wc + i * 16 + cl];
+        }
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+          for (int j = 0; j < 2; ++j) {
+            dr[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[i], br[j], dr[i][j], 0, 0, 0);
+            dr[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(nai[i], bi[j], dr[i][j], 0, 0, 0);
+            di[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[i], bi[j], di[i][j], 0, 0, 0);
+            di[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai[i], br[j], di[i][j], 0, 0, 0);
+          }
       }
+    } else {
+      f32x4 sr[2][2], si[2][2];
 #pragma unroll
       for (int i = 0; i < 2; ++i)
 #pragma unroll
         for (int j = 0; j < 2; ++j) {
-          dr[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[i], br[j], dr[i][j], 0, 0, 0);
-          dr[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(nai[i], bi[j], dr[i][j], 0, 0, 0);
-          di[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[i], bi[j], di[i][j], 0, 0, 0);
-          di[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai[i], br[j], di[i][j], 0, 0, 0);
+          sr[i][j] = (f32x4){0.f, 0.f, 0.f, 0.f};
+          si[i][j] = (f32x4){0.f, 0.f, 0.f, 0.f};
         }
+#pragma unroll
+      for (int kk = 0; kk < BK; kk += 4) {
+        const int kr = kk + (lane >> 4), cl = lane & 15;
+        float ar[2], ai[2], nai[2], br[2], bi[2];
+#pragma unroll
+        for (int i = 0; i < 2; ++i) {
+          ar[i] = As[0][kr][wr + i * 16 + cl];
+          ai[i] = As[1][kr][wr + i * 16 + cl];
+          nai[i] = -ai[i];
+          br[i] = Bs[0][kr][wc + i * 16 + cl];
+          bi[i] = Bs[1][kr][wc + i * 16 + cl];
+        }
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+          for (int j = 0; j < 2; ++j) {
+            sr[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(ar[i], br[j], sr[i][j], 0, 0, 0);
+            sr[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(nai[i], bi[j], sr[i][j], 0, 0, 0);
+            si[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(ar[i], bi[j], si[i][j], 0, 0, 0);
+            si[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(ai[i], br[j], si[i][j], 0, 0, 0);
+          }
+      }
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+          for (int v = 0; v < 4; ++v) {
+            dr[i][j][v] += (double)sr[i][j][v];
+            di[i][j][v] += (double)si[i][j][v];
+          }
     }
   }
 
-  // epilogue: lane holds D[row = 4*(lane/16)+v][col = lane%16] of each 16x16 block
+  // epilogue.  Output layouts on gfx950 (measured): v_mfma_f64_16x16x4f64 leaves
+  // D[row = lane/16 + 4v][col = lane%16] in element v of lane `lane`;
+  // v_mfma_f32_16x16x4f32 leaves D[row = 4(lane/16) + v][col = lane%16].  The FP32
+  // path folds its accumulators element-wise, so it keeps the FP32 layout.
   cdd acc = cdd_zero();
 #pragma unroll
   for (int j = 0; j < 2; ++j) {
@@ -156,7 +209,7 @@ __global__ __launch_bounds__(NT) void k_contract(ContractArgs a) {
     for (int i = 0; i < 2; ++i) {
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
-        const int r = rt * BM + wr + i * 16 + 4 * (lane >> 4) + v;
+        const int r = rt * BM + wr + i * 16 + (F32 ? 4 * (lane >> 4) + v : (lane >> 4) + 4 * v);
         if (r >= nrows) continue;
         const int n1 = rows[r];
         const double2 am = a.aqsm[(size_t)n1 * a.ncouls + igp];
@@ -189,12 +242,16 @@ int contract_launch(bgw_ctx* ctx, ContractArgs a, int njobs, int max_row_tiles, 
   a.partial = static_cast<double4*>(pp);
   {
     KernelTimer kt(ctx, tname);
-    if (a.Y)
-      hipLaunchKernelGGL(k_contract<kModeDiff>, grid, dim3(NT), 0, ctx->stream, a);
-    else if (a.thr2 > 0.0)
-      hipLaunchKernelGGL(k_contract<kModeMask>, grid, dim3(NT), 0, ctx->stream, a);
+    const bool f32 = ctx->precision == BGW_FP32_BULK;
+    const int mode = a.Y ? kModeDiff : (a.thr2 > 0.0 ? kModeMask : kModePlain);
+    void (*kern)(ContractArgs) = nullptr;
+    if (mode == kModeDiff)
+      kern = f32 ? k_contract<kModeDiff, true> : k_contract<kModeDiff, false>;
+    else if (mode == kModeMask)
+      kern = f32 ? k_contract<kModeMask, true> : k_contract<kModeMask, false>;
     else
-      hipLaunchKernelGGL(k_contract<kModePlain>, grid, dim3(NT), 0, ctx->stream, a);
+      kern = f32 ? k_contract<kModePlain, true> : k_contract<kModePlain, false>;
+    hipLaunchKernelGGL(kern, grid, dim3(NT), 0, ctx->stream, a);
   }
   if ((rc = check_launch(ctx, "k_contract"))) return rc;
   launch_reduce_cdd(static_cast<const double4*>(pp), (int)W, 1, a.out, ctx->stream);

@@ -74,12 +74,17 @@ __global__ void k_wx_range(const double* __restrict__ wxb, int n, double* __rest
     __syncthreads();
   }
   if (threadIdx.x == 0) {
-    // NaN anywhere in wx makes the range test fail, so every tile takes the
-    // reference-arithmetic path and propagates the NaN like the reference.
-    bool bad = false;
-    for (int i = 0; i < n; ++i) bad |= !(wxb[i] == wxb[i]);
-    range[0] = bad ? NAN : smin[0];
-    range[1] = bad ? NAN : smax[0];
+    // NaN anywhere in wx makes the range NaN, so every tile fails the range test
+    // and takes the reference-arithmetic path, propagating the NaN like the reference.
+    range[0] = smin[0];
+    range[1] = smax[0];
+  }
+  __syncthreads();
+  bool bad = false;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) bad |= !(wxb[i] == wxb[i]);
+  if (__syncthreads_or(bad) && threadIdx.x == 0) {
+    range[0] = NAN;
+    range[1] = NAN;
   }
 }
 
@@ -112,7 +117,9 @@ __device__ inline TileEntry make_entry(double2 w, double2 I, double wxmin, doubl
 }
 
 // Fast path: three reciprocals from one v_rcp_f64 plus one Newton step on the
-// product (batched inversion); all in FP64.
+// product (batched inversion); all in FP64.  v_rcp_f64 alone is accurate to
+// ~2^-23 relative; after the Newton step the reciprocal is within ~11 ulp
+// (mean -0.6 ulp), measured over 2^20 arguments spanning 2^-40..2^40.
 template <int BPL>
 __device__ inline void tile_fast(const TileEntry* __restrict__ tile, int n, const double (&x)[BPL][3],
                                  double (&acc)[BPL][3][2]) {
@@ -125,11 +132,9 @@ __device__ inline void tile_fast(const TileEntry* __restrict__ tile, int n, cons
       const double d0 = fma(u0, u0, e.wi2), d1 = fma(u1, u1, e.wi2), d2 = fma(u2, u2, e.wi2);
       const double m01 = d0 * d1;
       const double m = m01 * d2;
+      // v_rcp_f64 is good to ~2^-23; one Newton step brings it to a few ulp
       double r = __builtin_amdgcn_rcp(m);
-      double t = fma(-m, r, 1.0);
-      r = fma(r, t, r);
-      t = fma(-m, r, 1.0);
-      r = fma(r, t, r);
+      r = fma(r, fma(-m, r, 1.0), r);
       const double r2 = r * m01;
       const double q = r * d2;
       const double r0 = q * d1, r1 = q * d0;
@@ -168,7 +173,76 @@ __device__ inline void tile_exact(const TileEntry* __restrict__ tile, int n, con
   }
 }
 
-template <int NT, int BPL>
+// FP32-bulk form: the per-(p,g) constants and the complex products in FP32,
+// sums over runs of 32 G-vectors in FP32 folded into the FP64 accumulators.
+// Used for a tile only when every denominator of it is far from the pole:
+// |wx - wr| >= max|wx|/4 for all bands, so u = wx - wr loses at most 2 bits in
+// FP32 and d, d0*d1*d2 stay inside FP32 range; other tiles take the FP64 paths.
+struct TileEntry32 {
+  float wr, wi2, Ir, Ii, a, b;
+};
+
+__device__ inline TileEntry32 entry32(const TileEntry& e) {
+  TileEntry32 f;
+  f.wr = (float)e.wr;
+  f.wi2 = (float)e.wi2;
+  f.Ir = (float)e.Ir;
+  f.Ii = (float)e.Ii;
+  f.a = (float)e.a;
+  f.b = (float)e.b;
+  return f;
+}
+
+__device__ inline bool far_from_pole(double2 w, double wxmin, double wxmax) {
+  const double lo = w.x < wxmin ? wxmin - w.x : (w.x > wxmax ? w.x - wxmax : 0.0);
+  const double hi = fmax(fabs(wxmin - w.x), fabs(wxmax - w.x));
+  const double wi2 = w.y * w.y;
+  const double scale = fmax(fabs(wxmin), fabs(wxmax));
+  return lo >= 0.25 * scale && fma(lo, lo, wi2) >= 0x1p-40 && fma(hi, hi, wi2) <= 0x1p+40;
+}
+
+template <int BPL>
+__device__ inline void tile_f32(const TileEntry32* __restrict__ tile, int n, const float (&x)[BPL][3],
+                                double (&acc)[BPL][3][2]) {
+  for (int j0 = 0; j0 < n; j0 += 32) {
+    const int j1 = min(n, j0 + 32);
+    float s[BPL][3][2];
+#pragma unroll
+    for (int bb = 0; bb < BPL; ++bb)
+#pragma unroll
+      for (int iw = 0; iw < 3; ++iw) s[bb][iw][0] = s[bb][iw][1] = 0.f;
+#pragma unroll 2
+    for (int j = j0; j < j1; ++j) {
+      const TileEntry32 e = tile[j];
+#pragma unroll
+      for (int bb = 0; bb < BPL; ++bb) {
+        const float u0 = x[bb][0] - e.wr, u1 = x[bb][1] - e.wr, u2 = x[bb][2] - e.wr;
+        const float d0 = fmaf(u0, u0, e.wi2), d1 = fmaf(u1, u1, e.wi2), d2 = fmaf(u2, u2, e.wi2);
+        const float m01 = d0 * d1;
+        const float m = m01 * d2;
+        const float r = __builtin_amdgcn_rcpf(m);
+        const float r2 = r * m01;
+        const float q = r * d2;
+        const float r0 = q * d1, r1 = q * d0;
+        s[bb][0][0] = fmaf(fmaf(e.Ir, u0, e.a), r0, s[bb][0][0]);
+        s[bb][0][1] = fmaf(fmaf(e.Ii, u0, e.b), r0, s[bb][0][1]);
+        s[bb][1][0] = fmaf(fmaf(e.Ir, u1, e.a), r1, s[bb][1][0]);
+        s[bb][1][1] = fmaf(fmaf(e.Ii, u1, e.b), r1, s[bb][1][1]);
+        s[bb][2][0] = fmaf(fmaf(e.Ir, u2, e.a), r2, s[bb][2][0]);
+        s[bb][2][1] = fmaf(fmaf(e.Ii, u2, e.b), r2, s[bb][2][1]);
+      }
+    }
+#pragma unroll
+    for (int bb = 0; bb < BPL; ++bb)
+#pragma unroll
+      for (int iw = 0; iw < 3; ++iw) {
+        acc[bb][iw][0] += (double)s[bb][iw][0];
+        acc[bb][iw][1] += (double)s[bb][iw][1];
+      }
+  }
+}
+
+template <int NT, int BPL, bool F32>
 __global__ __launch_bounds__(NT) void k_gpp_main(const double2* __restrict__ wt,
                                                  const double2* __restrict__ Ieps,
                                                  const double2* __restrict__ S,
@@ -177,6 +251,7 @@ __global__ __launch_bounds__(NT) void k_gpp_main(const double2* __restrict__ wt,
                                                  int ngpown, int ncouls, int chunk, int nchunks,
                                                  double2* __restrict__ partial) {
   __shared__ TileEntry tile[NT];
+  __shared__ TileEntry32 tile32[F32 ? NT : 1];
   const int tid = threadIdx.x;
   const int band0 = blockIdx.y * NT * BPL;
   int n1[BPL];
@@ -194,6 +269,11 @@ __global__ __launch_bounds__(NT) void k_gpp_main(const double2* __restrict__ wt,
   }
   const double wxmin = wxrange[0], wxmax = wxrange[1];
   const long nitems = (long)ngpown * nchunks;
+  float x32[BPL][3];
+#pragma unroll
+  for (int bb = 0; bb < BPL; ++bb)
+#pragma unroll
+    for (int iw = 0; iw < 3; ++iw) x32[bb][iw] = (float)x[bb][iw];
 
   for (long item = blockIdx.x; item < nitems; item += gridDim.x) {
     const int p = (int)(item / nchunks);
@@ -211,19 +291,26 @@ __global__ __launch_bounds__(NT) void k_gpp_main(const double2* __restrict__ wt,
       const int g = g0 + tid;
       bool ok;
       TileEntry e;
-      if (g < ge) {
-        e = make_entry(wt[rowoff + g], Ieps[rowoff + g], wxmin, wxmax, ok);
-      } else {  // padding: contributes exactly zero, denominators wx^2+1
-        e = make_entry(make_double2(0.0, 1.0), make_double2(0.0, 0.0), wxmin, wxmax, ok);
-      }
+      const double2 wv = g < ge ? wt[rowoff + g] : make_double2(0.0, 1.0);
+      const double2 Iv = g < ge ? Ieps[rowoff + g] : make_double2(0.0, 0.0);
+      // padding entries (g >= ge) contribute exactly zero, denominators wx^2+1
+      e = make_entry(wv, Iv, wxmin, wxmax, ok);
       __syncthreads();  // previous tile fully consumed
       tile[tid] = e;
+      bool ok32 = false;
+      if (F32) {
+        ok32 = ok && far_from_pole(wv, wxmin, wxmax);
+        tile32[tid] = entry32(e);
+      }
       const int unsafe = __syncthreads_or(!ok);
       const int n = min(NT, ge - g0);
-      if (!unsafe)
-        tile_fast<BPL>(tile, n, x, acc);
-      else
+      if (unsafe) {
         tile_exact<BPL>(tile, n, x, acc);
+      } else if (F32 && !__syncthreads_or(!ok32)) {
+        tile_f32<BPL>(tile32, n, x32, acc);
+      } else {
+        tile_fast<BPL>(tile, n, x, acc);
+      }
     }
     // fold the G-chunk sums with sch_store1(n1,p)
 #pragma unroll
@@ -249,18 +336,30 @@ __global__ __launch_bounds__(NT) void k_gpp_main(const double2* __restrict__ wt,
 }
 
 // ------------------------------------------------------------------ reductions
-// acht_n1_loc[n1][iw] = sum_w partial[w][n1][iw] in fixed w order (double-double).
-__global__ void k_reduce_bands(const double2* __restrict__ partial, int W, int nb,
-                               double2* __restrict__ acht_n1_loc, double4* __restrict__ band_dd) {
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= nb * 3) return;
+// acht_n1_loc[n1][iw] = sum_w partial[w][n1][iw] (double-double).  A block owns 16
+// consecutive (n1,iw) entries; its 16 row groups each sum a strided set of w,
+// then the groups are added in fixed order, so the result is bitwise stable.
+__global__ __launch_bounds__(256) void k_reduce_bands(const double2* __restrict__ partial, int W, int nb,
+                                                      double2* __restrict__ acht_n1_loc,
+                                                      double4* __restrict__ band_dd) {
+  __shared__ cdd sh[16][16];
+  const int e = threadIdx.x & 15, grp = threadIdx.x >> 4;
+  const int k = blockIdx.x * 16 + e;
+  const int n = nb * 3;
   cdd a = cdd_zero();
-  for (int w = 0; w < W; ++w) {
-    const double2 v = partial[(size_t)w * nb * 3 + k];
-    a = cdd_add(a, v.x, v.y);
+  if (k < n)
+    for (int w = grp; w < W; w += 16) {
+      const double2 v = partial[(size_t)w * n + k];
+      a = cdd_add(a, v.x, v.y);
+    }
+  sh[grp][e] = a;
+  __syncthreads();
+  if (grp == 0 && k < n) {
+    cdd t = sh[0][e];
+    for (int q = 1; q < 16; ++q) t = cdd_add(t, sh[q][e]);
+    if (acht_n1_loc) acht_n1_loc[k] = make_double2(t.re.hi + t.re.lo, t.im.hi + t.im.lo);
+    cdd_store(&band_dd[k], t);
   }
-  if (acht_n1_loc) acht_n1_loc[k] = make_double2(a.re.hi + a.re.lo, a.im.hi + a.im.lo);
-  cdd_store(&band_dd[k], a);
 }
 
 // achtemp[iw] = sum_n1 band_dd[n1][iw], fixed order.
@@ -329,14 +428,14 @@ __global__ __launch_bounds__(NT) void k_gpp_asx(const int* __restrict__ inv, con
   }
 }
 
-template <int NT, int BPL>
+template <int NT, int BPL, bool F32>
 int launch_main(bgw_ctx* ctx, const double2* wt, const double2* I, const double2* S,
                 const double* wxb, const double* range, int nb, int ngpown, int ncouls,
                 double2* partial_ws, int* W_out, int chunk) {
   const int nchunks = (ncouls + chunk - 1) / chunk;
   const int btiles = (nb + NT * BPL - 1) / (NT * BPL);
   int per_cu = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_gpp_main<NT, BPL>, NT, 0) != hipSuccess ||
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_gpp_main<NT, BPL, F32>, NT, 0) != hipSuccess ||
       per_cu < 1)
     per_cu = 1;
   long items = (long)ngpown * nchunks;
@@ -346,7 +445,7 @@ int launch_main(bgw_ctx* ctx, const double2* wt, const double2* I, const double2
   *W_out = (int)W;
   if (!partial_ws) return BGW_OK;  // sizing query
   KernelTimer kt(ctx, "gpp_main");
-  hipLaunchKernelGGL((k_gpp_main<NT, BPL>), dim3((unsigned)W, btiles), dim3(NT), 0, ctx->stream, wt, I, S,
+  hipLaunchKernelGGL((k_gpp_main<NT, BPL, F32>), dim3((unsigned)W, btiles), dim3(NT), 0, ctx->stream, wt, I, S,
                      wxb, range, nb, ngpown, ncouls, chunk, nchunks, partial_ws);
   return check_launch(ctx, "k_gpp_main");
 }
@@ -362,20 +461,49 @@ MainCfg pick_main(int nb) {
   return {64, 1};
 }
 
+template <bool F32>
+int run_main_t(bgw_ctx* ctx, MainCfg cfg, const double2* wt, const double2* I, const double2* S, const double* wxb,
+               const double* range, int nb, int ngpown, int ncouls, double2* ws, int* W) {
+  const int chunk = 2048;
+  if (cfg.nt == 256 && cfg.bpl == 2)
+    return launch_main<256, 2, F32>(ctx, wt, I, S, wxb, range, nb, ngpown, ncouls, ws, W, chunk);
+  if (cfg.nt == 128 && cfg.bpl == 2)
+    return launch_main<128, 2, F32>(ctx, wt, I, S, wxb, range, nb, ngpown, ncouls, ws, W, chunk);
+  if (cfg.nt == 128)
+    return launch_main<128, 1, F32>(ctx, wt, I, S, wxb, range, nb, ngpown, ncouls, ws, W, chunk);
+  return launch_main<64, 1, F32>(ctx, wt, I, S, wxb, range, nb, ngpown, ncouls, ws, W, chunk);
+}
+
 int run_main(bgw_ctx* ctx, MainCfg cfg, const double2* wt, const double2* I, const double2* S,
              const double* wxb, const double* range, int nb, int ngpown, int ncouls, double2* ws,
              int* W) {
-  const int chunk = 2048;
-  if (cfg.nt == 256 && cfg.bpl == 2)
-    return launch_main<256, 2>(ctx, wt, I, S, wxb, range, nb, ngpown, ncouls, ws, W, chunk);
-  if (cfg.nt == 128 && cfg.bpl == 2)
-    return launch_main<128, 2>(ctx, wt, I, S, wxb, range, nb, ngpown, ncouls, ws, W, chunk);
-  if (cfg.nt == 128)
-    return launch_main<128, 1>(ctx, wt, I, S, wxb, range, nb, ngpown, ncouls, ws, W, chunk);
-  return launch_main<64, 1>(ctx, wt, I, S, wxb, range, nb, ngpown, ncouls, ws, W, chunk);
+  if (ctx->precision == BGW_FP32_BULK)
+    return run_main_t<true>(ctx, cfg, wt, I, S, wxb, range, nb, ngpown, ncouls, ws, W);
+  return run_main_t<false>(ctx, cfg, wt, I, S, wxb, range, nb, ngpown, ncouls, ws, W);
 }
 
 }  // namespace
+
+// S[p][n1], C[p][n1] (sch_store1 without its wtilde(p,igp) factor), the band
+// frequencies wxb[n1][3] and their [min,max]; buffers owned by the context.
+int gpp_prep_tables(bgw_ctx* ctx, int nb, int ngpown, int ncouls, const int* inv, const int* indinv,
+                    const double* wx, int wx_per_band, const double2* wt, const double2* am, const double2* an,
+                    const double* vcoul, double2** S, double2** C, double** wxb, double** range) {
+  void *pS, *pC, *pwx, *prange;
+  int rc = pool_get(ctx, "ws_S", (size_t)nb * ngpown * 16, &pS);
+  if (!rc) rc = pool_get(ctx, "ws_C", (size_t)nb * ngpown * 16, &pC);
+  if (!rc) rc = pool_get(ctx, "ws_wxb", (size_t)nb * 3 * 8, &pwx);
+  if (!rc) rc = pool_get(ctx, "ws_range", 64, &prange);
+  if (rc) return rc;
+  *S = static_cast<double2*>(pS);
+  *C = static_cast<double2*>(pC);
+  *wxb = static_cast<double*>(pwx);
+  *range = static_cast<double*>(prange);
+  hipLaunchKernelGGL(k_gpp_prep, dim3((nb + 255) / 256, ngpown), dim3(256), 0, ctx->stream, inv, indinv, wt, am, an,
+                     vcoul, wx, wx_per_band, nb, ngpown, ncouls, *S, *C, *wxb);
+  hipLaunchKernelGGL(k_wx_range, dim3(1), dim3(256), 0, ctx->stream, *wxb, nb * 3, *range);
+  return check_launch(ctx, "k_gpp_prep");
+}
 
 // Forward driver on device pointers; enqueues on ctx->stream.
 int gpp_forward_device(bgw_ctx* ctx, size_t nb_, size_t nvband_, size_t ngpown_, size_t ncouls_,
@@ -390,32 +518,22 @@ int gpp_forward_device(bgw_ctx* ctx, size_t nb_, size_t nvband_, size_t ngpown_,
   const double2* am = reinterpret_cast<const double2*>(aqsm);
   const double2* an = reinterpret_cast<const double2*>(aqsn);
 
-  // workspace
   const MainCfg cfg = pick_main(nb);
   int W = 0;
   run_main(ctx, cfg, nullptr, nullptr, nullptr, nullptr, nullptr, nb, ngpown, ncouls, nullptr, &W);
-  void *pS, *pC, *pwx, *prange, *ppart, *pband;
-  int rc = pool_get(ctx, "ws_S", (size_t)nb * ngpown * 16, &pS);
-  if (!rc) rc = pool_get(ctx, "ws_C", (size_t)nb * ngpown * 16, &pC);
-  if (!rc) rc = pool_get(ctx, "ws_wxb", (size_t)nb * 3 * 8, &pwx);
-  if (!rc) rc = pool_get(ctx, "ws_range", 64, &prange);
-  if (!rc) rc = pool_get(ctx, "ws_partial", (size_t)W * nb * 3 * 16, &ppart);
+  void *ppart, *pband;
+  int rc = pool_get(ctx, "ws_partial", (size_t)W * nb * 3 * 16, &ppart);
   if (!rc) rc = pool_get(ctx, "ws_band_dd", (size_t)nb * 3 * 32, &pband);
   if (rc) return rc;
-  double2* S = static_cast<double2*>(pS);
-  double2* C = static_cast<double2*>(pC);
-  double* wxb = static_cast<double*>(pwx);
-  double* range = static_cast<double*>(prange);
-
   launch_check_indices(inv, ngpown_, indinv, ncouls_ + 1, ncouls, ncouls, ctx->d_flag, s);
-  hipLaunchKernelGGL(k_gpp_prep, dim3((nb + 255) / 256, ngpown), dim3(256), 0, s, inv, indinv, wt, am, an,
-                     vcoul, wx, wx_per_band, nb, ngpown, ncouls, S, C, wxb);
-  hipLaunchKernelGGL(k_wx_range, dim3(1), dim3(256), 0, s, wxb, nb * 3, range);
-  if ((rc = check_launch(ctx, "k_gpp_prep"))) return rc;
+  double2 *S, *C;
+  double *wxb, *range;
+  rc = gpp_prep_tables(ctx, nb, ngpown, ncouls, inv, indinv, wx, wx_per_band, wt, am, an, vcoul, &S, &C, &wxb, &range);
+  if (rc) return rc;
 
   rc = run_main(ctx, cfg, wt, I, S, wxb, range, nb, ngpown, ncouls, static_cast<double2*>(ppart), &W);
   if (rc) return rc;
-  hipLaunchKernelGGL(k_reduce_bands, dim3((nb * 3 + 255) / 256), dim3(256), 0, s,
+  hipLaunchKernelGGL(k_reduce_bands, dim3((nb * 3 + 15) / 16), dim3(256), 0, s,
                      static_cast<const double2*>(ppart), W, nb, reinterpret_cast<double2*>(acht_n1_loc),
                      static_cast<double4*>(pband));
   hipLaunchKernelGGL(k_reduce_achtemp, dim3(1), dim3(256), 0, s, static_cast<const double4*>(pband), nb,

@@ -1,10 +1,265 @@
-// gpp_bwd.hip -- adjoint of the GPP forward (placeholder until the kernel lands).
+// gpp_bwd.hip -- adjoint of the GPP forward for L = Re sum_iw (asxtemp[iw] + achtemp[iw]).
+//
+// With c(n1,p) = conj(aqsm(n1,igp)) aqsn(n1,igp) 0.5 vcoul(igp) (so that
+// sch_store1 = c * wtilde(p,igp), GPP/gpp.cpp:369-371) and inv = 1/(wx(n1,iw) - wtilde(p,g)):
+//   df/dI(p,g)      = wtilde(p,igp) * sum_{n1,iw} c inv               (+ SX part)
+//   df/dwtilde(p,g) = wtilde(p,igp) * I(p,g) * sum_{n1,iw} c inv^2    (+ SX part)
+//                     + [g == igp(p)] * T(p),  T(p) = sum_g I(p,g) sum_{n1,iw} c inv
+//   dSigma/domega[iw] = -sum wtilde(p,igp) I sum_n1 c inv^2           (+ SX part)
+// and dL/dz = conj(df/dz) (dL/dRe z + i dL/dIm z for L = Re f, f holomorphic).
+// The SX part (bands n1 < nvband) differentiates the branch of BerkeleyGW's ssx
+// that the forward takes for each term.
+//
+// Mapping: one lane per (p, g) (coalesced along ig), the band loop runs over LDS
+// broadcasts of c(n1,p) and wx(n1,:); the pole terms are recomputed, never stored.
+// Each (p,g) output is owned by one lane, so the G-vector outputs are written
+// directly; dSigma/domega and T(p) go through fixed-order partial reductions.
+#include <cmath>
+
+#include "bgw_dd.h"
 #include "bgw_internal.h"
+#include "contract.h"
+#include "gpp_common.h"
 
 namespace bgw {
-int gpp_backward_device(bgw_ctx* ctx, size_t, size_t, size_t, size_t, const int*, const int*, const double*,
-                        const double*, const double*, const double*, const double*, const double*, double,
-                        double*, double*, double*) {
-  return fail(ctx, BGW_EINVAL, "gpp backward not built");
+int gpp_prep_tables(bgw_ctx* ctx, int nb, int ngpown, int ncouls, const int* inv, const int* indinv,
+                    const double* wx, int wx_per_band, const double2* wt, const double2* am, const double2* an,
+                    const double* vcoul, double2** S, double2** C, double** wxb, double** range);
+
+namespace {
+
+constexpr int BT = 256;  // threads per block
+constexpr int GPT = 2;   // G-vectors per thread
+constexpr int kMaxBandsLds = 1024;
+
+// holomorphic derivatives of gpp_ssx with respect to I, wtilde and wx (branch-wise)
+__device__ inline void ssx_grad(double wx, double2 wt, double2 I, double2& dI, double2& dW, double2& dX) {
+  const double2 ssx = gpp_ssx(wx, wt, I);
+  dI = dW = dX = make_double2(0.0, 0.0);
+  if (sqrt(cnorm2(ssx)) > kSexcut * sqrt(cnorm2(I)) && wx < 0.0) return;
+  const double2 wt2 = cmul(wt, wt);
+  const double2 wdiff = make_double2(wx - wt.x, -wt.y);
+  const double wdiffr = cnorm2(wdiff);
+  const double2 delw = cscale(cmul(wt, cconj(wdiff)), 1.0 / wdiffr);  // = wt / wdiff
+  const double delwr = cnorm2(delw);
+  // 1/z = conj(z)/|z|^2
+  auto inv = [](double2 z) { return cscale(cconj(z), 1.0 / cnorm2(z)); };
+  if (wdiffr > kLimitTwo && delwr < kLimitOne) {
+    const double2 cden = make_double2(wx * wx - wt2.x, -wt2.y);
+    const double2 ic = inv(cden);
+    const double2 ic2 = cmul(ic, ic);
+    dI = cmul(wt2, ic);
+    dW = cscale(cmul(cmul(wt, I), ic2), 2.0 * wx * wx);
+    dX = cscale(cmul(cmul(wt2, I), ic2), -2.0 * wx);
+  } else if (delwr > kTo1) {
+    const double2 dp = make_double2(delw.x + 0.5, delw.y);
+    const double2 idp = inv(dp);
+    const double2 q = cmul(delw, idp);
+    const double2 dq = cscale(cmul(idp, idp), 0.5);
+    const double2 iwd = inv(wdiff);
+    const double2 iwd2 = cmul(iwd, iwd);
+    dI = cscale(q, -0.25);
+    dW = cscale(cmul(cmul(I, dq), iwd2), -0.25 * wx);
+    dX = cscale(cmul(cmul(cmul(I, dq), iwd2), wt), 0.25);
+  }
 }
+
+__global__ __launch_bounds__(BT) void k_gpp_bwd(const double2* __restrict__ wt, const double2* __restrict__ Ieps,
+                                                const double2* __restrict__ C, const double* __restrict__ wxb,
+                                                const double* __restrict__ wxrange, const int* __restrict__ inv,
+                                                const int* __restrict__ indinv, const double2* __restrict__ aqsm,
+                                                const double2* __restrict__ aqsn, const double* __restrict__ vcoul,
+                                                int nb, int nvband, int ngpown, int ncouls, double occ,
+                                                double2* __restrict__ dL_dwt, double2* __restrict__ dL_dI,
+                                                double4* __restrict__ ds_part /*[blocks][3]*/,
+                                                double4* __restrict__ T_part /*[blocks]*/) {
+  __shared__ double2 sc[kMaxBandsLds];
+  __shared__ double sx[kMaxBandsLds * 3];
+  __shared__ cdd red[BT / 64];
+  const int nchunk = (ncouls + BT * GPT - 1) / (BT * GPT);
+  const int p = blockIdx.x / nchunk;
+  const int ch = blockIdx.x - p * nchunk;
+  const int igp = gpp_igp(inv, indinv, p, ncouls);
+  const double2 wt_igp = wt[(size_t)p * ncouls + igp];
+  const double wxmin = wxrange[0], wxmax = wxrange[1];
+
+  double2 w[GPT], I[GPT];
+  bool valid[GPT], fast[GPT];
+  double G[GPT][2], H[GPT][3][2], dIs[GPT][2], dWs[GPT][2], dsx[3][2];
+#pragma unroll
+  for (int iw = 0; iw < 3; ++iw) dsx[iw][0] = dsx[iw][1] = 0.0;
+#pragma unroll
+  for (int k = 0; k < GPT; ++k) {
+    const int g = ch * BT * GPT + k * BT + threadIdx.x;
+    valid[k] = g < ncouls;
+    w[k] = valid[k] ? wt[(size_t)p * ncouls + g] : make_double2(0.0, 1.0);
+    I[k] = valid[k] ? Ieps[(size_t)p * ncouls + g] : make_double2(0.0, 0.0);
+    const double lo = w[k].x < wxmin ? wxmin - w[k].x : (w[k].x > wxmax ? w[k].x - wxmax : 0.0);
+    const double hi = fmax(fabs(wxmin - w[k].x), fabs(wxmax - w[k].x));
+    const double wi2 = w[k].y * w[k].y;
+    fast[k] = (fma(lo, lo, wi2) >= 0x1p-300) && (fma(hi, hi, wi2) <= 0x1p+300);
+    G[k][0] = G[k][1] = dIs[k][0] = dIs[k][1] = dWs[k][0] = dWs[k][1] = 0.0;
+#pragma unroll
+    for (int iw = 0; iw < 3; ++iw) H[k][iw][0] = H[k][iw][1] = 0.0;
+  }
+
+  for (int b0 = 0; b0 < nb; b0 += kMaxBandsLds) {
+    const int nbt = min(kMaxBandsLds, nb - b0);
+    __syncthreads();
+    for (int i = threadIdx.x; i < nbt; i += BT) {
+      sc[i] = C[(size_t)p * nb + b0 + i];
+      sx[3 * i] = wxb[3 * (b0 + i)];
+      sx[3 * i + 1] = wxb[3 * (b0 + i) + 1];
+      sx[3 * i + 2] = wxb[3 * (b0 + i) + 2];
+    }
+    __syncthreads();
+    for (int j = 0; j < nbt; ++j) {
+      const double2 c = sc[j];
+      const double x0 = sx[3 * j], x1 = sx[3 * j + 1], x2 = sx[3 * j + 2];
+#pragma unroll
+      for (int k = 0; k < GPT; ++k) {
+        const double wr = w[k].x, wi = w[k].y, wi2 = wi * wi;
+        const double u0 = x0 - wr, u1 = x1 - wr, u2 = x2 - wr;
+        const double d0 = fma(u0, u0, wi2), d1 = fma(u1, u1, wi2), d2 = fma(u2, u2, wi2);
+        double r0, r1, r2;
+        if (fast[k]) {
+          const double m01 = d0 * d1, m = m01 * d2;
+          double r = __builtin_amdgcn_rcp(m);
+          r = fma(r, fma(-m, r, 1.0), r);
+          r2 = r * m01;
+          const double q = r * d2;
+          r0 = q * d1;
+          r1 = q * d0;
+        } else {
+          r0 = 1.0 / d0;
+          r1 = 1.0 / d1;
+          r2 = 1.0 / d2;
+        }
+        // inv_iw = 1/(wx - wtilde) = (u, wi) * r
+        const double a0 = u0 * r0, b0_ = wi * r0, a1 = u1 * r1, b1 = wi * r1, a2 = u2 * r2, b2 = wi * r2;
+        const double sr = a0 + a1 + a2, si = b0_ + b1 + b2;
+        G[k][0] = fma(c.x, sr, fma(-c.y, si, G[k][0]));
+        G[k][1] = fma(c.x, si, fma(c.y, sr, G[k][1]));
+        // inv^2 = (a^2 - b^2, 2ab)
+        const double q0r = fma(a0, a0, -b0_ * b0_), q0i = 2.0 * a0 * b0_;
+        const double q1r = fma(a1, a1, -b1 * b1), q1i = 2.0 * a1 * b1;
+        const double q2r = fma(a2, a2, -b2 * b2), q2i = 2.0 * a2 * b2;
+        H[k][0][0] = fma(c.x, q0r, fma(-c.y, q0i, H[k][0][0]));
+        H[k][0][1] = fma(c.x, q0i, fma(c.y, q0r, H[k][0][1]));
+        H[k][1][0] = fma(c.x, q1r, fma(-c.y, q1i, H[k][1][0]));
+        H[k][1][1] = fma(c.x, q1i, fma(c.y, q1r, H[k][1][1]));
+        H[k][2][0] = fma(c.x, q2r, fma(-c.y, q2i, H[k][2][0]));
+        H[k][2][1] = fma(c.x, q2i, fma(c.y, q2r, H[k][2][1]));
+      }
+    }
+  }
+
+  // screened-exchange part, bands n1 < nvband
+  for (int n1 = 0; n1 < nvband; ++n1) {
+    const double2 am = aqsm[(size_t)n1 * ncouls + igp];
+    const double vc = vcoul[igp];
+    const double2 m = make_double2(am.x * occ * vc, -am.y * occ * vc);
+#pragma unroll
+    for (int k = 0; k < GPT; ++k) {
+      if (!valid[k]) continue;
+      const int g = ch * BT * GPT + k * BT + threadIdx.x;
+      const double2 ma = cmul(m, aqsn[(size_t)n1 * ncouls + g]);
+#pragma unroll
+      for (int iw = 0; iw < 3; ++iw) {
+        double2 gI, gW, gX;
+        ssx_grad(wxb[3 * n1 + iw], w[k], I[k], gI, gW, gX);
+        const double2 tI = cmul(ma, gI), tW = cmul(ma, gW), tX = cmul(ma, gX);
+        dIs[k][0] += tI.x;
+        dIs[k][1] += tI.y;
+        dWs[k][0] += tW.x;
+        dWs[k][1] += tW.y;
+        dsx[iw][0] += tX.x;
+        dsx[iw][1] += tX.y;
+      }
+    }
+  }
+
+  cdd ds[3] = {cdd_zero(), cdd_zero(), cdd_zero()};
+  cdd Tp = cdd_zero();
+#pragma unroll
+  for (int k = 0; k < GPT; ++k) {
+    if (!valid[k]) continue;
+    const int g = ch * BT * GPT + k * BT + threadIdx.x;
+    const double2 Gc = make_double2(G[k][0], G[k][1]);
+    const double2 dfdI = cadd(cmul(wt_igp, Gc), make_double2(dIs[k][0], dIs[k][1]));
+    const double2 Hs = make_double2(H[k][0][0] + H[k][1][0] + H[k][2][0], H[k][0][1] + H[k][1][1] + H[k][2][1]);
+    const double2 sI = cmul(wt_igp, I[k]);
+    const double2 dfdW = cadd(cmul(sI, Hs), make_double2(dWs[k][0], dWs[k][1]));
+    if (dL_dI) dL_dI[(size_t)p * ncouls + g] = cconj(dfdI);
+    if (dL_dwt) dL_dwt[(size_t)p * ncouls + g] = cconj(dfdW);
+#pragma unroll
+    for (int iw = 0; iw < 3; ++iw) {
+      const double2 t = cmul(sI, make_double2(H[k][iw][0], H[k][iw][1]));
+      ds[iw] = cdd_add(ds[iw], -t.x, -t.y);
+    }
+    const double2 t = cmul(I[k], Gc);
+    Tp = cdd_add(Tp, t.x, t.y);
+  }
+#pragma unroll
+  for (int iw = 0; iw < 3; ++iw) {
+    ds[iw] = cdd_add(ds[iw], dsx[iw][0], dsx[iw][1]);
+    cdd r = cdd_block_reduce(ds[iw], red);
+    if (threadIdx.x == 0) cdd_store(&ds_part[(size_t)blockIdx.x * 3 + iw], r);
+    __syncthreads();
+  }
+  cdd r = cdd_block_reduce(Tp, red);
+  if (threadIdx.x == 0) cdd_store(&T_part[blockIdx.x], r);
+}
+
+// dL/dwtilde(p, igp(p)) += conj(T(p)), T(p) = sum over the G-chunks of p (fixed order).
+__global__ void k_gpp_bwd_T(const double4* __restrict__ T_part, int nchunk, const int* __restrict__ inv,
+                            const int* __restrict__ indinv, int ngpown, int ncouls, double2* __restrict__ dL_dwt) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= ngpown) return;
+  cdd t = cdd_zero();
+  for (int c = 0; c < nchunk; ++c) t = cdd_add(t, cdd_load(&T_part[(size_t)p * nchunk + c]));
+  const int igp = gpp_igp(inv, indinv, p, ncouls);
+  double2 v = dL_dwt[(size_t)p * ncouls + igp];
+  v.x += t.re.hi + t.re.lo;
+  v.y -= t.im.hi + t.im.lo;
+  dL_dwt[(size_t)p * ncouls + igp] = v;
+}
+
+}  // namespace
+
+int gpp_backward_device(bgw_ctx* ctx, size_t nb_, size_t nvband_, size_t ngpown_, size_t ncouls_, const int* inv,
+                        const int* indinv, const double* wx, const double* wtilde, const double* aqsm,
+                        const double* aqsn, const double* I_eps, const double* vcoul, double occ, double* dsigma,
+                        double* dwt, double* dI) {
+  const int nb = (int)nb_, nvband = (int)nvband_, ngpown = (int)ngpown_, ncouls = (int)ncouls_;
+  hipStream_t s = ctx->stream;
+  const double2* wt = reinterpret_cast<const double2*>(wtilde);
+  const double2* I = reinterpret_cast<const double2*>(I_eps);
+  const double2* am = reinterpret_cast<const double2*>(aqsm);
+  const double2* an = reinterpret_cast<const double2*>(aqsn);
+  double2 *S, *C;
+  double *wxb, *range;
+  launch_check_indices(inv, ngpown_, indinv, ncouls_ + 1, ncouls, ncouls, ctx->d_flag, s);
+  int rc = gpp_prep_tables(ctx, nb, ngpown, ncouls, inv, indinv, wx, 1, wt, am, an, vcoul, &S, &C, &wxb, &range);
+  if (rc) return rc;
+  const int nchunk = (ncouls + BT * GPT - 1) / (BT * GPT);
+  const size_t blocks = (size_t)ngpown * nchunk;
+  void *pds, *pT;
+  if ((rc = pool_get(ctx, "ws_bwd_ds", blocks * 3 * 32, &pds))) return rc;
+  if ((rc = pool_get(ctx, "ws_bwd_T", blocks * 32, &pT))) return rc;
+  {
+    KernelTimer kt(ctx, "gpp_bwd");
+    hipLaunchKernelGGL(k_gpp_bwd, dim3((unsigned)blocks), dim3(BT), 0, s, wt, I, C, wxb, range, inv, indinv, am, an,
+                       vcoul, nb, nvband, ngpown, ncouls, occ, reinterpret_cast<double2*>(dwt),
+                       reinterpret_cast<double2*>(dI), static_cast<double4*>(pds), static_cast<double4*>(pT));
+  }
+  if ((rc = check_launch(ctx, "k_gpp_bwd"))) return rc;
+  if (dwt) {
+    hipLaunchKernelGGL(k_gpp_bwd_T, dim3((ngpown + 255) / 256), dim3(256), 0, s, static_cast<const double4*>(pT),
+                       nchunk, inv, indinv, ngpown, ncouls, reinterpret_cast<double2*>(dwt));
+  }
+  if (dsigma) launch_reduce_cdd(static_cast<const double4*>(pds), (int)blocks, 3, reinterpret_cast<double2*>(dsigma), s);
+  return check_launch(ctx, "k_gpp_bwd_T");
+}
+
 }  // namespace bgw

@@ -1,0 +1,89 @@
+// bgw_reference_api.hpp -- the reference mini-apps' kernel API on top of libbgw.
+//
+// Same function names, argument order and meaning as the reference:
+//   noflagOCC_solver      common/Defines.h:50-62, GPP/gpp.cpp:307-312
+//   achsDtemp_Kernel      FF/ff.cpp:171-175
+//   asxDtemp_Kernel       FF/ff.cpp:211-217
+//   achDtemp_cor_Kernel   FF/ff.cpp:295-302
+// Array arguments are any ArrayMD-like container exposing `dptr` (row-major,
+// std::complex<double> or a layout-compatible complex type), so the reference's
+// own ArrayMD<..., Device::cpu> objects can be passed unchanged.  The calls are
+// synchronous like the reference; failures throw bgw::Error (the reference has no
+// error channel).  One process-wide context on device 0 is created on first use.
+#pragma once
+
+#include <stdexcept>
+#include <string>
+
+#include "bgw.h"
+
+namespace bgw {
+
+struct Error : std::runtime_error {
+  int code;
+  Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+inline bgw_ctx* default_context() {
+  static bgw_ctx* ctx = [] {
+    bgw_ctx* c = nullptr;
+    int rc = bgw_create(&c, 0);
+    if (rc != BGW_OK) throw Error(rc, "bgw_create failed: no usable GPU");
+    return c;
+  }();
+  return ctx;
+}
+
+inline void check(int rc) {
+  if (rc != BGW_OK) throw Error(rc, bgw_last_error(default_context()));
+}
+
+template <class A>
+inline const double* cptr(A& a) {
+  return reinterpret_cast<const double*>(a.dptr);
+}
+template <class A>
+inline double* mptr(A& a) {
+  return reinterpret_cast<double*>(a.dptr);
+}
+
+}  // namespace bgw
+
+template <class AInt, class AReal, class ACplx2, class ACplx1>
+void noflagOCC_solver(size_t number_bands, size_t ngpown, size_t ncouls, AInt& inv_igp_index, AInt& indinv,
+                      AReal& wx_array, ACplx2& wtilde_array, ACplx2& aqsmtemp, ACplx2& aqsntemp,
+                      ACplx2& I_eps_array, AReal& vcoul, ACplx1& achtemp) {
+  bgw::check(bgw_noflagOCC_solver(bgw::default_context(), number_bands, ngpown, ncouls, inv_igp_index.dptr,
+                                  indinv.dptr, wx_array.dptr, bgw::cptr(wtilde_array), bgw::cptr(aqsmtemp),
+                                  bgw::cptr(aqsntemp), bgw::cptr(I_eps_array), vcoul.dptr, bgw::mptr(achtemp)));
+}
+
+template <class AInt, class ACplx2, class ACplx3, class AReal, class C>
+void achsDtemp_Kernel(int number_bands, int ngpown, int ncouls, AInt& inv_igp_index, AInt& indinv,
+                      ACplx2& aqsntemp, ACplx2& aqsmtemp, ACplx3& I_epsR_array, AReal& vcoul, C& achsDtemp) {
+  bgw::check(bgw_achsDtemp_Kernel(bgw::default_context(), number_bands, ngpown, ncouls, inv_igp_index.dptr,
+                                  indinv.dptr, bgw::cptr(aqsntemp), bgw::cptr(aqsmtemp), bgw::cptr(I_epsR_array),
+                                  vcoul.dptr, reinterpret_cast<double*>(&achsDtemp)));
+}
+
+template <class AReal, class AInt, class ACplx2, class ACplx3, class ACplx1>
+void asxDtemp_Kernel(int nvband, int nfreqeval, int ncouls, int ngpown, int nFreq, double freqevalmin,
+                     double freqevalstep, double occ, AReal& ekq, AReal& dFreqGrid, AInt& inv_igp_index,
+                     AInt& indinv, ACplx2& aqsmtemp, ACplx2& aqsntemp, AReal& vcoul, ACplx3& I_epsR_array,
+                     ACplx3& I_epsA_array, ACplx1& asxDtemp) {
+  bgw::check(bgw_asxDtemp_Kernel(bgw::default_context(), nvband, nfreqeval, ncouls, ngpown, nFreq, freqevalmin,
+                                 freqevalstep, occ, ekq.dptr, dFreqGrid.dptr, inv_igp_index.dptr, indinv.dptr,
+                                 bgw::cptr(aqsmtemp), bgw::cptr(aqsntemp), vcoul.dptr, bgw::cptr(I_epsR_array),
+                                 bgw::cptr(I_epsA_array), bgw::mptr(asxDtemp)));
+}
+
+template <class AReal, class AInt, class ACplx2, class ACplx3, class ACplx1>
+void achDtemp_cor_Kernel(int number_bands, int nvband, int nfreqeval, int ncouls, int ngpown, int nFreq,
+                         double freqevalmin, double freqevalstep, AReal& ekq, AReal& dFreqGrid,
+                         AInt& inv_igp_index, AInt& indinv, ACplx2& aqsmtemp, ACplx2& aqsntemp, AReal& vcoul,
+                         ACplx3& I_epsR_array, ACplx3& I_epsA_array, ACplx1& achDtemp_cor) {
+  bgw::check(bgw_achDtemp_cor_Kernel(bgw::default_context(), number_bands, nvband, nfreqeval, ncouls, ngpown, nFreq,
+                                     freqevalmin, freqevalstep, ekq.dptr, dFreqGrid.dptr, inv_igp_index.dptr,
+                                     indinv.dptr, bgw::cptr(aqsmtemp), bgw::cptr(aqsntemp), vcoul.dptr,
+                                     bgw::cptr(I_epsR_array), bgw::cptr(I_epsA_array), bgw::mptr(achDtemp_cor)));
+}

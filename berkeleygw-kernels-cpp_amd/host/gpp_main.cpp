@@ -1,0 +1,162 @@
+// gpp_main.cpp -- GPP driver with the reference's command line
+// (GPP/gpp.cpp:133-305): `gpp.ex [number_bands nvband ncouls nodes_per_group]`,
+// defaults 512 2 32768 20, ngpown = ncouls / (nodes_per_group * npes), npes = 1.
+//
+// Inputs are the reference's constants (GPP/gpp.cpp:246-276) unless --random SEED
+// selects the seeded benchmark distributions.  The one-sided golden check of the
+// reference (GPP/gpp.cpp:117-131) is replaced by a two-sided check against the
+// exact closed form of the constant inputs.  Extra flags: --random SEED,
+// --precision fp64|fp32, --all (also prints acht_n1_loc sum, achstemp, asxtemp).
+#include <chrono>
+#include <cmath>
+#include <complex>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "bgw.h"
+#include "bgw_reference_api.hpp"
+#include "host_array.hpp"
+
+using cplx = std::complex<double>;
+using clk = std::chrono::steady_clock;
+
+static void usage() {
+  std::printf("The correct form of input is :\n ./gpp.ex <number_bands> <number_valence_bands> "
+              "<number_plane_waves> <nodes_per_mpi_group> [--random SEED] [--precision fp64|fp32] [--all]\n");
+}
+
+int main(int argc, char** argv) {
+  std::vector<std::string> pos;
+  long long seed = -1;
+  int precision = BGW_FP64;
+  bool all = false;
+  for (int i = 1; i < argc; ++i) {
+    std::string a = argv[i];
+    if (a == "--random" && i + 1 < argc) {
+      seed = std::atoll(argv[++i]);
+    } else if (a == "--precision" && i + 1 < argc) {
+      std::string p = argv[++i];
+      precision = p == "fp32" ? BGW_FP32_BULK : BGW_FP64;
+    } else if (a == "--all") {
+      all = true;
+    } else if (a == "-h" || a == "--help") {
+      usage();
+      return 0;
+    } else {
+      pos.push_back(a);
+    }
+  }
+  int number_bands = 512, nvband = 2, ncouls = 32768, nodes_per_group = 20;
+  const int npes = 1;
+  if (pos.size() == 4) {
+    number_bands = std::atoi(pos[0].c_str());
+    nvband = std::atoi(pos[1].c_str());
+    ncouls = std::atoi(pos[2].c_str());
+    nodes_per_group = std::atoi(pos[3].c_str());
+  } else if (!pos.empty()) {
+    usage();
+    return 1;
+  }
+  if (number_bands <= 0 || ncouls <= 0 || nodes_per_group <= 0 || nvband < 0 || nvband > number_bands) {
+    usage();
+    return 1;
+  }
+  const int ngpown = ncouls / (nodes_per_group * npes);
+  if (ngpown <= 0) {
+    usage();
+    return 1;
+  }
+  const double e_lk = 10, dw = 1, to1 = 1e-6, e_n1kq = 6.0;
+  auto t_start = clk::now();
+
+  std::printf("number_bands = %d\t nvband = %d\t ncouls = %d\t nodes_per_group  = %d\t ngpown = %d\t nend = 3\t "
+              "nstart = 0\n", number_bands, nvband, ncouls, nodes_per_group, ngpown);
+  HostArray<cplx, 2> aqsmtemp(number_bands, ncouls), aqsntemp(number_bands, ncouls);
+  HostArray<cplx, 2> I_eps_array(ngpown, ncouls), wtilde_array(ngpown, ncouls);
+  HostArray<double, 1> vcoul(ncouls), wx3(3), wx_all(number_bands * 3);
+  HostArray<int, 1> inv_igp_index(ngpown), indinv(ncouls + 1);
+  HostArray<cplx, 1> achtemp(3);
+  const double mem = (2.0 * number_bands * ncouls + 2.0 * ngpown * ncouls) * 16 + ncouls * 8.0 + ngpown * 4.0 +
+                     (ncouls + 1) * 4.0;
+  std::printf("Memory Foot Print = %g GBs\n", mem / std::pow(1024, 3));
+
+  if (seed < 0) {  // reference constants
+    const cplx expr(0.5, 0.5);
+    for (size_t i = 0; i < aqsmtemp.size; ++i) aqsmtemp.dptr[i] = aqsntemp.dptr[i] = expr;
+    for (size_t i = 0; i < I_eps_array.size; ++i) I_eps_array.dptr[i] = wtilde_array.dptr[i] = expr;
+    for (int i = 0; i < ncouls; ++i) vcoul(i) = 1.0;
+    for (int p = 0; p < ngpown; ++p) inv_igp_index(p) = (int)((long long)(p + 1) * ncouls / ngpown);
+    for (int g = 0; g < ncouls; ++g) indinv(g) = g;
+    indinv(ncouls) = ncouls - 1;
+    for (int iw = 0; iw < 3; ++iw) {
+      wx3(iw) = e_lk - e_n1kq + dw * ((iw + 1) - 2);
+      if (wx3(iw) < to1) wx3(iw) = to1;
+    }
+    for (int n1 = 0; n1 < number_bands; ++n1)
+      for (int iw = 0; iw < 3; ++iw) wx_all(n1 * 3 + iw) = wx3(iw);
+  } else {  // benchmark distributions (same streams as the Python tooling)
+    const unsigned long long s = (unsigned long long)seed;
+    bgw_synth_complex(s + 1, -1, 1, -1, 1, reinterpret_cast<double*>(aqsmtemp.dptr), aqsmtemp.size);
+    bgw_synth_complex(s + 2, -1, 1, -1, 1, reinterpret_cast<double*>(aqsntemp.dptr), aqsntemp.size);
+    bgw_synth_complex(s + 3, 0, 0.5, -0.05, 0.05, reinterpret_cast<double*>(I_eps_array.dptr), I_eps_array.size);
+    bgw_synth_complex(s + 4, 0.5, 3.0, -0.1, 0.0, reinterpret_cast<double*>(wtilde_array.dptr), wtilde_array.size);
+    bgw_synth_uniform(s + 5, 0.1, 1.0, vcoul.dptr, vcoul.size);
+    bgw_synth_index(s + 6, ncouls, inv_igp_index.dptr, inv_igp_index.size);
+    bgw_synth_permutation(s + 7, indinv.dptr, ncouls);
+    indinv(ncouls) = ncouls - 1;
+    std::vector<double> ek(number_bands);
+    bgw_synth_uniform(s + 8, -1, 1, ek.data(), ek.size());
+    for (int n1 = 0; n1 < number_bands; ++n1)
+      for (int iw = 0; iw < 3; ++iw) {
+        double w = e_lk - ek[n1] + dw * (iw - 1);
+        wx_all(n1 * 3 + iw) = w < to1 ? to1 : w;
+      }
+    for (int iw = 0; iw < 3; ++iw) wx3(iw) = wx_all(iw);
+  }
+
+  bgw::check(bgw_set_precision(bgw::default_context(), precision));
+  auto k_start = clk::now();
+  if (seed < 0) {
+    noflagOCC_solver(number_bands, ngpown, ncouls, inv_igp_index, indinv, wx3, wtilde_array, aqsmtemp, aqsntemp,
+                     I_eps_array, vcoul, achtemp);
+  } else {
+    bgw::check(bgw_gpp_solver(bgw::default_context(), number_bands, nvband, ngpown, ncouls, inv_igp_index.dptr,
+                              indinv.dptr, wx_all.dptr, bgw::cptr(wtilde_array), bgw::cptr(aqsmtemp),
+                              bgw::cptr(aqsntemp), bgw::cptr(I_eps_array), vcoul.dptr, 1.0,
+                              bgw::mptr(achtemp), nullptr, nullptr, nullptr));
+  }
+  auto k_end = clk::now();
+  const double kernel_s = std::chrono::duration<double>(k_end - k_start).count();
+
+  if (seed < 0) {
+    // exact value for the constant inputs: nb*ngpown*ncouls * 0.125i / (wx - 0.5 - 0.5i)
+    const cplx cf = (double)number_bands * ngpown * ncouls * cplx(0.0, 0.125) / (wx3(0) - cplx(0.5, 0.5));
+    const double rel = std::abs(achtemp(0) - cf) / std::abs(cf);
+    const double tol = precision == BGW_FP64 ? 1e-12 : 1e-6;
+    if (rel < tol)
+      std::printf("\n!!!! SUCCESS - !!!! Correctness test passed :-D :-D  (rel err %.3e vs exact)\n\n", rel);
+    else
+      std::printf("\n!!!! FAILURE - Correctness test failed :-( :-(  (rel err %.3e vs exact)\n", rel);
+  }
+  std::printf("\n Final achtemp\n(%f,%f) \n", achtemp(0).real(), achtemp(0).imag());
+  if (all) {
+    std::vector<cplx> loc(number_bands * 3);
+    cplx achs, asx[3];
+    bgw::check(bgw_gpp_solver(bgw::default_context(), number_bands, nvband, ngpown, ncouls, inv_igp_index.dptr,
+                              indinv.dptr, wx_all.dptr, bgw::cptr(wtilde_array), bgw::cptr(aqsmtemp),
+                              bgw::cptr(aqsntemp), bgw::cptr(I_eps_array), vcoul.dptr, 1.0, bgw::mptr(achtemp),
+                              reinterpret_cast<double*>(loc.data()), reinterpret_cast<double*>(&achs),
+                              reinterpret_cast<double*>(asx)));
+    cplx sum = 0;
+    for (int n1 = 0; n1 < number_bands; ++n1) sum += loc[n1 * 3];
+    std::printf(" sum_n1 acht_n1_loc[n1][0] = (%f,%f)\n achstemp = (%f,%f)\n asxtemp[0] = (%f,%f)\n", sum.real(),
+                sum.imag(), achs.real(), achs.imag(), asx[0].real(), asx[0].imag());
+  }
+  const double total_s = std::chrono::duration<double>(clk::now() - t_start).count();
+  std::printf("********** Kernel Time Taken **********= %g secs\n", kernel_s);
+  std::printf("********** Total Time Taken **********= %g secs\n", total_s);
+  return 0;
+}

@@ -31,12 +31,24 @@ class BgwError(RuntimeError):
         self.code = code
 
 
+def _share_torch_hip_runtime() -> None:
+    """PyTorch-ROCm ships its own libamdhip64 with the same soname as the system
+    one.  Loading torch first makes libbgw bind to that already-loaded runtime, so
+    the process has one HIP runtime and torch tensors/streams are valid in libbgw
+    calls.  Without torch, libbgw uses the system runtime."""
+    try:
+        import torch  # noqa: F401
+    except Exception:
+        pass
+
+
 def lib() -> C.CDLL:
     """Loads build/libbgw.so (raises if it has not been built)."""
     global _lib
     if _lib is None:
         if not LIB_PATH.exists():
             raise FileNotFoundError(f"{LIB_PATH} not built; run __graft_entry__.build()")
+        _share_torch_hip_runtime()
         _lib = C.CDLL(str(LIB_PATH))
         _declare(_lib)
     return _lib

@@ -1,0 +1,79 @@
+"""FP32-bulk precision form of the GPP forward (bgw_set_precision(BGW_FP32_BULK)):
+bulk complex products in FP32, FP64 pole denominators near poles and FP64
+reductions.  Target: <= 1e-6 relative error per output vs the FP64 oracle."""
+import numpy as np
+import pytest
+
+import bgw
+import oracle as O
+from conftest import rel_err
+
+pytestmark = pytest.mark.gpu
+TOL32 = 1e-6
+
+
+@pytest.fixture
+def ctx32(ctx):
+    ctx.set_precision(bgw.BGW_FP32_BULK)
+    yield ctx
+    ctx.set_precision(bgw.BGW_FP64)
+
+
+def test_config0_fp32_bulk_vs_oracle(ctx32):
+    d = bgw.gpp_config_inputs(0)
+    r = ctx32.gpp_solver(d)
+    o = O.gpp_forward(d)
+    assert rel_err(r["achtemp"], o["achtemp"]) < TOL32
+    assert rel_err(r["acht_n1_loc"], o["acht_n1_loc"]) < TOL32
+    assert rel_err(r["achstemp"], o["achstemp"]) < TOL32  # split-FP32 MFMA contraction
+    assert rel_err(r["asxtemp"], o["asxtemp"]) < TOL32
+    # and it is not silently the FP64 path: the error is visibly above FP64 rounding
+    assert rel_err(r["achtemp"], o["achtemp"]) > 1e-13
+
+
+@pytest.mark.parametrize("nb", [5, 130, 300])
+def test_band_counts_fp32_bulk(ctx32, nb):
+    d = bgw.gpp_inputs(nb, 2, 1500, 21, 40 + nb)
+    r = ctx32.gpp_solver(d, outputs=("achtemp", "acht_n1_loc"))
+    o = O.gpp_forward(d, want=("achtemp", "acht_n1_loc"))
+    assert rel_err(r["acht_n1_loc"], o["acht_n1_loc"]) < TOL32
+
+
+def test_near_pole_tiles_use_fp64(ctx32):
+    # wtilde close to a band frequency: the FP32 guard sends those tiles to FP64
+    d = bgw.gpp_inputs(16, 2, 2000, 6, 77)
+    wx = d["wx"].reshape(16, 3)
+    wt = d["wtilde"].reshape(6, 2000)
+    wt[:, ::97] = wx[5, 1] + 1e-3 - 1e-4j
+    d["wtilde"] = wt.reshape(-1)
+    r = ctx32.gpp_solver(d, outputs=("achtemp", "acht_n1_loc"))
+    o = O.gpp_forward(d, want=("achtemp", "acht_n1_loc"))
+    assert rel_err(r["acht_n1_loc"], o["acht_n1_loc"]) < TOL32
+
+
+@pytest.mark.slow
+def test_config2_reduced_bands_fp32_bulk(ctx32):
+    d = bgw.gpp_config_inputs(2, nb=12)
+    r = ctx32.gpp_solver(d, outputs=("achtemp", "acht_n1_loc"))
+    o = O.gpp_forward(d, want=("achtemp", "acht_n1_loc"))
+    assert rel_err(r["achtemp"], o["achtemp"]) < TOL32
+    assert rel_err(r["acht_n1_loc"], o["acht_n1_loc"]) < TOL32
+
+
+def test_ff_split_fp32_vs_oracle(ctx32):
+    f = bgw.ff_inputs(130, 5, 1000, 65, 15, 3, 1030)
+    achs = ctx32.achsDtemp_Kernel(f)
+    asx = ctx32.asxDtemp_Kernel(f)
+    cor = ctx32.achDtemp_cor_Kernel(f)
+    assert abs(achs - O.achsDtemp(f)) <= TOL32 * abs(O.achsDtemp(f))
+    assert rel_err(asx, O.asxDtemp(f)) < TOL32
+    assert rel_err(cor, O.achDtemp_cor(f)) < TOL32
+
+
+@pytest.mark.slow
+def test_ff_config4_shape_split_fp32(ctx32):
+    f = bgw.ff_config4_inputs(ncouls=2048)
+    achs = ctx32.achsDtemp_Kernel(f)
+    cor = ctx32.achDtemp_cor_Kernel(f)
+    assert abs(achs - O.achsDtemp(f)) <= TOL32 * abs(O.achsDtemp(f))
+    assert rel_err(cor, O.achDtemp_cor(f)) < TOL32
