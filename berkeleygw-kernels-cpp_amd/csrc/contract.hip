@@ -106,8 +106,19 @@ __global__ __launch_bounds__(NT) void k_contract(ContractArgs a) {
     bbase[i] = (size_t)(bok[i] ? gc : 0) * K;
   }
 
-  for (int kb = k0; kb < k1; kb += BK) {
-    double2 av[4], bv[4];
+  // 16x16 blocks of this wave that hold at least one live row / column: the
+  // others (row-tile tails of short band lists, the last G' tile) skip their MFMAs
+  bool live_r[2], live_c[2];
+#pragma unroll
+  for (int i = 0; i < 2; ++i) {
+    live_r[i] = rt * BM + wr + i * 16 < nrows;
+    live_c[i] = ct * BN + wc + i * 16 < a.N;
+  }
+
+  // software pipeline: the global loads of stage kb+BK are in flight while the
+  // MFMAs of stage kb run
+  double2 av[4], bv[4];
+  auto fetch = [&](int kb) {
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
       const int idx = tid + NT * i;
@@ -117,6 +128,9 @@ __global__ __launch_bounds__(NT) void k_contract(ContractArgs a) {
       av[i] = (aok[i] && kin) ? a.A[(size_t)arow[i] * K + k] : make_double2(0.0, 0.0);
       bv[i] = (bok[i] && kin) ? load_b<MODE>(X, Y, bbase[i] + k, a.thr2) : make_double2(0.0, 0.0);
     }
+  };
+  if (k0 < k1) fetch(k0);
+  for (int kb = k0; kb < k1; kb += BK) {
     __syncthreads();
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
@@ -128,6 +142,7 @@ __global__ __launch_bounds__(NT) void k_contract(ContractArgs a) {
       Bs[1][kl][r] = (T)bv[i].y;
     }
     __syncthreads();
+    if (kb + BK < k1) fetch(kb + BK);
     if constexpr (!F32) {
 #pragma unroll
       for (int kk = 0; kk < BK; kk += 4) {
@@ -145,6 +160,7 @@ __global__ __launch_bounds__(NT) void k_contract(ContractArgs a) {
         for (int i = 0; i < 2; ++i)
 #pragma unroll
           for (int j = 0; j < 2; ++j) {
+            if (!live_r[i] || !live_c[j]) continue;
             dr[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[i], br[j], dr[i][j], 0, 0, 0);
             dr[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(nai[i], bi[j], dr[i][j], 0, 0, 0);
             di[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[i], bi[j], di[i][j], 0, 0, 0);
@@ -176,6 +192,7 @@ __global__ __launch_bounds__(NT) void k_contract(ContractArgs a) {
         for (int i = 0; i < 2; ++i)
 #pragma unroll
           for (int j = 0; j < 2; ++j) {
+            if (!live_r[i] || !live_c[j]) continue;
             sr[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(ar[i], br[j], sr[i][j], 0, 0, 0);
             sr[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(nai[i], bi[j], sr[i][j], 0, 0, 0);
             si[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(ar[i], bi[j], si[i][j], 0, 0, 0);

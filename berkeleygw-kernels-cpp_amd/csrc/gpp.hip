@@ -96,10 +96,37 @@ struct __align__(16) TileEntry {
   double wr, wi2, Ir, Ii, a, b;
 };
 
-// Bounds of the pole denominators over all bands: every d in [2^-300, 2^300]
-// keeps the batched product d0*d1*d2 and its reciprocal normal.
-constexpr double kDMin = 0x1p-300;
-constexpr double kDMax = 0x1p+300;
+// Bounds of the pole denominators over all bands: every d in [2^-150, 2^150]
+// keeps the batched products of up to six denominators and their reciprocal normal.
+constexpr double kDMin = 0x1p-150;
+constexpr double kDMax = 0x1p+150;
+
+// Reciprocals of the N denominators of one lane from a single hardware
+// reciprocal (batched inversion): prefix products, one v_rcp of the total,
+// then back-substitution.  FP64 adds one Newton step: v_rcp_f64 alone is good to
+// ~2^-23 relative, after the step the reciprocal is within ~11 ulp (mean -0.6
+// ulp), measured over 2^20 arguments spanning 2^-40..2^40.  The products stay
+// normal because every denominator is inside the tile guard's range.
+__device__ inline double hw_rcp(double m) {
+  const double r = __builtin_amdgcn_rcp(m);
+  return fma(r, fma(-m, r, 1.0), r);
+}
+__device__ inline float hw_rcp(float m) { return __builtin_amdgcn_rcpf(m); }
+
+template <typename T, int N>
+__device__ inline void batch_inverse(const T (&d)[N], T (&r)[N]) {
+  T p[N];
+  p[0] = d[0];
+#pragma unroll
+  for (int i = 1; i < N; ++i) p[i] = p[i - 1] * d[i];
+  T inv = hw_rcp(p[N - 1]);
+#pragma unroll
+  for (int i = N - 1; i > 0; --i) {
+    r[i] = inv * p[i - 1];
+    inv = inv * d[i];
+  }
+  r[0] = inv;
+}
 
 __device__ inline TileEntry make_entry(double2 w, double2 I, double wxmin, double wxmax, bool& ok) {
   TileEntry e;
@@ -116,40 +143,31 @@ __device__ inline TileEntry make_entry(double2 w, double2 I, double wxmin, doubl
   return e;
 }
 
-// Fast path: three reciprocals from one v_rcp_f64 plus one Newton step on the
-// product (batched inversion); all in FP64.  v_rcp_f64 alone is accurate to
-// ~2^-23 relative; after the Newton step the reciprocal is within ~11 ulp
-// (mean -0.6 ulp), measured over 2^20 arguments spanning 2^-40..2^40.
+// Fast path, all FP64: the 3*BPL reciprocals of a lane's (p,g) terms come from
+// one batched inversion.
 template <int BPL>
 __device__ inline void tile_fast(const TileEntry* __restrict__ tile, int n, const double (&x)[BPL][3],
                                  double (&acc)[BPL][3][2]) {
 #pragma unroll 2
   for (int j = 0; j < n; ++j) {
     const TileEntry e = tile[j];
+    double u[BPL * 3], d[BPL * 3], r[BPL * 3];
 #pragma unroll
-    for (int bb = 0; bb < BPL; ++bb) {
-      const double u0 = x[bb][0] - e.wr, u1 = x[bb][1] - e.wr, u2 = x[bb][2] - e.wr;
-      const double d0 = fma(u0, u0, e.wi2), d1 = fma(u1, u1, e.wi2), d2 = fma(u2, u2, e.wi2);
-      const double m01 = d0 * d1;
-      const double m = m01 * d2;
-      // v_rcp_f64 is good to ~2^-23; one Newton step brings it to a few ulp
-      double r = __builtin_amdgcn_rcp(m);
-      r = fma(r, fma(-m, r, 1.0), r);
-      const double r2 = r * m01;
-      const double q = r * d2;
-      const double r0 = q * d1, r1 = q * d0;
-      acc[bb][0][0] = fma(fma(e.Ir, u0, e.a), r0, acc[bb][0][0]);
-      acc[bb][0][1] = fma(fma(e.Ii, u0, e.b), r0, acc[bb][0][1]);
-      acc[bb][1][0] = fma(fma(e.Ir, u1, e.a), r1, acc[bb][1][0]);
-      acc[bb][1][1] = fma(fma(e.Ii, u1, e.b), r1, acc[bb][1][1]);
-      acc[bb][2][0] = fma(fma(e.Ir, u2, e.a), r2, acc[bb][2][0]);
-      acc[bb][2][1] = fma(fma(e.Ii, u2, e.b), r2, acc[bb][2][1]);
+    for (int k = 0; k < BPL * 3; ++k) {
+      u[k] = x[k / 3][k % 3] - e.wr;
+      d[k] = fma(u[k], u[k], e.wi2);
+    }
+    batch_inverse(d, r);
+#pragma unroll
+    for (int k = 0; k < BPL * 3; ++k) {
+      acc[k / 3][k % 3][0] = fma(fma(e.Ir, u[k], e.a), r[k], acc[k / 3][k % 3][0]);
+      acc[k / 3][k % 3][1] = fma(fma(e.Ii, u[k], e.b), r[k], acc[k / 3][k % 3][1]);
     }
   }
 }
 
 // Reference-arithmetic path (GPP/gpp.cpp:374-383): IEEE division per term.  Used
-// for a tile whose denominators are not all safely inside [2^-300, 2^300]
+// for a tile whose denominators are not all safely inside [2^-150, 2^150]
 // (poles, zeros, NaN/Inf), so such inputs give what the reference gives.
 template <int BPL>
 __device__ inline void tile_exact(const TileEntry* __restrict__ tile, int n, const double (&x)[BPL][3],
@@ -177,7 +195,8 @@ __device__ inline void tile_exact(const TileEntry* __restrict__ tile, int n, con
 // sums over runs of 32 G-vectors in FP32 folded into the FP64 accumulators.
 // Used for a tile only when every denominator of it is far from the pole:
 // |wx - wr| >= max|wx|/4 for all bands, so u = wx - wr loses at most 2 bits in
-// FP32 and d, d0*d1*d2 stay inside FP32 range; other tiles take the FP64 paths.
+// FP32, and every d in [2^-20, 2^20], so products of six stay inside FP32 range;
+// other tiles take the FP64 paths.
 struct TileEntry32 {
   float wr, wi2, Ir, Ii, a, b;
 };
@@ -198,7 +217,7 @@ __device__ inline bool far_from_pole(double2 w, double wxmin, double wxmax) {
   const double hi = fmax(fabs(wxmin - w.x), fabs(wxmax - w.x));
   const double wi2 = w.y * w.y;
   const double scale = fmax(fabs(wxmin), fabs(wxmax));
-  return lo >= 0.25 * scale && fma(lo, lo, wi2) >= 0x1p-40 && fma(hi, hi, wi2) <= 0x1p+40;
+  return lo >= 0.25 * scale && fma(lo, lo, wi2) >= 0x1p-20 && fma(hi, hi, wi2) <= 0x1p+20;
 }
 
 template <int BPL>
@@ -214,22 +233,17 @@ __device__ inline void tile_f32(const TileEntry32* __restrict__ tile, int n, con
 #pragma unroll 2
     for (int j = j0; j < j1; ++j) {
       const TileEntry32 e = tile[j];
+      float u[BPL * 3], d[BPL * 3], r[BPL * 3];
 #pragma unroll
-      for (int bb = 0; bb < BPL; ++bb) {
-        const float u0 = x[bb][0] - e.wr, u1 = x[bb][1] - e.wr, u2 = x[bb][2] - e.wr;
-        const float d0 = fmaf(u0, u0, e.wi2), d1 = fmaf(u1, u1, e.wi2), d2 = fmaf(u2, u2, e.wi2);
-        const float m01 = d0 * d1;
-        const float m = m01 * d2;
-        const float r = __builtin_amdgcn_rcpf(m);
-        const float r2 = r * m01;
-        const float q = r * d2;
-        const float r0 = q * d1, r1 = q * d0;
-        s[bb][0][0] = fmaf(fmaf(e.Ir, u0, e.a), r0, s[bb][0][0]);
-        s[bb][0][1] = fmaf(fmaf(e.Ii, u0, e.b), r0, s[bb][0][1]);
-        s[bb][1][0] = fmaf(fmaf(e.Ir, u1, e.a), r1, s[bb][1][0]);
-        s[bb][1][1] = fmaf(fmaf(e.Ii, u1, e.b), r1, s[bb][1][1]);
-        s[bb][2][0] = fmaf(fmaf(e.Ir, u2, e.a), r2, s[bb][2][0]);
-        s[bb][2][1] = fmaf(fmaf(e.Ii, u2, e.b), r2, s[bb][2][1]);
+      for (int k = 0; k < BPL * 3; ++k) {
+        u[k] = x[k / 3][k % 3] - e.wr;
+        d[k] = fmaf(u[k], u[k], e.wi2);
+      }
+      batch_inverse(d, r);
+#pragma unroll
+      for (int k = 0; k < BPL * 3; ++k) {
+        s[k / 3][k % 3][0] = fmaf(fmaf(e.Ir, u[k], e.a), r[k], s[k / 3][k % 3][0]);
+        s[k / 3][k % 3][1] = fmaf(fmaf(e.Ii, u[k], e.b), r[k], s[k / 3][k % 3][1]);
       }
     }
 #pragma unroll
@@ -243,7 +257,7 @@ __device__ inline void tile_f32(const TileEntry32* __restrict__ tile, int n, con
 }
 
 template <int NT, int BPL, bool F32>
-__global__ __launch_bounds__(NT) void k_gpp_main(const double2* __restrict__ wt,
+__global__ __launch_bounds__(NT, 4) void k_gpp_main(const double2* __restrict__ wt,
                                                  const double2* __restrict__ Ieps,
                                                  const double2* __restrict__ S,
                                                  const double* __restrict__ wxb,
@@ -304,10 +318,14 @@ __global__ __launch_bounds__(NT) void k_gpp_main(const double2* __restrict__ wt,
       }
       const int unsafe = __syncthreads_or(!ok);
       const int n = min(NT, ge - g0);
-      if (unsafe) {
+      if (F32) {
+        // FP32-bulk form: FP32 tiles far from every pole, FP64 (IEEE division) otherwise
+        if (!unsafe && !__syncthreads_or(!ok32))
+          tile_f32<BPL>(tile32, n, x32, acc);
+        else
+          tile_exact<BPL>(tile, n, x, acc);
+      } else if (unsafe) {
         tile_exact<BPL>(tile, n, x, acc);
-      } else if (F32 && !__syncthreads_or(!ok32)) {
-        tile_f32<BPL>(tile32, n, x32, acc);
       } else {
         tile_fast<BPL>(tile, n, x, acc);
       }
