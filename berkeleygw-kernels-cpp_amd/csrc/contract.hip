@@ -27,11 +27,13 @@ namespace {
 typedef double f64x4 __attribute__((ext_vector_type(4)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
-constexpr int BM = 64;   // rows (bands) per workgroup tile
+constexpr int BM = 128;  // rows (bands) per workgroup tile: one tile holds a frequency
+                         // slice's whole band list in the FF cases, so each slice is read once
 constexpr int BN = 64;   // columns (G') per workgroup tile
 constexpr int BK = 16;   // G-vectors per LDS stage
-constexpr int NT = 256;  // 4 waves, each a 32x32 sub-tile (2x2 MFMA blocks)
-constexpr int LDP = BM + 2;  // padded LDS row
+constexpr int NT = 512;  // 8 waves (4 x 2), each a 32x32 sub-tile (2x2 MFMA blocks)
+constexpr int AE = BM * BK / NT;  // A elements staged per thread and stage
+constexpr int BE = BN * BK / NT;  // B elements staged per thread and stage
 
 template <int MODE>
 __device__ inline double2 load_b(const double2* __restrict__ X, const double2* __restrict__ Y, size_t idx,
@@ -54,8 +56,8 @@ __device__ inline double2 load_b(const double2* __restrict__ X, const double2* _
 template <int MODE, bool F32>
 __global__ __launch_bounds__(NT) void k_contract(ContractArgs a) {
   typedef typename std::conditional<F32, float, double>::type T;
-  __shared__ T As[2][BK][LDP];  // [re/im][k][row]
-  __shared__ T Bs[2][BK][LDP];  // [re/im][k][col]
+  __shared__ T As[2][BK][BM + 2];  // [re/im][k][row], padded
+  __shared__ T Bs[2][BK][BN + 2];  // [re/im][k][col]
   __shared__ cdd red[NT / 64];
 
   const int job = blockIdx.z;
@@ -78,7 +80,7 @@ __global__ __launch_bounds__(NT) void k_contract(ContractArgs a) {
   const int k0 = ks * kper, k1 = min(K, k0 + kper);
 
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int wr = (wave >> 1) * 32, wc = (wave & 1) * 32;
+  const int wr = (wave >> 1) * 32, wc = (wave & 1) * 32;  // waves 4 (rows) x 2 (cols)
 
   f64x4 dr[2][2], di[2][2];
 #pragma unroll
@@ -89,19 +91,21 @@ __global__ __launch_bounds__(NT) void k_contract(ContractArgs a) {
       di[i][j] = (f64x4){0.0, 0.0, 0.0, 0.0};
     }
 
-  // each thread stages 4 A and 4 B elements per stage: idx = tid + 256*i -> (r, k)
-  int arow[4];
-  bool aok[4];
-  size_t bbase[4];
-  bool bok[4];
+  // each thread stages AE elements of A and BE of B per stage: idx = tid + NT*i -> (row, k)
+  int arow[AE];
+  bool aok[AE];
+  size_t bbase[BE];
+  bool bok[BE];
 #pragma unroll
-  for (int i = 0; i < 4; ++i) {
-    const int idx = tid + NT * i;
-    const int r = idx / BK;
+  for (int i = 0; i < AE; ++i) {
+    const int r = (tid + NT * i) / BK;
     const int gr = rt * BM + r;
     aok[i] = gr < nrows;
     arow[i] = aok[i] ? rows[gr] : 0;
-    const int gc = ct * BN + r;
+  }
+#pragma unroll
+  for (int i = 0; i < BE; ++i) {
+    const int gc = ct * BN + (tid + NT * i) / BK;
     bok[i] = gc < a.N;
     bbase[i] = (size_t)(bok[i] ? gc : 0) * K;
   }
@@ -117,29 +121,33 @@ __global__ __launch_bounds__(NT) void k_contract(ContractArgs a) {
 
   // software pipeline: the global loads of stage kb+BK are in flight while the
   // MFMAs of stage kb run
-  double2 av[4], bv[4];
+  double2 av[AE], bv[BE];
   auto fetch = [&](int kb) {
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      const int idx = tid + NT * i;
-      const int kl = idx % BK;
-      const int k = kb + kl;
-      const bool kin = k < k1;
-      av[i] = (aok[i] && kin) ? a.A[(size_t)arow[i] * K + k] : make_double2(0.0, 0.0);
-      bv[i] = (bok[i] && kin) ? load_b<MODE>(X, Y, bbase[i] + k, a.thr2) : make_double2(0.0, 0.0);
+    for (int i = 0; i < AE; ++i) {
+      const int k = kb + (tid + NT * i) % BK;
+      av[i] = (aok[i] && k < k1) ? a.A[(size_t)arow[i] * K + k] : make_double2(0.0, 0.0);
+    }
+#pragma unroll
+    for (int i = 0; i < BE; ++i) {
+      const int k = kb + (tid + NT * i) % BK;
+      bv[i] = (bok[i] && k < k1) ? load_b<MODE>(X, Y, bbase[i] + k, a.thr2) : make_double2(0.0, 0.0);
     }
   };
   if (k0 < k1) fetch(k0);
   for (int kb = k0; kb < k1; kb += BK) {
     __syncthreads();
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
+    for (int i = 0; i < AE; ++i) {
       const int idx = tid + NT * i;
-      const int r = idx / BK, kl = idx % BK;
-      As[0][kl][r] = (T)av[i].x;
-      As[1][kl][r] = (T)av[i].y;
-      Bs[0][kl][r] = (T)bv[i].x;
-      Bs[1][kl][r] = (T)bv[i].y;
+      As[0][idx % BK][idx / BK] = (T)av[i].x;
+      As[1][idx % BK][idx / BK] = (T)av[i].y;
+    }
+#pragma unroll
+    for (int i = 0; i < BE; ++i) {
+      const int idx = tid + NT * i;
+      Bs[0][idx % BK][idx / BK] = (T)bv[i].x;
+      Bs[1][idx % BK][idx / BK] = (T)bv[i].y;
     }
     __syncthreads();
     if (kb + BK < k1) fetch(kb + BK);

@@ -2,7 +2,7 @@
 //
 // Each reference kernel is a sum over bands of an interpolated frequency slice
 // contracted with aqsntemp over ig and weighted by vcoul*conj(aqsm) over my_igp.
-// A single-thread plan kernel evaluates compute_fact per band on the device
+// A one-block plan kernel evaluates compute_fact per band on the device
 // (FF/ff.cpp:245-252, :337-342) and lists, for every frequency slice, the bands
 // that use it with their interpolation weight; the MFMA contraction then reads
 // each touched slice once for all bands of its list.  Because wx has no iw
@@ -19,48 +19,68 @@
 namespace bgw {
 namespace {
 
+constexpr int kMaxLdsJobs = 4096;
+
 // op 1 (asxDtemp): bands n1 < nvband, loop 1, jobs [0,nFreq) on I_epsR, [nFreq,2nFreq) on I_epsA.
 // op 2 (achDtemp_cor): all bands, loop 2, jobs [0,nFreq) on I_epsR - I_epsA;
 //   wx>0 & unoccupied -> schDttt_corKernel1 (weights fact1,fact2),
 //   wx<=0 & occupied  -> schDttt_corKernel2 (weights -1/2 fact1, -1/2 fact2).
-__global__ void k_ff_plan(int op, int nb, int nvband, int nFreq, double freqevalmin, double freqevalstep,
-                          double occ, const double* __restrict__ ekq, const double* __restrict__ grid,
-                          int* __restrict__ rows, double2* __restrict__ wts, int* __restrict__ cnt,
-                          int* __restrict__ slice_of_job) {
-  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+__global__ __launch_bounds__(256) void k_ff_plan(int op, int nb, int nvband, int nFreq, double freqevalmin,
+                                                 double freqevalstep, double occ, const double* __restrict__ ekq,
+                                                 const double* __restrict__ grid, int* __restrict__ rows,
+                                                 double2* __restrict__ wts, int* __restrict__ cnt,
+                                                 int* __restrict__ slice_of_job, int* __restrict__ band_job,
+                                                 double2* __restrict__ band_w) {
+  // per-slice counters in LDS when they fit (kMaxLdsJobs), else in `cnt` itself
+  extern __shared__ int lds_cnt[];
   const int njobs = op == 1 ? 2 * nFreq : nFreq;
-  for (int j = 0; j < njobs; ++j) {
-    cnt[j] = 0;
-    slice_of_job[j] = j % nFreq;
-  }
+  int* scnt = njobs <= kMaxLdsJobs ? lds_cnt : cnt;
   const int nbands = op == 1 ? nvband : nb;
-  for (int n1 = 0; n1 < nbands; ++n1) {
+  // 1) compute_fact for every band in parallel: band_job = (first job, or -1), band_w = (w1, w2)
+  for (int n1 = threadIdx.x; n1 < nbands; n1 += blockDim.x) {
     const double wx = freqevalmin - ekq[n1] + freqevalstep;
     double f1 = 0.0, f2 = 0.0;
     int ifreq = 0;
-    double scale;
-    int base = 0;
+    int job = -1;
+    double scale = 0.0;
     if (op == 1) {
       compute_fact(wx, nFreq, grid, f1, f2, ifreq, 1, false);
       scale = occ;
-      base = wx > 0 ? 0 : nFreq;
+      job = (wx > 0 ? 0 : nFreq) + ifreq;
     } else {
       const bool occd = n1 < nvband;
       compute_fact(wx, nFreq, grid, f1, f2, ifreq, 2, occd);
-      if (wx > 0 && !occd)
+      if (wx > 0 && !occd) {
         scale = 1.0;
-      else if (!(wx > 0) && occd)
+        job = ifreq;
+      } else if (!(wx > 0) && occd) {
         scale = -0.5;
-      else
-        continue;
+        job = ifreq;
+      }
     }
-    const int ja = base + ifreq, jb = base + ifreq + 1;
-    rows[(size_t)ja * nb + cnt[ja]] = n1;
-    wts[(size_t)ja * nb + cnt[ja]] = make_double2(scale * f1, 0.0);
-    cnt[ja] += 1;
-    rows[(size_t)jb * nb + cnt[jb]] = n1;
-    wts[(size_t)jb * nb + cnt[jb]] = make_double2(scale * f2, 0.0);
-    cnt[jb] += 1;
+    band_job[n1] = job;
+    band_w[n1] = make_double2(scale * f1, scale * f2);
+  }
+  for (int j = threadIdx.x; j < njobs; j += blockDim.x) scnt[j] = 0;
+  __syncthreads();
+  // 2) one thread appends the bands to their two slices in ascending band order
+  //    (a fixed order, so the contraction's summation order is reproducible)
+  if (threadIdx.x == 0) {
+    for (int j = 0; j < njobs; ++j) slice_of_job[j] = j % nFreq;
+    for (int n1 = 0; n1 < nbands; ++n1) {
+      const int ja = band_job[n1];
+      if (ja < 0) continue;
+      const double2 w = band_w[n1];
+      const int ca = scnt[ja], cb = scnt[ja + 1];
+      rows[(size_t)ja * nb + ca] = n1;
+      wts[(size_t)ja * nb + ca] = make_double2(w.x, 0.0);
+      rows[(size_t)(ja + 1) * nb + cb] = n1;
+      wts[(size_t)(ja + 1) * nb + cb] = make_double2(w.y, 0.0);
+      scnt[ja] = ca + 1;
+      scnt[ja + 1] = cb + 1;
+    }
+    if (scnt != cnt)
+      for (int j = 0; j < njobs; ++j) cnt[j] = scnt[j];
   }
 }
 
@@ -107,9 +127,13 @@ int ff_device(bgw_ctx* ctx, int op, int nb, int nvband, int nfreqeval, int ncoul
     a.jobs_a = 1;
     return contract_launch(ctx, a, 1, (nb + BMr - 1) / BMr, "contract_ff_achs");
   }
-  hipLaunchKernelGGL(k_ff_plan, dim3(1), dim3(64), 0, s, op, nb, nvband, nFreq, freqevalmin, freqevalstep, occ,
+  void *pbj, *pbw;
+  if ((rc = pool_get(ctx, "ws_ff_band_job", (size_t)nb * sizeof(int), &pbj))) return rc;
+  if ((rc = pool_get(ctx, "ws_ff_band_w", (size_t)nb * 16, &pbw))) return rc;
+  const size_t plan_lds = (size_t)(njobs <= kMaxLdsJobs ? njobs : 1) * sizeof(int);
+  hipLaunchKernelGGL(k_ff_plan, dim3(1), dim3(256), plan_lds, s, op, nb, nvband, nFreq, freqevalmin, freqevalstep, occ,
                      ekq, dFreqGrid, static_cast<int*>(prow), static_cast<double2*>(pw), static_cast<int*>(pcnt),
-                     static_cast<int*>(pslice));
+                     static_cast<int*>(pslice), static_cast<int*>(pbj), static_cast<double2*>(pbw));
   if ((rc = check_launch(ctx, "k_ff_plan"))) return rc;
   if (op == 1) {
     a.Xa = reinterpret_cast<const double2*>(I_epsR);
@@ -142,6 +166,8 @@ int ff_validate(bgw_ctx* ctx, int op, int nb, int nvband, int nfreqeval, int nco
   if (nb <= 0 || ncouls <= 0 || ngpown <= 0) return fail(ctx, BGW_EINVAL, "zero or negative size");
   if (ngpown > ncouls) return fail(ctx, BGW_EINVAL, "ngpown cannot be greater than ncouls");  // FF/ff.cpp:492-499
   if (op != 0 && (nFreq < 2 || nfreqeval <= 0)) return fail(ctx, BGW_EINVAL, "nFreq < 2 or nfreqeval <= 0");
+  // one grid z-slice per frequency slice (two per slice for asxDtemp)
+  if (op != 0 && (op == 1 ? 2L * nFreq : (long)nFreq) > 65535) return fail(ctx, BGW_EINVAL, "nFreq too large");
   if (op != 0 && (nvband < 0 || nvband > nb)) return fail(ctx, BGW_EINVAL, "nvband out of [0, number_bands]");
   size_t t;
   if (!mul_ok((size_t)ngpown, (size_t)ncouls, &t) || !mul_ok(t, (size_t)(nFreq > 2 ? nFreq : 2), &t) ||

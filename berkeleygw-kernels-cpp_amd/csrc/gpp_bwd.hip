@@ -34,9 +34,8 @@ constexpr int kMaxBandsLds = 1024;
 
 // holomorphic derivatives of gpp_ssx with respect to I, wtilde and wx (branch-wise)
 __device__ inline void ssx_grad(double wx, double2 wt, double2 I, double2& dI, double2& dW, double2& dX) {
-  const double2 ssx = gpp_ssx(wx, wt, I);
   dI = dW = dX = make_double2(0.0, 0.0);
-  if (sqrt(cnorm2(ssx)) > kSexcut * sqrt(cnorm2(I)) && wx < 0.0) return;
+  if (gpp_ssx_cut(wx, gpp_ssx_raw(wx, wt, I), I)) return;  // cut terms are constant zero
   const double2 wt2 = cmul(wt, wt);
   const double2 wdiff = make_double2(wx - wt.x, -wt.y);
   const double wdiffr = cnorm2(wdiff);

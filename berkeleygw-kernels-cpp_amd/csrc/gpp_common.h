@@ -42,8 +42,9 @@ __device__ inline int ff_igp(const int* __restrict__ inv, const int* __restrict_
 // ssx of BerkeleyGW's occupied-band GPP branch for one (wx, wtilde, I_eps):
 // regular region ssx = Omega2 conj(cden)/|cden|^2 with cden = wx^2 - wtilde^2,
 // near-pole region ssx = -Omega2 conj(cden)/|cden|^2 delw with cden = 4 wtilde^2 (delw+1/2),
-// zero otherwise, and zero when |ssx| > sexcut |I_eps| for wx < 0.
-__device__ inline double2 gpp_ssx(double wx, double2 wt, double2 I) {
+// zero otherwise (gpp_ssx_raw); gpp_ssx also applies the cut-off to zero when
+// |ssx| > sexcut |I_eps| for wx < 0.
+__device__ inline double2 gpp_ssx_raw(double wx, double2 wt, double2 I) {
   const double2 wt2 = cmul(wt, wt);
   const double2 Omega2 = cmul(wt2, I);
   const double2 wdiff = make_double2(wx - wt.x, -wt.y);
@@ -64,9 +65,17 @@ __device__ inline double2 gpp_ssx(double wx, double2 wt, double2 I) {
   } else {
     ssx = make_double2(0.0, 0.0);
   }
-  const double cutoff = kSexcut * sqrt(cnorm2(I));
-  if (sqrt(cnorm2(ssx)) > cutoff && wx < 0.0) ssx = make_double2(0.0, 0.0);
   return ssx;
+}
+
+// the |ssx| > sexcut |I| cut-off for wx < 0, decided on the value before the cut
+__device__ inline bool gpp_ssx_cut(double wx, double2 ssx, double2 I) {
+  return sqrt(cnorm2(ssx)) > kSexcut * sqrt(cnorm2(I)) && wx < 0.0;
+}
+
+__device__ inline double2 gpp_ssx(double wx, double2 wt, double2 I) {
+  const double2 ssx = gpp_ssx_raw(wx, wt, I);
+  return gpp_ssx_cut(wx, ssx, I) ? make_double2(0.0, 0.0) : ssx;
 }
 
 // MFMA trace contraction (contract.hip):

@@ -115,8 +115,9 @@ static inline double complex sch_store1(double complex aqsm, double complex aqsn
   return t * wt_igp;
 }
 
-/* ssx of the occupied-band (flagOCC) branch of BerkeleyGW's GPP kernel. */
-static inline double complex ssx_term(double wx, double complex wt, double complex Ie) {
+/* ssx of the occupied-band (flagOCC) branch of BerkeleyGW's GPP kernel, before
+ * the |ssx| > sexcut |I| cut-off for wx < 0. */
+static inline double complex ssx_raw(double wx, double complex wt, double complex Ie) {
   double complex wt2 = wt * wt;
   double complex Omega2 = wt2 * Ie;
   double complex wdiff = wx - wt;
@@ -136,9 +137,16 @@ static inline double complex ssx_term(double wx, double complex wt, double compl
   } else {
     ssx = 0.0;
   }
-  double cutoff = SEXCUT * cabs(Ie);
-  if (cabs(ssx) > cutoff && wx < 0.0) ssx = 0.0;
   return ssx;
+}
+
+static inline int ssx_cut(double wx, double complex ssx, double complex Ie) {
+  return cabs(ssx) > SEXCUT * cabs(Ie) && wx < 0.0;
+}
+
+static inline double complex ssx_term(double wx, double complex wt, double complex Ie) {
+  const double complex ssx = ssx_raw(wx, wt, Ie);
+  return ssx_cut(wx, ssx, Ie) ? 0.0 : ssx;
 }
 
 /* Holomorphic derivatives of ssx_term with respect to Ie, wtilde and wx. */
@@ -149,10 +157,8 @@ static inline void ssx_grad(double wx, double complex wt, double complex Ie, dou
   double wdiffr = creal(wdiff * conj(wdiff));
   double complex delw = wt / wdiff;
   double delwr = creal(delw * conj(delw));
-  double complex ssx = ssx_term(wx, wt, Ie);
-  double cutoff = SEXCUT * cabs(Ie);
   *dI = *dW = *dX = 0.0;
-  if (cabs(ssx) > cutoff && wx < 0.0) return;
+  if (ssx_cut(wx, ssx_raw(wx, wt, Ie), Ie)) return; /* cut terms are constant zero */
   if (wdiffr > LIMITTWO && delwr < LIMITONE) {
     double complex cden = wx * wx - wt2;
     *dI = wt2 / cden;

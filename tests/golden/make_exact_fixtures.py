@@ -63,7 +63,35 @@ def dy(rng, lo, hi):
     return F(rng.randint(int(lo * 64), int(hi * 64)), 64)
 
 
-def gpp_case(seed, nb, nvband, ngpown, ncouls):
+TO1 = F(1, 10**6)
+LIMITTWO = F(1, 4)
+LIMITONE = 1 / (4 * TO1)
+SEXCUT = F(4)
+
+
+def ssx_exact(wx, wt, I):
+    """BerkeleyGW occupied-band ssx in exact arithmetic (three branches + cut-off);
+    |z| comparisons are done on squares."""
+    wt2 = wt * wt
+    Omega2 = wt2 * I
+    wdiff = Cx(wx) - wt
+    wdiffr = wdiff.norm2()
+    delw = wt * wdiff.conj() * (F(1) / wdiffr)
+    delwr = delw.norm2()
+    if wdiffr > LIMITTWO and delwr < LIMITONE:
+        cden = Cx(wx * wx) - wt2
+        ssx = Omega2 * cden.conj() * (F(1) / cden.norm2())
+    elif delwr > TO1:
+        cden = F(4) * wt2 * (delw + F(1, 2))
+        ssx = (Cx(0) - Omega2) * cden.conj() * (F(1) / cden.norm2()) * delw
+    else:
+        ssx = Cx(0)
+    if ssx.norm2() > SEXCUT * SEXCUT * I.norm2() and wx < 0:
+        ssx = Cx(0)
+    return ssx
+
+
+def gpp_case(seed, nb, nvband, ngpown, ncouls, near_pole=False):
     rng = random.Random(seed)
     aqsm = [Cx(dy(rng, -1, 1), dy(rng, -1, 1)) for _ in range(nb * ncouls)]
     aqsn = [Cx(dy(rng, -1, 1), dy(rng, -1, 1)) for _ in range(nb * ncouls)]
@@ -77,6 +105,14 @@ def gpp_case(seed, nb, nvband, ngpown, ncouls):
     rng.shuffle(perm)
     indinv = perm + [ncouls - 1]
     wx = [dy(rng, 8, 12) for _ in range(nb * 3)]
+    if near_pole:
+        # band 0 at negative frequencies (the cut-off applies there) and a few
+        # wtilde within 1/2 of a band frequency (the near-pole branch)
+        wx[0], wx[1], wx[2] = F(-1, 2), F(-3, 16), F(-5, 4)
+        wt[0] = Cx(wx[3] + F(1, 8), F(-1, 16))
+        wt[ncouls + 1] = Cx(wx[4] - F(3, 16), F(-1, 32))
+        wt[2] = Cx(F(-5, 16), F(-1, 8))
+        wt[3] = Cx(F(1, 8192), F(0))  # |delw|^2 <= to1 next to wx = -3/16: the zero branch
     ach = [[Cx(0) for _ in range(3)] for _ in range(nb)]
     achs = Cx(0)
     for p in range(ngpown):
@@ -95,13 +131,23 @@ def gpp_case(seed, nb, nvband, ngpown, ncouls):
                 m = m + aqsn[n1 * ncouls + g] * I[p * ncouls + g]
             achs = achs + F(-1, 2) * vcoul[igp] * aqsm[n1 * ncouls + igp].conj() * m
     achtemp = [sum((ach[n1][iw] for n1 in range(nb)), Cx(0)) for iw in range(3)]
+    asx = [Cx(0) for _ in range(3)]
+    for n1 in range(nvband):
+        for p in range(ngpown):
+            igp = indinv[inv[p]]
+            m = aqsm[n1 * ncouls + igp].conj() * vcoul[igp]
+            for iw in range(3):
+                t = Cx(0)
+                for g in range(ncouls):
+                    t = t + aqsn[n1 * ncouls + g] * ssx_exact(wx[n1 * 3 + iw], wt[p * ncouls + g], I[p * ncouls + g])
+                asx[iw] = asx[iw] + m * t
     return {
         "nb": nb, "nvband": nvband, "ngpown": ngpown, "ncouls": ncouls,
         "aqsm": [c.out() for c in aqsm], "aqsn": [c.out() for c in aqsn], "I_eps": [c.out() for c in I],
         "wtilde": [c.out() for c in wt], "vcoul": [float(v) for v in vcoul], "inv": inv, "indinv": indinv,
         "wx": [float(v) for v in wx],
         "expect": {"achtemp": [c.out() for c in achtemp], "acht_n1_loc": [[c.out() for c in r] for r in ach],
-                   "achstemp": achs.out()},
+                   "achstemp": achs.out(), "asxtemp": [c.out() for c in asx]},
     }
 
 
@@ -202,7 +248,7 @@ def ff_case(seed, nb, nvband, ngpown, ncouls, nFreq, nfreqeval):
 
 
 def main():
-    cases = [gpp_case(11, 3, 1, 4, 8), gpp_case(12, 5, 2, 3, 7)]
+    cases = [gpp_case(11, 3, 1, 4, 8), gpp_case(12, 5, 2, 3, 7), gpp_case(13, 4, 2, 3, 6, near_pole=True)]
     (OUT / "exact_gpp.json").write_text(json.dumps(cases))
     ff = [ff_case(21, 6, 2, 3, 8, 5, 2), ff_case(22, 9, 3, 2, 6, 6, 3)]
     (OUT / "exact_ff.json").write_text(json.dumps(ff))

@@ -22,7 +22,8 @@ def load_exact_gpp():
                  vcoul=np.asarray(c["vcoul"], np.float64), inv=np.asarray(c["inv"], np.int32),
                  indinv=np.asarray(c["indinv"], np.int32), wx=np.asarray(c["wx"], np.float64))
         e = c["expect"]
-        exp = dict(achtemp=_c(e["achtemp"]), acht_n1_loc=_c(e["acht_n1_loc"]), achstemp=_c([e["achstemp"]]))
+        exp = dict(achtemp=_c(e["achtemp"]), acht_n1_loc=_c(e["acht_n1_loc"]), achstemp=_c([e["achstemp"]]),
+                   asxtemp=_c(e["asxtemp"]))
         out.append((d, exp))
     return out
 

@@ -62,6 +62,16 @@ def test_committed_reference_vectors(ctx):
         assert rel_err(cor, rc[:, 0] + 1j * rc[:, 1]) < 1e-11
 
 
+def test_large_frequency_grid_vs_oracle(ctx):
+    # the reference's "compute bound" regime has many frequencies (FF/ff.cpp:56-57);
+    # 2*nFreq slices exceed the plan kernel's LDS counters and use its global path
+    f = bgw.ff_inputs(8, 2, 100, 10, 5000, 7, 5000)
+    achs, asx, cor = _all(ctx, f)
+    assert abs(achs - O.achsDtemp(f)) <= TOL * abs(O.achsDtemp(f))
+    assert rel_err(asx, O.asxDtemp(f)) < TOL
+    assert rel_err(cor, O.achDtemp_cor(f)) < TOL
+
+
 def test_validation(ctx):
     f = bgw.ff_inputs(10, 2, 20, 30, 5, 2, 3)
     with pytest.raises(bgw.BgwError) as e:  # ngpown > ncouls (FF/ff.cpp:492-499)

@@ -31,6 +31,18 @@ def test_small_full_gradient_vs_oracle(ctx):
     assert rel_err(g["dsigma_domega"], O.gpp_dsigma_domega(d, occ=0.8)) < 1e-12
 
 
+def test_near_pole_branches_vs_oracle(ctx):
+    from helpers import load_exact_gpp
+
+    d, _ = load_exact_gpp()[2]  # near-pole, zero-branch and cut-off terms in asxtemp
+    g = ctx.gpp_backward(d, occ=1.0)
+    e = np.arange(d["ngpown"] * d["ncouls"], dtype=np.int64)
+    dw, dI = O.gpp_grad_entries(d, e, occ=1.0)
+    assert np.max(np.abs(g["dL_dwtilde"] - dw)) <= 1e-12 * np.max(np.abs(dw))
+    assert np.max(np.abs(g["dL_dI_eps"] - dI)) <= 1e-12 * np.max(np.abs(dI))
+    assert rel_err(g["dsigma_domega"], O.gpp_dsigma_domega(d, occ=1.0)) < 1e-12
+
+
 def test_bitwise_reproducible(ctx):
     d = bgw.gpp_inputs(40, 2, 700, 9, 1)
     a = ctx.gpp_backward(d)

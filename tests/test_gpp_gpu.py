@@ -21,13 +21,14 @@ def _check_all(r, o, tol=TOL):
     assert rel_err(r["asxtemp"], o["asxtemp"]) < tol
 
 
-@pytest.mark.parametrize("case", range(2))
+@pytest.mark.parametrize("case", range(3))
 def test_exact_fixture(ctx, case):
     d, exp = load_exact_gpp()[case]
     r = ctx.gpp_solver(d, occ=1.0)
     assert rel_err(r["achtemp"], exp["achtemp"]) < 1e-13
     assert rel_err(r["acht_n1_loc"], exp["acht_n1_loc"]) < 1e-13
     assert rel_err(r["achstemp"], exp["achstemp"]) < 1e-13
+    assert rel_err(r["asxtemp"], exp["asxtemp"]) < 1e-13
 
 
 @pytest.mark.parametrize("nb,ngpown,ncouls", [(1, 1, 1), (40, 50, 1000), (512, 7, 3000)])

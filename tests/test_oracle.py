@@ -21,13 +21,14 @@ from helpers import (ff_reference_closed_form, gpp_reference_achstemp, gpp_refer
 # ------------------------------------------------------------------------- GPP
 
 
-@pytest.mark.parametrize("case", range(2))
+@pytest.mark.parametrize("case", range(3))
 def test_gpp_oracle_exact_fixture(case):
     d, exp = load_exact_gpp()[case]
     r = O.gpp_forward(d)
     assert rel_err(r["achtemp"], exp["achtemp"]) < 1e-14
     assert rel_err(r["acht_n1_loc"], exp["acht_n1_loc"]) < 1e-14
     assert rel_err(r["achstemp"], exp["achstemp"]) < 1e-14
+    assert rel_err(r["asxtemp"], exp["asxtemp"]) < 1e-14  # case 2 covers near-pole and cut-off
 
 
 @pytest.mark.parametrize("nb,ngpown,ncouls", [(1, 1, 1), (3, 5, 17), (7, 33, 130), (12, 64, 1000)])
@@ -156,6 +157,32 @@ def test_gpp_oracle_gradient_vs_finite_differences():
                 fd.append((_L(dp) - _L(dm)) / (2 * h))
             fdc = fd[0] + 1j * fd[1]
             assert abs(fdc - grad[k]) <= 1e-6 * max(1.0, abs(grad[k])), (name, e, fdc, grad[k])
+
+
+def test_gpp_oracle_gradient_near_pole_branches_vs_finite_differences():
+    # exact fixture 2: terms in the near-pole branch, the zero branch and the cut-off
+    d, _ = load_exact_gpp()[2]
+    n = d["ngpown"] * d["ncouls"]
+    entries = np.arange(n, dtype=np.int64)
+    dw, dI = O.gpp_grad_entries(d, entries)
+    h = 1e-7
+    for k in entries:
+        for name, grad in (("wtilde", dw), ("I_eps", dI)):
+            fd = []
+            for unit in (1.0, 1j):
+                dp = {**d, name: d[name].copy()}
+                dm = {**d, name: d[name].copy()}
+                dp[name][k] += h * unit
+                dm[name][k] -= h * unit
+                fd.append((_L(dp) - _L(dm)) / (2 * h))
+            fdc = fd[0] + 1j * fd[1]
+            assert abs(fdc - grad[k]) <= 1e-6 * max(1.0, abs(grad[k])), (name, k, fdc, grad[k])
+    ds = O.gpp_dsigma_domega(d)
+    hw = 1e-7
+    rp = O.gpp_forward(dict(d, wx=d["wx"] + hw), want=("achtemp", "asxtemp"))
+    rm = O.gpp_forward(dict(d, wx=d["wx"] - hw), want=("achtemp", "asxtemp"))
+    fd = ((rp["achtemp"] + rp["asxtemp"]) - (rm["achtemp"] + rm["asxtemp"])) / (2 * hw)
+    assert np.max(np.abs(fd - ds)) <= 1e-6 * np.max(np.abs(ds))
 
 
 def test_gpp_oracle_dsigma_domega_vs_finite_differences():
