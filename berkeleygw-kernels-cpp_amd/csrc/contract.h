@@ -33,11 +33,16 @@ struct ContractArgs {
   int ncouls;
   double thr2;             // mask threshold (kModeMask)
   int ksplit;              // set by contract_launch
+  int tiles;               // set by contract_launch: row-tile slots per (G' tile, K split)
+  int njobs;               // set by contract_launch
+  const int* tile_start;   // set by contract_launch: [njobs+1] prefix of ceil(cnt/BM)
+  int items;               // set by contract_launch: workgroup tasks (= partials)
   double4* partial;        // set by contract_launch
   double2* out;            // c128[1] result (device)
 };
 
-int contract_launch(bgw_ctx* ctx, ContractArgs a, int njobs, int max_row_tiles, const char* tname);
+// max_tiles: a bound on sum_j ceil(cnt[j] / contract_row_tile()) known on the host.
+int contract_launch(bgw_ctx* ctx, ContractArgs a, int njobs, int max_tiles, const char* tname);
 int contract_row_tile();
 
 // rows[i] = i, wts[i] = w, cnt[0] = n, slice[0] = slice.

@@ -27,11 +27,12 @@ namespace {
 typedef double f64x4 __attribute__((ext_vector_type(4)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
-constexpr int BM = 128;  // rows (bands) per workgroup tile: one tile holds a frequency
-                         // slice's whole band list in the FF cases, so each slice is read once
+constexpr int BM = 64;   // rows (bands) per workgroup tile
 constexpr int BN = 64;   // columns (G') per workgroup tile
 constexpr int BK = 16;   // G-vectors per LDS stage
-constexpr int NT = 512;  // 8 waves (4 x 2), each a 32x32 sub-tile (2x2 MFMA blocks)
+constexpr int NT = 256;  // 4 waves (2 x 2), each a 32x32 sub-tile (2x2 MFMA blocks)
+constexpr int kXcds = 8;  // MI355X: 8 XCDs, workgroups dealt round-robin by linear id
+constexpr int kChunk = 8; // consecutive work items kept on one XCD
 constexpr int AE = BM * BK / NT;  // A elements staged per thread and stage
 constexpr int BE = BN * BK / NT;  // B elements staged per thread and stage
 
@@ -60,17 +61,36 @@ __global__ __launch_bounds__(NT) void k_contract(ContractArgs a) {
   __shared__ T Bs[2][BK][BN + 2];  // [re/im][k][col]
   __shared__ cdd red[NT / 64];
 
-  const int job = blockIdx.z;
-  const int ntn = (a.N + BN - 1) / BN;
-  const int ct = blockIdx.x % ntn;
-  const int ks = blockIdx.x / ntn;
-  const int rt = blockIdx.y;
-  const int nrows = a.cnt[job];
-  const size_t pid = ((size_t)job * gridDim.y + rt) * gridDim.x + blockIdx.x;
-  if (rt * BM >= nrows) {  // empty row tile: a zero partial keeps the reduction order fixed
+  // Work items are (G' tile ct, K split ks, live row tile t), t fastest.  The live
+  // row tiles of all jobs are numbered densely by tile_start (prefix of
+  // ceil(cnt/BM)), so empty jobs and empty row-tile tails take no workgroup.
+  // XCD-aware order: the hardware deals workgroup L to XCD L % 8; chunks of
+  // kChunk consecutive items are dealt round-robin over the XCDs, so the row tiles
+  // of one (slice, ct, ks) run side by side on one XCD and share their B tile
+  // through its L2, while every XCD gets an even share of every slice.
+  const int slot = (int)(blockIdx.x / kXcds);
+  const int item = ((slot / kChunk) * kXcds + (int)(blockIdx.x % kXcds)) * kChunk + slot % kChunk;
+  if (item >= a.items) return;  // grid padding to whole chunks
+  const size_t pid = (size_t)item;
+  const int t = item % a.tiles;
+  const int ks = (item / a.tiles) % a.ksplit;
+  const int ct = item / a.tiles / a.ksplit;
+  // job = the largest j with tile_start[j] <= t (jobs without tiles repeat a value)
+  int lo = 0, hi = a.njobs;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (a.tile_start[mid] <= t)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  if (lo >= a.njobs) {  // slot beyond the live tiles: a zero partial keeps the reduction order fixed
     if (threadIdx.x == 0) cdd_store(&a.partial[pid], cdd_zero());
     return;
   }
+  const int job = lo;
+  const int rt = t - a.tile_start[job];
+  const int nrows = a.cnt[job];
   const int slice = a.slice_of_job[job];
   const double2* __restrict__ X = (a.Xb && job >= a.jobs_a ? a.Xb : a.Xa) + (size_t)slice * a.slice_stride;
   const double2* __restrict__ Y = a.Y ? a.Y + (size_t)slice * a.slice_stride : nullptr;
@@ -80,7 +100,7 @@ __global__ __launch_bounds__(NT) void k_contract(ContractArgs a) {
   const int k0 = ks * kper, k1 = min(K, k0 + kper);
 
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int wr = (wave >> 1) * 32, wc = (wave & 1) * 32;  // waves 4 (rows) x 2 (cols)
+  const int wr = (wave >> 1) * 32, wc = (wave & 1) * 32;  // waves 2 (rows) x 2 (cols)
 
   f64x4 dr[2][2], di[2][2];
 #pragma unroll
@@ -248,21 +268,61 @@ __global__ __launch_bounds__(NT) void k_contract(ContractArgs a) {
   if (tid == 0) cdd_store(&a.partial[pid], tot);
 }
 
+// tile_start[j] = sum_{i<j} ceil(cnt[i]/BM), j <= njobs: each thread sums a run
+// of consecutive jobs, then one LDS scan over the runs.
+constexpr int kPrefixThreads = 1024;
+__global__ __launch_bounds__(kPrefixThreads) void k_tile_prefix(const int* __restrict__ cnt, int njobs,
+                                                               int* __restrict__ tile_start) {
+  __shared__ int part[kPrefixThreads];
+  const int per = (njobs + kPrefixThreads - 1) / kPrefixThreads;
+  const int j0 = min(njobs, (int)threadIdx.x * per), j1 = min(njobs, j0 + per);
+  int s = 0;
+  for (int j = j0; j < j1; ++j) s += (cnt[j] + BM - 1) / BM;
+  part[threadIdx.x] = s;
+  __syncthreads();
+  for (int off = 1; off < kPrefixThreads; off <<= 1) {
+    const int v = threadIdx.x >= (unsigned)off ? part[threadIdx.x - off] : 0;
+    __syncthreads();
+    part[threadIdx.x] += v;
+    __syncthreads();
+  }
+  int run = part[threadIdx.x] - s;  // exclusive prefix of this thread's run
+  for (int j = j0; j < j1; ++j) {
+    tile_start[j] = run;
+    run += (cnt[j] + BM - 1) / BM;
+  }
+  if (threadIdx.x == kPrefixThreads - 1) tile_start[njobs] = part[threadIdx.x];
+}
+
 }  // namespace
 
-int contract_launch(bgw_ctx* ctx, ContractArgs a, int njobs, int max_row_tiles, const char* tname) {
+int contract_launch(bgw_ctx* ctx, ContractArgs a, int njobs, int max_tiles, const char* tname) {
   const int ntn = (a.N + BN - 1) / BN;
   // K split: ~1024 G-vectors per workgroup.  Row counts per job are only known on
-  // the device (plan kernel), so the grid covers the largest possible row count;
-  // workgroups of empty row tiles exit after writing a zero partial.
+  // the device (plan kernel), so the grid covers max_tiles row tiles, a host bound
+  // on sum_j ceil(cnt[j]/BM); slots beyond the live tiles exit after a zero partial.
   int ksplit = a.K / 1024;
   if (ksplit < 1) ksplit = 1;
   if (ksplit > 64) ksplit = 64;
+  if (max_tiles < 1) max_tiles = 1;
   a.ksplit = ksplit;
-  dim3 grid(ntn * ksplit, max_row_tiles, njobs);
-  size_t W = (size_t)grid.x * grid.y * grid.z;
+  a.tiles = max_tiles;
+  a.njobs = njobs;
+  const size_t W = (size_t)ntn * ksplit * max_tiles;
+  const size_t per_round = (size_t)kXcds * kChunk;
+  const size_t G = (W + per_round - 1) / per_round * per_round;
+  if (G > (size_t)INT32_MAX) return fail(ctx, BGW_EINVAL, "contraction grid too large");
+  a.items = (int)W;
+  dim3 grid((unsigned)G);
+  void* pts;
+  int rc = pool_get(ctx, "ws_tile_start", (size_t)(njobs + 1) * sizeof(int), &pts);
+  if (rc) return rc;
+  a.tile_start = static_cast<const int*>(pts);
+  hipLaunchKernelGGL(k_tile_prefix, dim3(1), dim3(kPrefixThreads), 0, ctx->stream, a.cnt, njobs,
+                     static_cast<int*>(pts));
+  if ((rc = check_launch(ctx, "k_tile_prefix"))) return rc;
   void* pp;
-  int rc = pool_get(ctx, "ws_contract", W * sizeof(double4), &pp);
+  rc = pool_get(ctx, "ws_contract", W * sizeof(double4), &pp);
   if (rc) return rc;
   a.partial = static_cast<double4*>(pp);
   {

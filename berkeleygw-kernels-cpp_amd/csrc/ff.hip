@@ -135,16 +135,21 @@ int ff_device(bgw_ctx* ctx, int op, int nb, int nvband, int nfreqeval, int ncoul
                      ekq, dFreqGrid, static_cast<int*>(prow), static_cast<double2*>(pw), static_cast<int*>(pcnt),
                      static_cast<int*>(pslice), static_cast<int*>(pbj), static_cast<double2*>(pbw));
   if ((rc = check_launch(ctx, "k_ff_plan"))) return rc;
+  // every listed band is a row of exactly two jobs, so sum_j ceil(cnt_j/BM) is at
+  // most njobs + ceil(2*maxrows/BM), and never above njobs * ceil(maxrows/BM)
+  const long long tiles_dense = (long long)njobs * ((maxrows + BMr - 1) / BMr);
+  const long long tiles_rows = (long long)njobs + (2LL * maxrows + BMr - 1) / BMr;
+  const int max_tiles = (int)(tiles_dense < tiles_rows ? tiles_dense : tiles_rows);
   if (op == 1) {
     a.Xa = reinterpret_cast<const double2*>(I_epsR);
     a.Xb = reinterpret_cast<const double2*>(I_epsA);
     a.jobs_a = nFreq;
-    rc = contract_launch(ctx, a, njobs, (maxrows + BMr - 1) / BMr, "contract_ff_asx");
+    rc = contract_launch(ctx, a, njobs, max_tiles, "contract_ff_asx");
   } else {
     a.Xa = reinterpret_cast<const double2*>(I_epsR);
     a.Y = reinterpret_cast<const double2*>(I_epsA);
     a.jobs_a = nFreq;
-    rc = contract_launch(ctx, a, njobs, (maxrows + BMr - 1) / BMr, "contract_ff_cor");
+    rc = contract_launch(ctx, a, njobs, max_tiles, "contract_ff_cor");
   }
   if (rc) return rc;
   launch_replicate(a.out, nfreqeval, s);
