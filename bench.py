@@ -182,6 +182,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-bands", type=int, default=512)
     ap.add_argument("--no-extras", action="store_true", help="skip the FP32-bulk, full-output, FF and adjoint legs")
+    ap.add_argument("--ff-only", action="store_true",
+                    help="only run the FF leg (config 4) and print its results as one JSON line")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run a device-to-device copy of BYTES (reference for rocprofv3 FETCH/WRITE_SIZE)")
     args = ap.parse_args()
@@ -206,6 +208,15 @@ def main():
         b.copy_(a)
         torch.cuda.synchronize()
         print(json.dumps({"copy_bytes": n * 8}), flush=True)
+        return
+
+    if args.ff_only:
+        stream = torch.cuda.Stream()
+        torch.cuda.set_stream(stream)
+        ctx = bgw.Context(0)
+        ctx.set_stream(stream.cuda_stream)
+        print(json.dumps({"ff_config4": ff_leg(bgw, torch, dev, stream, ctx)}), flush=True)
+        ctx.close()
         return
 
     log("generating inputs")

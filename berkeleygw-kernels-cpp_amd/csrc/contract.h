@@ -37,12 +37,14 @@ struct ContractArgs {
   int njobs;               // set by contract_launch
   const int* tile_start;   // set by contract_launch: [njobs+1] prefix of ceil(cnt/BM)
   int items;               // set by contract_launch: workgroup tasks (= partials)
+  int chunk;               // set by contract_launch: consecutive items kept on one XCD
   double4* partial;        // set by contract_launch
   double2* out;            // c128[1] result (device)
 };
 
-// max_tiles: a bound on sum_j ceil(cnt[j] / contract_row_tile()) known on the host.
-int contract_launch(bgw_ctx* ctx, ContractArgs a, int njobs, int max_tiles, const char* tname);
+// max_tiles: a bound on sum_j ceil(cnt[j] / contract_row_tile()) known on the host,
+// exact_tiles: whether the bound is met with equality (identity row lists).
+int contract_launch(bgw_ctx* ctx, ContractArgs a, int njobs, int max_tiles, bool exact_tiles, const char* tname);
 int contract_row_tile();
 
 // rows[i] = i, wts[i] = w, cnt[0] = n, slice[0] = slice.

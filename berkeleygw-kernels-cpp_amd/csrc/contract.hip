@@ -13,6 +13,8 @@
 // The inner sum_g A*B^T is a complex NT-GEMM tile on v_mfma_f64_16x16x4f64 (four
 // real products per complex block); the weights are applied in the epilogue and
 // every workgroup writes one double-double partial, summed in fixed order.
+#include <algorithm>
+#include <cmath>
 #include <type_traits>
 
 #include "bgw_dd.h"
@@ -32,7 +34,6 @@ constexpr int BN = 64;   // columns (G') per workgroup tile
 constexpr int BK = 16;   // G-vectors per LDS stage
 constexpr int NT = 256;  // 4 waves (2 x 2), each a 32x32 sub-tile (2x2 MFMA blocks)
 constexpr int kXcds = 8;  // MI355X: 8 XCDs, workgroups dealt round-robin by linear id
-constexpr int kChunk = 8; // consecutive work items kept on one XCD
 constexpr int AE = BM * BK / NT;  // A elements staged per thread and stage
 constexpr int BE = BN * BK / NT;  // B elements staged per thread and stage
 
@@ -61,20 +62,26 @@ __global__ __launch_bounds__(NT) void k_contract(ContractArgs a) {
   __shared__ T Bs[2][BK][BN + 2];  // [re/im][k][col]
   __shared__ cdd red[NT / 64];
 
-  // Work items are (G' tile ct, K split ks, live row tile t), t fastest.  The live
+  // Work items are (G' tile ct, live row tile t, K split ks), ct fastest.  The live
   // row tiles of all jobs are numbered densely by tile_start (prefix of
   // ceil(cnt/BM)), so empty jobs and empty row-tile tails take no workgroup.
   // XCD-aware order: the hardware deals workgroup L to XCD L % 8; chunks of
-  // kChunk consecutive items are dealt round-robin over the XCDs, so the row tiles
-  // of one (slice, ct, ks) run side by side on one XCD and share their B tile
-  // through its L2, while every XCD gets an even share of every slice.
+  // a.chunk = 2*ntn consecutive items are dealt round-robin over the XCDs.  The
+  // G' tiles of a row tile (sharing its A rows) and the row tiles of a slice
+  // (sharing its B tile) then run side by side on one XCD and meet in its L2,
+  // while every XCD gets an even share of the work.
   const int slot = (int)(blockIdx.x / kXcds);
-  const int item = ((slot / kChunk) * kXcds + (int)(blockIdx.x % kXcds)) * kChunk + slot % kChunk;
+  const int item = ((slot / a.chunk) * kXcds + (int)(blockIdx.x % kXcds)) * a.chunk + slot % a.chunk;
   if (item >= a.items) return;  // grid padding to whole chunks
   const size_t pid = (size_t)item;
-  const int t = item % a.tiles;
-  const int ks = (item / a.tiles) % a.ksplit;
-  const int ct = item / a.tiles / a.ksplit;
+  const int ntn = (a.N + BN - 1) / BN;
+  const int ct = item % ntn;
+  const int t = (item / ntn) % a.tiles;
+  const int ks = item / ntn / a.tiles;
+  if (t >= a.tile_start[a.njobs]) {  // slot beyond the live tiles: a zero partial keeps the order fixed
+    if (threadIdx.x == 0) cdd_store(&a.partial[pid], cdd_zero());
+    return;
+  }
   // job = the largest j with tile_start[j] <= t (jobs without tiles repeat a value)
   int lo = 0, hi = a.njobs;
   while (lo < hi) {
@@ -83,10 +90,6 @@ __global__ __launch_bounds__(NT) void k_contract(ContractArgs a) {
       lo = mid;
     else
       hi = mid - 1;
-  }
-  if (lo >= a.njobs) {  // slot beyond the live tiles: a zero partial keeps the reduction order fixed
-    if (threadIdx.x == 0) cdd_store(&a.partial[pid], cdd_zero());
-    return;
   }
   const int job = lo;
   const int rt = t - a.tile_start[job];
@@ -294,22 +297,60 @@ __global__ __launch_bounds__(kPrefixThreads) void k_tile_prefix(const int* __res
   if (threadIdx.x == kPrefixThreads - 1) tile_start[njobs] = part[threadIdx.x];
 }
 
+// Workgroups of `kern` resident on the whole device at once.
+int resident_slots(bgw_ctx* ctx, const void* kern) {
+  int per_cu = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, NT, 0) != hipSuccess || per_cu < 1) per_cu = 1;
+  return per_cu * ctx->cus;
+}
+
+// K split minimising the estimated makespan: whole rounds of `slots` resident
+// workgroups, each item costing its BK-stages plus ~2 stages of epilogue.
+int pick_ksplit(int K, size_t tiles_x_ntn, int slots) {
+  int best = 1;
+  double best_cost = 0.0;
+  const int max_ks = std::max(1, std::min(64, K / BK));
+  for (int ks = 1; ks <= max_ks; ++ks) {
+    const double items = (double)tiles_x_ntn * ks;
+    const double rounds = std::ceil(items / slots);
+    const double stages = std::ceil(std::ceil((double)K / ks) / BK);
+    const double cost = rounds * (stages + 2.0);
+    if (ks == 1 || cost < best_cost * 0.999) {
+      best = ks;
+      best_cost = cost;
+    }
+  }
+  return best;
+}
+
 }  // namespace
 
-int contract_launch(bgw_ctx* ctx, ContractArgs a, int njobs, int max_tiles, const char* tname) {
+int contract_launch(bgw_ctx* ctx, ContractArgs a, int njobs, int max_tiles, bool exact_tiles, const char* tname) {
   const int ntn = (a.N + BN - 1) / BN;
-  // K split: ~1024 G-vectors per workgroup.  Row counts per job are only known on
-  // the device (plan kernel), so the grid covers max_tiles row tiles, a host bound
-  // on sum_j ceil(cnt[j]/BM); slots beyond the live tiles exit after a zero partial.
-  int ksplit = a.K / 1024;
-  if (ksplit < 1) ksplit = 1;
-  if (ksplit > 64) ksplit = 64;
+  const bool f32 = ctx->precision == BGW_FP32_BULK;
+  const int mode = a.Y ? kModeDiff : (a.thr2 > 0.0 ? kModeMask : kModePlain);
+  void (*kern)(ContractArgs) = nullptr;
+  if (mode == kModeDiff)
+    kern = f32 ? k_contract<kModeDiff, true> : k_contract<kModeDiff, false>;
+  else if (mode == kModeMask)
+    kern = f32 ? k_contract<kModeMask, true> : k_contract<kModeMask, false>;
+  else
+    kern = f32 ? k_contract<kModePlain, true> : k_contract<kModePlain, false>;
+  // Row counts per job are only known on the device (plan kernel), so the grid
+  // covers max_tiles row tiles, a host bound on sum_j ceil(cnt[j]/BM); slots
+  // beyond the live tiles exit after a zero partial.  When max_tiles is exact the
+  // K split is chosen for whole rounds of resident workgroups; otherwise each
+  // workgroup takes ~1024 G-vectors, which keeps sparse plans (few live tiles)
+  // spread over the device.
   if (max_tiles < 1) max_tiles = 1;
+  const int ksplit = exact_tiles ? pick_ksplit(a.K, (size_t)ntn * max_tiles, resident_slots(ctx, (const void*)kern))
+                                 : std::max(1, std::min(64, a.K / 1024));
   a.ksplit = ksplit;
   a.tiles = max_tiles;
   a.njobs = njobs;
   const size_t W = (size_t)ntn * ksplit * max_tiles;
-  const size_t per_round = (size_t)kXcds * kChunk;
+  a.chunk = 2 * ntn;
+  const size_t per_round = (size_t)kXcds * a.chunk;
   const size_t G = (W + per_round - 1) / per_round * per_round;
   if (G > (size_t)INT32_MAX) return fail(ctx, BGW_EINVAL, "contraction grid too large");
   a.items = (int)W;
@@ -327,15 +368,6 @@ int contract_launch(bgw_ctx* ctx, ContractArgs a, int njobs, int max_tiles, cons
   a.partial = static_cast<double4*>(pp);
   {
     KernelTimer kt(ctx, tname);
-    const bool f32 = ctx->precision == BGW_FP32_BULK;
-    const int mode = a.Y ? kModeDiff : (a.thr2 > 0.0 ? kModeMask : kModePlain);
-    void (*kern)(ContractArgs) = nullptr;
-    if (mode == kModeDiff)
-      kern = f32 ? k_contract<kModeDiff, true> : k_contract<kModeDiff, false>;
-    else if (mode == kModeMask)
-      kern = f32 ? k_contract<kModeMask, true> : k_contract<kModeMask, false>;
-    else
-      kern = f32 ? k_contract<kModePlain, true> : k_contract<kModePlain, false>;
     hipLaunchKernelGGL(kern, grid, dim3(NT), 0, ctx->stream, a);
   }
   if ((rc = check_launch(ctx, "k_contract"))) return rc;
@@ -374,7 +406,7 @@ int contract_achstemp(bgw_ctx* ctx, int nb, int ngpown, int ncouls, const int* i
   a.ncouls = ncouls;
   a.thr2 = kTo1 * kTo1;  // |schs| > to1 cut of the static COH term
   a.out = achstemp;
-  return contract_launch(ctx, a, 1, (nb + BM - 1) / BM, "contract_achs");
+  return contract_launch(ctx, a, 1, (nb + BM - 1) / BM, true, "contract_achs");
 }
 
 int contract_row_tile() { return BM; }

@@ -19,8 +19,6 @@
 namespace bgw {
 namespace {
 
-constexpr int kMaxLdsJobs = 4096;
-
 // op 1 (asxDtemp): bands n1 < nvband, loop 1, jobs [0,nFreq) on I_epsR, [nFreq,2nFreq) on I_epsA.
 // op 2 (achDtemp_cor): all bands, loop 2, jobs [0,nFreq) on I_epsR - I_epsA;
 //   wx>0 & unoccupied -> schDttt_corKernel1 (weights fact1,fact2),
@@ -31,10 +29,7 @@ __global__ __launch_bounds__(256) void k_ff_plan(int op, int nb, int nvband, int
                                                  double2* __restrict__ wts, int* __restrict__ cnt,
                                                  int* __restrict__ slice_of_job, int* __restrict__ band_job,
                                                  double2* __restrict__ band_w) {
-  // per-slice counters in LDS when they fit (kMaxLdsJobs), else in `cnt` itself
-  extern __shared__ int lds_cnt[];
   const int njobs = op == 1 ? 2 * nFreq : nFreq;
-  int* scnt = njobs <= kMaxLdsJobs ? lds_cnt : cnt;
   const int nbands = op == 1 ? nvband : nb;
   // 1) compute_fact for every band in parallel: band_job = (first job, or -1), band_w = (w1, w2)
   for (int n1 = threadIdx.x; n1 < nbands; n1 += blockDim.x) {
@@ -61,26 +56,32 @@ __global__ __launch_bounds__(256) void k_ff_plan(int op, int nb, int nvband, int
     band_job[n1] = job;
     band_w[n1] = make_double2(scale * f1, scale * f2);
   }
-  for (int j = threadIdx.x; j < njobs; j += blockDim.x) scnt[j] = 0;
   __syncthreads();
-  // 2) one thread appends the bands to their two slices in ascending band order
-  //    (a fixed order, so the contraction's summation order is reproducible)
-  if (threadIdx.x == 0) {
-    for (int j = 0; j < njobs; ++j) slice_of_job[j] = j % nFreq;
-    for (int n1 = 0; n1 < nbands; ++n1) {
-      const int ja = band_job[n1];
-      if (ja < 0) continue;
-      const double2 w = band_w[n1];
-      const int ca = scnt[ja], cb = scnt[ja + 1];
-      rows[(size_t)ja * nb + ca] = n1;
-      wts[(size_t)ja * nb + ca] = make_double2(w.x, 0.0);
-      rows[(size_t)(ja + 1) * nb + cb] = n1;
-      wts[(size_t)(ja + 1) * nb + cb] = make_double2(w.y, 0.0);
-      scnt[ja] = ca + 1;
-      scnt[ja + 1] = cb + 1;
+  // 2) one wave per job.  compute_fact keeps ifreq in [0, nFreq-2], so a band with
+  //    first job ja is a row of ja (weight fact1) and of ja+1 (weight fact2), both in
+  //    the same half of the asxDtemp job list.  Job j's rows are therefore the bands
+  //    with band_job == j or band_job == j-1, appended in ascending band order (a
+  //    fixed order, so the contraction's summation order is reproducible).
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwaves = blockDim.x >> 6;
+  for (int j = wave; j < njobs; j += nwaves) {
+    int c = 0;
+    for (int base = 0; base < nbands; base += 64) {
+      const int n1 = base + lane;
+      const int ja = n1 < nbands ? band_job[n1] : -1;
+      const bool first = ja == j, second = ja >= 0 && ja + 1 == j;
+      const unsigned long long m = __ballot(first || second);
+      if (first || second) {
+        const int pos = c + __popcll(m & ((1ull << lane) - 1ull));
+        const double2 w = band_w[n1];
+        rows[(size_t)j * nb + pos] = n1;
+        wts[(size_t)j * nb + pos] = make_double2(first ? w.x : w.y, 0.0);
+      }
+      c += __popcll(m);
     }
-    if (scnt != cnt)
-      for (int j = 0; j < njobs; ++j) cnt[j] = scnt[j];
+    if (lane == 0) {
+      cnt[j] = c;
+      slice_of_job[j] = j % nFreq;
+    }
   }
 }
 
@@ -125,13 +126,12 @@ int ff_device(bgw_ctx* ctx, int op, int nb, int nvband, int nfreqeval, int ncoul
                          static_cast<int*>(pslice), nb, make_double2(-0.5, 0.0), 1, s);
     a.Xa = reinterpret_cast<const double2*>(I_epsR);
     a.jobs_a = 1;
-    return contract_launch(ctx, a, 1, (nb + BMr - 1) / BMr, "contract_ff_achs");
+    return contract_launch(ctx, a, 1, (nb + BMr - 1) / BMr, true, "contract_ff_achs");
   }
   void *pbj, *pbw;
   if ((rc = pool_get(ctx, "ws_ff_band_job", (size_t)nb * sizeof(int), &pbj))) return rc;
   if ((rc = pool_get(ctx, "ws_ff_band_w", (size_t)nb * 16, &pbw))) return rc;
-  const size_t plan_lds = (size_t)(njobs <= kMaxLdsJobs ? njobs : 1) * sizeof(int);
-  hipLaunchKernelGGL(k_ff_plan, dim3(1), dim3(256), plan_lds, s, op, nb, nvband, nFreq, freqevalmin, freqevalstep, occ,
+  hipLaunchKernelGGL(k_ff_plan, dim3(1), dim3(256), 0, s, op, nb, nvband, nFreq, freqevalmin, freqevalstep, occ,
                      ekq, dFreqGrid, static_cast<int*>(prow), static_cast<double2*>(pw), static_cast<int*>(pcnt),
                      static_cast<int*>(pslice), static_cast<int*>(pbj), static_cast<double2*>(pbw));
   if ((rc = check_launch(ctx, "k_ff_plan"))) return rc;
@@ -144,12 +144,12 @@ int ff_device(bgw_ctx* ctx, int op, int nb, int nvband, int nfreqeval, int ncoul
     a.Xa = reinterpret_cast<const double2*>(I_epsR);
     a.Xb = reinterpret_cast<const double2*>(I_epsA);
     a.jobs_a = nFreq;
-    rc = contract_launch(ctx, a, njobs, max_tiles, "contract_ff_asx");
+    rc = contract_launch(ctx, a, njobs, max_tiles, false, "contract_ff_asx");
   } else {
     a.Xa = reinterpret_cast<const double2*>(I_epsR);
     a.Y = reinterpret_cast<const double2*>(I_epsA);
     a.jobs_a = nFreq;
-    rc = contract_launch(ctx, a, njobs, max_tiles, "contract_ff_cor");
+    rc = contract_launch(ctx, a, njobs, max_tiles, false, "contract_ff_cor");
   }
   if (rc) return rc;
   launch_replicate(a.out, nfreqeval, s);

@@ -68,6 +68,9 @@ def test_ff_split_fp32_vs_oracle(ctx32):
     assert abs(achs - O.achsDtemp(f)) <= TOL32 * abs(O.achsDtemp(f))
     assert rel_err(asx, O.asxDtemp(f)) < TOL32
     assert rel_err(cor, O.achDtemp_cor(f)) < TOL32
+    # and the FP32 MFMA path ran: the error is visibly above FP64 rounding
+    assert abs(achs - O.achsDtemp(f)) > 1e-13 * abs(O.achsDtemp(f))
+    assert rel_err(cor, O.achDtemp_cor(f)) > 1e-13
 
 
 @pytest.mark.slow
