@@ -10,8 +10,8 @@
 //   asxDtemp      FF/ff.cpp:242-281   (jobs = slices of I_epsR / I_epsA, w = occ*fact1|fact2)
 //   achDtemp_cor  FF/ff.cpp:332-372 + :381-451 (jobs = slices of R - A, w = fact1|fact2, *-1/2 in Kernel2)
 //   achstemp      static COH term of the GPP kernel (one job: I_eps masked, w = -1/2)
-// The inner sum_g A*B^T is a complex NT-GEMM tile on v_mfma_f64_16x16x4f64 (four
-// real products per complex block); the weights are applied in the epilogue and
+// The inner sum_g A*B^T is a complex NT-GEMM tile on v_mfma_f64_16x16x4f64 (three
+// real products per complex block, 3M scheme); the weights are applied in the epilogue and
 // every workgroup writes one double-double partial, summed in fixed order.
 #include <algorithm>
 #include <cmath>
@@ -31,11 +31,14 @@ typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 constexpr int BM = 64;   // rows (bands) per workgroup tile
 constexpr int BN = 64;   // columns (G') per workgroup tile
-constexpr int BK = 16;   // G-vectors per LDS stage
+// G-vectors per LDS stage: 32 for FP64 (a longer MFMA phase per stage hides the
+// prefetch of the next one), 16 for the FP32 form (its FP32 sums span one stage)
+template <bool F32>
+constexpr int stage_k() {
+  return F32 ? 16 : 32;
+}
 constexpr int NT = 256;  // 4 waves (2 x 2), each a 32x32 sub-tile (2x2 MFMA blocks)
 constexpr int kXcds = 8;  // MI355X: 8 XCDs, workgroups dealt round-robin by linear id
-constexpr int AE = BM * BK / NT;  // A elements staged per thread and stage
-constexpr int BE = BN * BK / NT;  // B elements staged per thread and stage
 
 template <int MODE>
 __device__ inline double2 load_b(const double2* __restrict__ X, const double2* __restrict__ Y, size_t idx,
@@ -56,8 +59,11 @@ __device__ inline double2 load_b(const double2* __restrict__ X, const double2* _
 // v_mfma_f32_16x16x4f32 with FP32 accumulation over one LDS stage (16 G-vectors),
 // folded into the FP64 accumulators after every stage.
 template <int MODE, bool F32>
-__global__ __launch_bounds__(NT) void k_contract(ContractArgs a) {
+__global__ __launch_bounds__(NT, 2) void k_contract(ContractArgs a) {
   typedef typename std::conditional<F32, float, double>::type T;
+  constexpr int BK = stage_k<F32>();
+  constexpr int AE = BM * BK / NT;  // A elements staged per thread and stage
+  constexpr int BE = BN * BK / NT;  // B elements staged per thread and stage
   __shared__ T As[2][BK][BM + 2];  // [re/im][k][row], padded
   __shared__ T Bs[2][BK][BN + 2];  // [re/im][k][col]
   __shared__ cdd red[NT / 64];
@@ -105,13 +111,16 @@ __global__ __launch_bounds__(NT) void k_contract(ContractArgs a) {
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wr = (wave >> 1) * 32, wc = (wave & 1) * 32;  // waves 2 (rows) x 2 (cols)
 
-  f64x4 dr[2][2], di[2][2];
+  // complex blocks by the 3M scheme: e = Ar*Br, f = Ai*Bi, h = (Ar+Ai)*(Br+Bi), so
+  // re = e - f and im = h - e - f take three real MFMAs instead of four
+  f64x4 de[2][2], df[2][2], dh[2][2];
 #pragma unroll
   for (int i = 0; i < 2; ++i)
 #pragma unroll
     for (int j = 0; j < 2; ++j) {
-      dr[i][j] = (f64x4){0.0, 0.0, 0.0, 0.0};
-      di[i][j] = (f64x4){0.0, 0.0, 0.0, 0.0};
+      de[i][j] = (f64x4){0.0, 0.0, 0.0, 0.0};
+      df[i][j] = (f64x4){0.0, 0.0, 0.0, 0.0};
+      dh[i][j] = (f64x4){0.0, 0.0, 0.0, 0.0};
     }
 
   // each thread stages AE elements of A and BE of B per stage: idx = tid + NT*i -> (row, k)
@@ -178,56 +187,57 @@ __global__ __launch_bounds__(NT) void k_contract(ContractArgs a) {
 #pragma unroll
       for (int kk = 0; kk < BK; kk += 4) {
         const int kr = kk + (lane >> 4), cl = lane & 15;
-        double ar[2], ai[2], nai[2], br[2], bi[2];
+        double ar[2], ai[2], sa[2], br[2], bi[2], sb[2];
 #pragma unroll
         for (int i = 0; i < 2; ++i) {
           ar[i] = As[0][kr][wr + i * 16 + cl];
           ai[i] = As[1][kr][wr + i * 16 + cl];
-          nai[i] = -ai[i];
+          sa[i] = ar[i] + ai[i];
           br[i] = Bs[0][kr][wc + i * 16 + cl];
           bi[i] = Bs[1][kr][wc + i * 16 + cl];
+          sb[i] = br[i] + bi[i];
         }
 #pragma unroll
         for (int i = 0; i < 2; ++i)
 #pragma unroll
           for (int j = 0; j < 2; ++j) {
             if (!live_r[i] || !live_c[j]) continue;
-            dr[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[i], br[j], dr[i][j], 0, 0, 0);
-            dr[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(nai[i], bi[j], dr[i][j], 0, 0, 0);
-            di[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[i], bi[j], di[i][j], 0, 0, 0);
-            di[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai[i], br[j], di[i][j], 0, 0, 0);
+            de[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[i], br[j], de[i][j], 0, 0, 0);
+            df[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai[i], bi[j], df[i][j], 0, 0, 0);
+            dh[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(sa[i], sb[j], dh[i][j], 0, 0, 0);
           }
       }
     } else {
-      f32x4 sr[2][2], si[2][2];
+      f32x4 se[2][2], sf[2][2], sh[2][2];
 #pragma unroll
       for (int i = 0; i < 2; ++i)
 #pragma unroll
         for (int j = 0; j < 2; ++j) {
-          sr[i][j] = (f32x4){0.f, 0.f, 0.f, 0.f};
-          si[i][j] = (f32x4){0.f, 0.f, 0.f, 0.f};
+          se[i][j] = (f32x4){0.f, 0.f, 0.f, 0.f};
+          sf[i][j] = (f32x4){0.f, 0.f, 0.f, 0.f};
+          sh[i][j] = (f32x4){0.f, 0.f, 0.f, 0.f};
         }
 #pragma unroll
       for (int kk = 0; kk < BK; kk += 4) {
         const int kr = kk + (lane >> 4), cl = lane & 15;
-        float ar[2], ai[2], nai[2], br[2], bi[2];
+        float ar[2], ai[2], sa[2], br[2], bi[2], sb[2];
 #pragma unroll
         for (int i = 0; i < 2; ++i) {
           ar[i] = As[0][kr][wr + i * 16 + cl];
           ai[i] = As[1][kr][wr + i * 16 + cl];
-          nai[i] = -ai[i];
+          sa[i] = ar[i] + ai[i];
           br[i] = Bs[0][kr][wc + i * 16 + cl];
           bi[i] = Bs[1][kr][wc + i * 16 + cl];
+          sb[i] = br[i] + bi[i];
         }
 #pragma unroll
         for (int i = 0; i < 2; ++i)
 #pragma unroll
           for (int j = 0; j < 2; ++j) {
             if (!live_r[i] || !live_c[j]) continue;
-            sr[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(ar[i], br[j], sr[i][j], 0, 0, 0);
-            sr[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(nai[i], bi[j], sr[i][j], 0, 0, 0);
-            si[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(ar[i], bi[j], si[i][j], 0, 0, 0);
-            si[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(ai[i], br[j], si[i][j], 0, 0, 0);
+            se[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(ar[i], br[j], se[i][j], 0, 0, 0);
+            sf[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(ai[i], bi[j], sf[i][j], 0, 0, 0);
+            sh[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(sa[i], sb[j], sh[i][j], 0, 0, 0);
           }
       }
 #pragma unroll
@@ -236,8 +246,9 @@ __global__ __launch_bounds__(NT) void k_contract(ContractArgs a) {
         for (int j = 0; j < 2; ++j)
 #pragma unroll
           for (int v = 0; v < 4; ++v) {
-            dr[i][j][v] += (double)sr[i][j][v];
-            di[i][j][v] += (double)si[i][j][v];
+            de[i][j][v] += (double)se[i][j][v];
+            df[i][j][v] += (double)sf[i][j][v];
+            dh[i][j][v] += (double)sh[i][j][v];
           }
     }
   }
@@ -262,7 +273,8 @@ __global__ __launch_bounds__(NT) void k_contract(ContractArgs a) {
         const int n1 = rows[r];
         const double2 am = a.aqsm[(size_t)n1 * a.ncouls + igp];
         const double2 w = cmul(a.wts[(size_t)job * a.max_rows + r], make_double2(vc * am.x, -vc * am.y));
-        const double2 t = cmul(w, make_double2(dr[i][j][v], di[i][j][v]));
+        const double e = de[i][j][v], f = df[i][j][v];
+        const double2 t = cmul(w, make_double2(e - f, dh[i][j][v] - e - f));
         acc = cdd_add(acc, t.x, t.y);
       }
     }
@@ -305,8 +317,8 @@ int resident_slots(bgw_ctx* ctx, const void* kern) {
 }
 
 // K split minimising the estimated makespan: whole rounds of `slots` resident
-// workgroups, each item costing its BK-stages plus ~2 stages of epilogue.
-int pick_ksplit(int K, size_t tiles_x_ntn, int slots) {
+// workgroups, each item costing its bk-wide stages plus ~2 stages of epilogue.
+int pick_ksplit(int K, int BK, size_t tiles_x_ntn, int slots) {
   int best = 1;
   double best_cost = 0.0;
   const int max_ks = std::max(1, std::min(64, K / BK));
@@ -343,7 +355,8 @@ int contract_launch(bgw_ctx* ctx, ContractArgs a, int njobs, int max_tiles, bool
   // workgroup takes ~1024 G-vectors, which keeps sparse plans (few live tiles)
   // spread over the device.
   if (max_tiles < 1) max_tiles = 1;
-  const int ksplit = exact_tiles ? pick_ksplit(a.K, (size_t)ntn * max_tiles, resident_slots(ctx, (const void*)kern))
+  const int bk = f32 ? stage_k<true>() : stage_k<false>();
+  const int ksplit = exact_tiles ? pick_ksplit(a.K, bk, (size_t)ntn * max_tiles, resident_slots(ctx, (const void*)kern))
                                  : std::max(1, std::min(64, a.K / 1024));
   a.ksplit = ksplit;
   a.tiles = max_tiles;
