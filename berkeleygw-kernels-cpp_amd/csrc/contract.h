@@ -43,8 +43,10 @@ struct ContractArgs {
 };
 
 // max_tiles: a bound on sum_j ceil(cnt[j] / contract_row_tile()) known on the host,
-// exact_tiles: whether the bound is met with equality (identity row lists).
-int contract_launch(bgw_ctx* ctx, ContractArgs a, int njobs, int max_tiles, bool exact_tiles, const char* tname);
+// exact_tiles: whether the bound is met with equality (identity row lists),
+// est_tiles: a tighter estimate of the live tiles, used to size the K split.
+int contract_launch(bgw_ctx* ctx, ContractArgs a, int njobs, int max_tiles, bool exact_tiles, int est_tiles,
+                    const char* tname);
 int contract_row_tile();
 
 // rows[i] = i, wts[i] = w, cnt[0] = n, slice[0] = slice.

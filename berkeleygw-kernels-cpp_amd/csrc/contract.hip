@@ -31,12 +31,7 @@ typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 constexpr int BM = 64;   // rows (bands) per workgroup tile
 constexpr int BN = 64;   // columns (G') per workgroup tile
-// G-vectors per LDS stage: 32 for FP64 (a longer MFMA phase per stage hides the
-// prefetch of the next one), 16 for the FP32 form (its FP32 sums span one stage)
-template <bool F32>
-constexpr int stage_k() {
-  return F32 ? 16 : 32;
-}
+constexpr int BK = 16;   // G-vectors per LDS stage (double-buffered)
 constexpr int NT = 256;  // 4 waves (2 x 2), each a 32x32 sub-tile (2x2 MFMA blocks)
 constexpr int kXcds = 8;  // MI355X: 8 XCDs, workgroups dealt round-robin by linear id
 
@@ -61,11 +56,14 @@ __device__ inline double2 load_b(const double2* __restrict__ X, const double2* _
 template <int MODE, bool F32>
 __global__ __launch_bounds__(NT, 2) void k_contract(ContractArgs a) {
   typedef typename std::conditional<F32, float, double>::type T;
-  constexpr int BK = stage_k<F32>();
   constexpr int AE = BM * BK / NT;  // A elements staged per thread and stage
   constexpr int BE = BN * BK / NT;  // B elements staged per thread and stage
-  __shared__ T As[2][BK][BM + 2];  // [re/im][k][row], padded
-  __shared__ T Bs[2][BK][BN + 2];  // [re/im][k][col]
+  // [buffer][re/im][k][row ^ swz(k)]: two stages in flight, one barrier per stage.
+  // The XOR swizzle swz(k) = 16*(k&1) + k keeps both the staging writes (16 k x 2
+  // rows per half-wave) and the MFMA operand reads (16 rows x 2 k per half-wave)
+  // on distinct LDS banks.
+  __shared__ T As[2][2][BK][BM];
+  __shared__ T Bs[2][2][BK][BN];
   __shared__ cdd red[NT / 64];
 
   // Work items are (G' tile ct, live row tile t, K split ks), ct fastest.  The live
@@ -166,35 +164,45 @@ __global__ __launch_bounds__(NT, 2) void k_contract(ContractArgs a) {
       bv[i] = (bok[i] && k < k1) ? load_b<MODE>(X, Y, bbase[i] + k, a.thr2) : make_double2(0.0, 0.0);
     }
   };
-  if (k0 < k1) fetch(k0);
-  for (int kb = k0; kb < k1; kb += BK) {
-    __syncthreads();
+  auto swz = [](int k) { return 16 * (k & 1) + k; };
+  auto stage_store = [&](int buf) {
 #pragma unroll
     for (int i = 0; i < AE; ++i) {
-      const int idx = tid + NT * i;
-      As[0][idx % BK][idx / BK] = (T)av[i].x;
-      As[1][idx % BK][idx / BK] = (T)av[i].y;
+      const int idx = tid + NT * i, k = idx % BK, col = (idx / BK) ^ swz(k);
+      As[buf][0][k][col] = (T)av[i].x;
+      As[buf][1][k][col] = (T)av[i].y;
     }
 #pragma unroll
     for (int i = 0; i < BE; ++i) {
-      const int idx = tid + NT * i;
-      Bs[0][idx % BK][idx / BK] = (T)bv[i].x;
-      Bs[1][idx % BK][idx / BK] = (T)bv[i].y;
+      const int idx = tid + NT * i, k = idx % BK, col = (idx / BK) ^ swz(k);
+      Bs[buf][0][k][col] = (T)bv[i].x;
+      Bs[buf][1][k][col] = (T)bv[i].y;
     }
-    __syncthreads();
-    if (kb + BK < k1) fetch(kb + BK);
+  };
+  if (k0 < k1) {
+    fetch(k0);
+    stage_store(0);
+  }
+  __syncthreads();
+  int buf = 0;
+  for (int kb = k0; kb < k1; kb += BK) {
+    // stage kb+BK is loaded into registers while stage kb is multiplied, then
+    // stored to the other buffer; the barrier at the end of the iteration both
+    // publishes it and retires the reads of this one
+    const bool more = kb + BK < k1;
+    if (more) fetch(kb + BK);
     if constexpr (!F32) {
 #pragma unroll
       for (int kk = 0; kk < BK; kk += 4) {
-        const int kr = kk + (lane >> 4), cl = lane & 15;
+        const int kr = kk + (lane >> 4), cl = lane & 15, sk = swz(kr);
         double ar[2], ai[2], sa[2], br[2], bi[2], sb[2];
 #pragma unroll
         for (int i = 0; i < 2; ++i) {
-          ar[i] = As[0][kr][wr + i * 16 + cl];
-          ai[i] = As[1][kr][wr + i * 16 + cl];
+          ar[i] = As[buf][0][kr][(wr + i * 16 + cl) ^ sk];
+          ai[i] = As[buf][1][kr][(wr + i * 16 + cl) ^ sk];
           sa[i] = ar[i] + ai[i];
-          br[i] = Bs[0][kr][wc + i * 16 + cl];
-          bi[i] = Bs[1][kr][wc + i * 16 + cl];
+          br[i] = Bs[buf][0][kr][(wc + i * 16 + cl) ^ sk];
+          bi[i] = Bs[buf][1][kr][(wc + i * 16 + cl) ^ sk];
           sb[i] = br[i] + bi[i];
         }
 #pragma unroll
@@ -219,15 +227,15 @@ __global__ __launch_bounds__(NT, 2) void k_contract(ContractArgs a) {
         }
 #pragma unroll
       for (int kk = 0; kk < BK; kk += 4) {
-        const int kr = kk + (lane >> 4), cl = lane & 15;
+        const int kr = kk + (lane >> 4), cl = lane & 15, sk = swz(kr);
         float ar[2], ai[2], sa[2], br[2], bi[2], sb[2];
 #pragma unroll
         for (int i = 0; i < 2; ++i) {
-          ar[i] = As[0][kr][wr + i * 16 + cl];
-          ai[i] = As[1][kr][wr + i * 16 + cl];
+          ar[i] = As[buf][0][kr][(wr + i * 16 + cl) ^ sk];
+          ai[i] = As[buf][1][kr][(wr + i * 16 + cl) ^ sk];
           sa[i] = ar[i] + ai[i];
-          br[i] = Bs[0][kr][wc + i * 16 + cl];
-          bi[i] = Bs[1][kr][wc + i * 16 + cl];
+          br[i] = Bs[buf][0][kr][(wc + i * 16 + cl) ^ sk];
+          bi[i] = Bs[buf][1][kr][(wc + i * 16 + cl) ^ sk];
           sb[i] = br[i] + bi[i];
         }
 #pragma unroll
@@ -251,6 +259,9 @@ __global__ __launch_bounds__(NT, 2) void k_contract(ContractArgs a) {
             dh[i][j][v] += (double)sh[i][j][v];
           }
     }
+    if (more) stage_store(buf ^ 1);
+    __syncthreads();
+    buf ^= 1;
   }
 
   // epilogue.  Output layouts on gfx950 (measured): v_mfma_f64_16x16x4f64 leaves
@@ -337,7 +348,8 @@ int pick_ksplit(int K, int BK, size_t tiles_x_ntn, int slots) {
 
 }  // namespace
 
-int contract_launch(bgw_ctx* ctx, ContractArgs a, int njobs, int max_tiles, bool exact_tiles, const char* tname) {
+int contract_launch(bgw_ctx* ctx, ContractArgs a, int njobs, int max_tiles, bool exact_tiles, int est_tiles,
+                    const char* tname) {
   const int ntn = (a.N + BN - 1) / BN;
   const bool f32 = ctx->precision == BGW_FP32_BULK;
   const int mode = a.Y ? kModeDiff : (a.thr2 > 0.0 ? kModeMask : kModePlain);
@@ -351,13 +363,22 @@ int contract_launch(bgw_ctx* ctx, ContractArgs a, int njobs, int max_tiles, bool
   // Row counts per job are only known on the device (plan kernel), so the grid
   // covers max_tiles row tiles, a host bound on sum_j ceil(cnt[j]/BM); slots
   // beyond the live tiles exit after a zero partial.  When max_tiles is exact the
-  // K split is chosen for whole rounds of resident workgroups; otherwise each
-  // workgroup takes ~1024 G-vectors, which keeps sparse plans (few live tiles)
-  // spread over the device.
+  // K split is chosen for whole rounds of resident workgroups.  Otherwise each
+  // workgroup takes at most ~1024 G-vectors, and plans with few live tiles
+  // (est_tiles, a host bound) are split further so that about four workgroups per
+  // resident slot keep enough loads in flight.
   if (max_tiles < 1) max_tiles = 1;
-  const int bk = f32 ? stage_k<true>() : stage_k<false>();
-  const int ksplit = exact_tiles ? pick_ksplit(a.K, bk, (size_t)ntn * max_tiles, resident_slots(ctx, (const void*)kern))
-                                 : std::max(1, std::min(64, a.K / 1024));
+  const int bk = BK;
+  const int slots = resident_slots(ctx, (const void*)kern);
+  int ksplit;
+  if (exact_tiles) {
+    ksplit = pick_ksplit(a.K, bk, (size_t)ntn * max_tiles, slots);
+  } else {
+    const long long est = std::max(1, std::min(max_tiles, est_tiles));
+    const long long want = (4LL * slots + (long long)ntn * est - 1) / ((long long)ntn * est);
+    ksplit = (int)std::max<long long>(a.K / 1024, want);
+    ksplit = std::max(1, std::min({ksplit, 64, std::max(1, a.K / bk)}));
+  }
   a.ksplit = ksplit;
   a.tiles = max_tiles;
   a.njobs = njobs;
@@ -419,7 +440,8 @@ int contract_achstemp(bgw_ctx* ctx, int nb, int ngpown, int ncouls, const int* i
   a.ncouls = ncouls;
   a.thr2 = kTo1 * kTo1;  // |schs| > to1 cut of the static COH term
   a.out = achstemp;
-  return contract_launch(ctx, a, 1, (nb + BM - 1) / BM, true, "contract_achs");
+  const int tiles = (nb + BM - 1) / BM;
+  return contract_launch(ctx, a, 1, tiles, true, tiles, "contract_achs");
 }
 
 int contract_row_tile() { return BM; }

@@ -8,6 +8,7 @@
 // each touched slice once for all bands of its list.  Because wx has no iw
 // dependence in the reference (FF/ff.cpp:245,337) each output is computed once
 // and written to all nfreqeval entries.
+#include <algorithm>
 #include <cstring>
 #include <string>
 
@@ -126,7 +127,8 @@ int ff_device(bgw_ctx* ctx, int op, int nb, int nvband, int nfreqeval, int ncoul
                          static_cast<int*>(pslice), nb, make_double2(-0.5, 0.0), 1, s);
     a.Xa = reinterpret_cast<const double2*>(I_epsR);
     a.jobs_a = 1;
-    return contract_launch(ctx, a, 1, (nb + BMr - 1) / BMr, true, "contract_ff_achs");
+    const int tiles = (nb + BMr - 1) / BMr;
+    return contract_launch(ctx, a, 1, tiles, true, tiles, "contract_ff_achs");
   }
   void *pbj, *pbw;
   if ((rc = pool_get(ctx, "ws_ff_band_job", (size_t)nb * sizeof(int), &pbj))) return rc;
@@ -140,16 +142,18 @@ int ff_device(bgw_ctx* ctx, int op, int nb, int nvband, int nfreqeval, int ncoul
   const long long tiles_dense = (long long)njobs * ((maxrows + BMr - 1) / BMr);
   const long long tiles_rows = (long long)njobs + (2LL * maxrows + BMr - 1) / BMr;
   const int max_tiles = (int)(tiles_dense < tiles_rows ? tiles_dense : tiles_rows);
+  // live tiles: at most one per listed row when rows are few (2*maxrows rows in all)
+  const int est_tiles = (int)std::min<long long>(max_tiles, 2LL * maxrows);
   if (op == 1) {
     a.Xa = reinterpret_cast<const double2*>(I_epsR);
     a.Xb = reinterpret_cast<const double2*>(I_epsA);
     a.jobs_a = nFreq;
-    rc = contract_launch(ctx, a, njobs, max_tiles, false, "contract_ff_asx");
+    rc = contract_launch(ctx, a, njobs, max_tiles, false, est_tiles, "contract_ff_asx");
   } else {
     a.Xa = reinterpret_cast<const double2*>(I_epsR);
     a.Y = reinterpret_cast<const double2*>(I_epsA);
     a.jobs_a = nFreq;
-    rc = contract_launch(ctx, a, njobs, max_tiles, false, "contract_ff_cor");
+    rc = contract_launch(ctx, a, njobs, max_tiles, false, est_tiles, "contract_ff_cor");
   }
   if (rc) return rc;
   launch_replicate(a.out, nfreqeval, s);
