@@ -192,7 +192,7 @@ __device__ inline void tile_exact(const TileEntry* __restrict__ tile, int n, con
 }
 
 // FP32-bulk form: the per-(p,g) constants and the complex products in FP32,
-// sums over runs of 32 G-vectors in FP32 folded into the FP64 accumulators.
+// sums over runs of 32 G-vectors in FP32 folded into the FP64 totals.
 // Used for a tile only when every denominator of it is far from the pole:
 // |wx - wr| >= max|wx|/4 for all bands, so u = wx - wr loses at most 2 bits in
 // FP32, and every d in [2^-20, 2^20], so products of six stay inside FP32 range;
@@ -220,16 +220,20 @@ __device__ inline bool far_from_pole(double2 w, double wxmin, double wxmax) {
   return lo >= 0.25 * scale && fma(lo, lo, wi2) >= 0x1p-20 && fma(hi, hi, wi2) <= 0x1p+20;
 }
 
+constexpr int kRun32 = 32;  // G-vectors per FP32 run
+
+// Runs of kRun32 G-vectors are summed in FP32 and folded straight into the
+// per-lane FP64 totals through sch_store1: tot += s(n1,p) * run sum.
 template <int BPL>
 __device__ inline void tile_f32(const TileEntry32* __restrict__ tile, int n, const float (&x)[BPL][3],
-                                double (&acc)[BPL][3][2]) {
-  for (int j0 = 0; j0 < n; j0 += 32) {
-    const int j1 = min(n, j0 + 32);
-    float s[BPL][3][2];
+                                const double2 (&s)[BPL], double (&tot)[BPL][3][2]) {
+  for (int j0 = 0; j0 < n; j0 += kRun32) {
+    const int j1 = min(n, j0 + kRun32);
+    float sr[BPL][3][2];
 #pragma unroll
     for (int bb = 0; bb < BPL; ++bb)
 #pragma unroll
-      for (int iw = 0; iw < 3; ++iw) s[bb][iw][0] = s[bb][iw][1] = 0.f;
+      for (int iw = 0; iw < 3; ++iw) sr[bb][iw][0] = sr[bb][iw][1] = 0.f;
 #pragma unroll 2
     for (int j = j0; j < j1; ++j) {
       const TileEntry32 e = tile[j];
@@ -242,22 +246,23 @@ __device__ inline void tile_f32(const TileEntry32* __restrict__ tile, int n, con
       batch_inverse(d, r);
 #pragma unroll
       for (int k = 0; k < BPL * 3; ++k) {
-        s[k / 3][k % 3][0] = fmaf(fmaf(e.Ir, u[k], e.a), r[k], s[k / 3][k % 3][0]);
-        s[k / 3][k % 3][1] = fmaf(fmaf(e.Ii, u[k], e.b), r[k], s[k / 3][k % 3][1]);
+        sr[k / 3][k % 3][0] = fmaf(fmaf(e.Ir, u[k], e.a), r[k], sr[k / 3][k % 3][0]);
+        sr[k / 3][k % 3][1] = fmaf(fmaf(e.Ii, u[k], e.b), r[k], sr[k / 3][k % 3][1]);
       }
     }
 #pragma unroll
     for (int bb = 0; bb < BPL; ++bb)
 #pragma unroll
       for (int iw = 0; iw < 3; ++iw) {
-        acc[bb][iw][0] += (double)s[bb][iw][0];
-        acc[bb][iw][1] += (double)s[bb][iw][1];
+        const double re = sr[bb][iw][0], im = sr[bb][iw][1];
+        tot[bb][iw][0] = fma(s[bb].x, re, fma(-s[bb].y, im, tot[bb][iw][0]));
+        tot[bb][iw][1] = fma(s[bb].x, im, fma(s[bb].y, re, tot[bb][iw][1]));
       }
   }
 }
 
 template <int NT, int BPL, bool F32>
-__global__ __launch_bounds__(NT, 4) void k_gpp_main(const double2* __restrict__ wt,
+__global__ __launch_bounds__(NT, F32 ? 6 : 4) void k_gpp_main(const double2* __restrict__ wt,
                                                  const double2* __restrict__ Ieps,
                                                  const double2* __restrict__ S,
                                                  const double* __restrict__ wxb,
@@ -294,13 +299,19 @@ __global__ __launch_bounds__(NT, 4) void k_gpp_main(const double2* __restrict__ 
     const int c = (int)(item - (long)p * nchunks);
     const int gb = c * chunk;
     const int ge = min(ncouls, gb + chunk);
+    const size_t rowoff = (size_t)p * ncouls;
+    // sch_store1(n1,p) of this lane's bands (zero for idle lanes)
+    double2 sv[BPL];
+#pragma unroll
+    for (int bb = 0; bb < BPL; ++bb) sv[bb] = n1[bb] < nb ? S[(size_t)p * nb + n1[bb]] : make_double2(0.0, 0.0);
+    // FP64 form: G-chunk sums in acc, folded with sch_store1 at the end of the item.
+    // FP32 form: every FP32 run is folded into tot at once (no per-item state).
     double acc[BPL][3][2];
 #pragma unroll
     for (int bb = 0; bb < BPL; ++bb)
 #pragma unroll
       for (int iw = 0; iw < 3; ++iw) acc[bb][iw][0] = acc[bb][iw][1] = 0.0;
 
-    const size_t rowoff = (size_t)p * ncouls;
     for (int g0 = gb; g0 < ge; g0 += NT) {
       const int g = g0 + tid;
       bool ok;
@@ -318,29 +329,46 @@ __global__ __launch_bounds__(NT, 4) void k_gpp_main(const double2* __restrict__ 
       }
       const int unsafe = __syncthreads_or(!ok);
       const int n = min(NT, ge - g0);
-      if (F32) {
+      if constexpr (F32) {
         // FP32-bulk form: FP32 tiles far from every pole, FP64 (IEEE division) otherwise
-        if (!unsafe && !__syncthreads_or(!ok32))
-          tile_f32<BPL>(tile32, n, x32, acc);
-        else
-          tile_exact<BPL>(tile, n, x, acc);
-      } else if (unsafe) {
-        tile_exact<BPL>(tile, n, x, acc);
+        if (!unsafe && !__syncthreads_or(!ok32)) {
+          tile_f32<BPL>(tile32, n, x32, sv, tot);
+        } else {
+          double xe[BPL][3], at[BPL][3][2];
+#pragma unroll
+          for (int bb = 0; bb < BPL; ++bb) {
+            const int nc = n1[bb] < nb ? n1[bb] : nb - 1;
+#pragma unroll
+            for (int iw = 0; iw < 3; ++iw) {
+              xe[bb][iw] = wxb[nc * 3 + iw];
+              at[bb][iw][0] = at[bb][iw][1] = 0.0;
+            }
+          }
+          tile_exact<BPL>(tile, n, xe, at);
+#pragma unroll
+          for (int bb = 0; bb < BPL; ++bb)
+#pragma unroll
+            for (int iw = 0; iw < 3; ++iw) {
+              tot[bb][iw][0] = fma(sv[bb].x, at[bb][iw][0], fma(-sv[bb].y, at[bb][iw][1], tot[bb][iw][0]));
+              tot[bb][iw][1] = fma(sv[bb].x, at[bb][iw][1], fma(sv[bb].y, at[bb][iw][0], tot[bb][iw][1]));
+            }
+        }
       } else {
-        tile_fast<BPL>(tile, n, x, acc);
+        if (unsafe)
+          tile_exact<BPL>(tile, n, x, acc);
+        else
+          tile_fast<BPL>(tile, n, x, acc);
       }
     }
-    // fold the G-chunk sums with sch_store1(n1,p)
+    if constexpr (!F32) {
+      // fold the G-chunk sums with sch_store1(n1,p)
 #pragma unroll
-    for (int bb = 0; bb < BPL; ++bb) {
-      if (n1[bb] < nb) {
-        const double2 s = S[(size_t)p * nb + n1[bb]];
+      for (int bb = 0; bb < BPL; ++bb)
 #pragma unroll
         for (int iw = 0; iw < 3; ++iw) {
-          tot[bb][iw][0] = fma(s.x, acc[bb][iw][0], fma(-s.y, acc[bb][iw][1], tot[bb][iw][0]));
-          tot[bb][iw][1] = fma(s.x, acc[bb][iw][1], fma(s.y, acc[bb][iw][0], tot[bb][iw][1]));
+          tot[bb][iw][0] = fma(sv[bb].x, acc[bb][iw][0], fma(-sv[bb].y, acc[bb][iw][1], tot[bb][iw][0]));
+          tot[bb][iw][1] = fma(sv[bb].x, acc[bb][iw][1], fma(sv[bb].y, acc[bb][iw][0], tot[bb][iw][1]));
         }
-      }
     }
   }
 #pragma unroll
