@@ -35,20 +35,6 @@ constexpr int BK = 16;   // G-vectors per LDS stage (double-buffered)
 constexpr int NT = 256;  // 4 waves (2 x 2), each a 32x32 sub-tile (2x2 MFMA blocks)
 constexpr int kXcds = 8;  // MI355X: 8 XCDs, workgroups dealt round-robin by linear id
 
-template <int MODE>
-__device__ inline double2 load_b(const double2* __restrict__ X, const double2* __restrict__ Y, size_t idx,
-                                 double thr2) {
-  double2 v = X[idx];
-  if (MODE == kModeDiff) {
-    const double2 y = Y[idx];
-    v.x -= y.x;
-    v.y -= y.y;
-  } else if (MODE == kModeMask) {
-    if (!(v.x * v.x + v.y * v.y > thr2)) v = make_double2(0.0, 0.0);
-  }
-  return v;
-}
-
 // F32 = false: FP64 MFMA throughout.
 // F32 = true (precision form): operands rounded to FP32 in LDS, products on
 // v_mfma_f32_16x16x4f32 with FP32 accumulation over one LDS stage (16 G-vectors),
@@ -150,38 +136,51 @@ __global__ __launch_bounds__(NT, 2) void k_contract(ContractArgs a) {
   }
 
   // software pipeline: the global loads of stage kb+BK are in flight while the
-  // MFMAs of stage kb run
-  double2 av[AE], bv[BE];
+  // MFMAs of stage kb run.  The loads are unconditional, from clamped in-range
+  // addresses, and land in registers as loaded; the R - A difference, the mask
+  // and the zeroing of out-of-range elements are applied when the stage is
+  // stored to LDS, so no wave waits on its loads before it has to.
+  double2 av[AE], bx[BE], by[MODE == kModeDiff ? BE : 1];
   auto fetch = [&](int kb) {
 #pragma unroll
     for (int i = 0; i < AE; ++i) {
-      const int k = kb + (tid + NT * i) % BK;
-      av[i] = (aok[i] && k < k1) ? a.A[(size_t)arow[i] * K + k] : make_double2(0.0, 0.0);
+      const int k = min(kb + (tid + NT * i) % BK, K - 1);
+      av[i] = a.A[(size_t)arow[i] * K + k];
     }
 #pragma unroll
     for (int i = 0; i < BE; ++i) {
-      const int k = kb + (tid + NT * i) % BK;
-      bv[i] = (bok[i] && k < k1) ? load_b<MODE>(X, Y, bbase[i] + k, a.thr2) : make_double2(0.0, 0.0);
+      const int k = min(kb + (tid + NT * i) % BK, K - 1);
+      bx[i] = X[bbase[i] + k];
+      if constexpr (MODE == kModeDiff) by[i] = Y[bbase[i] + k];
     }
   };
   auto swz = [](int k) { return 16 * (k & 1) + k; };
-  auto stage_store = [&](int buf) {
+  auto stage_store = [&](int buf, int kb) {
 #pragma unroll
     for (int i = 0; i < AE; ++i) {
       const int idx = tid + NT * i, k = idx % BK, col = (idx / BK) ^ swz(k);
-      As[buf][0][k][col] = (T)av[i].x;
-      As[buf][1][k][col] = (T)av[i].y;
+      const bool ok = aok[i] && kb + k < k1;
+      As[buf][0][k][col] = ok ? (T)av[i].x : (T)0;
+      As[buf][1][k][col] = ok ? (T)av[i].y : (T)0;
     }
 #pragma unroll
     for (int i = 0; i < BE; ++i) {
       const int idx = tid + NT * i, k = idx % BK, col = (idx / BK) ^ swz(k);
-      Bs[buf][0][k][col] = (T)bv[i].x;
-      Bs[buf][1][k][col] = (T)bv[i].y;
+      double2 v = bx[i];
+      if constexpr (MODE == kModeDiff) {
+        v.x -= by[i].x;
+        v.y -= by[i].y;
+      } else if constexpr (MODE == kModeMask) {
+        if (!(v.x * v.x + v.y * v.y > a.thr2)) v = make_double2(0.0, 0.0);
+      }
+      const bool ok = bok[i] && kb + k < k1;
+      Bs[buf][0][k][col] = ok ? (T)v.x : (T)0;
+      Bs[buf][1][k][col] = ok ? (T)v.y : (T)0;
     }
   };
   if (k0 < k1) {
     fetch(k0);
-    stage_store(0);
+    stage_store(0, k0);
   }
   __syncthreads();
   int buf = 0;
@@ -259,7 +258,7 @@ __global__ __launch_bounds__(NT, 2) void k_contract(ContractArgs a) {
             dh[i][j][v] += (double)sh[i][j][v];
           }
     }
-    if (more) stage_store(buf ^ 1);
+    if (more) stage_store(buf ^ 1, kb + BK);
     __syncthreads();
     buf ^= 1;
   }

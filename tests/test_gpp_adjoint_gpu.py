@@ -43,6 +43,17 @@ def test_near_pole_branches_vs_oracle(ctx):
     assert rel_err(g["dsigma_domega"], O.gpp_dsigma_domega(d, occ=1.0)) < 1e-12
 
 
+def test_more_bands_than_one_lds_chunk_vs_oracle(ctx):
+    # 1100 bands: the band loop stages c(n1,p) and wx in LDS chunks of 1024
+    d = bgw.gpp_inputs(1100, 3, 300, 4, 1100)
+    g = ctx.gpp_backward(d, occ=1.0)
+    e = np.arange(d["ngpown"] * d["ncouls"], dtype=np.int64)
+    dw, dI = O.gpp_grad_entries(d, e, occ=1.0)
+    assert np.max(np.abs(g["dL_dwtilde"] - dw)) <= 1e-12 * np.max(np.abs(dw))
+    assert np.max(np.abs(g["dL_dI_eps"] - dI)) <= 1e-12 * np.max(np.abs(dI))
+    assert rel_err(g["dsigma_domega"], O.gpp_dsigma_domega(d, occ=1.0)) < 1e-12
+
+
 def test_bitwise_reproducible(ctx):
     d = bgw.gpp_inputs(40, 2, 700, 9, 1)
     a = ctx.gpp_backward(d)

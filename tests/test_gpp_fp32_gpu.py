@@ -31,7 +31,7 @@ def test_config0_fp32_bulk_vs_oracle(ctx32):
     assert rel_err(r["achtemp"], o["achtemp"]) > 1e-13
 
 
-@pytest.mark.parametrize("nb", [5, 130, 300])
+@pytest.mark.parametrize("nb", [5, 130, 300, 600])
 def test_band_counts_fp32_bulk(ctx32, nb):
     d = bgw.gpp_inputs(nb, 2, 1500, 21, 40 + nb)
     r = ctx32.gpp_solver(d, outputs=("achtemp", "acht_n1_loc"))
