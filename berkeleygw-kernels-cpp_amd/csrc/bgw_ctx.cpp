@@ -94,15 +94,15 @@ int bgw_create(bgw_ctx** out, int device) {
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&ctx->own_stream, hipStreamNonBlocking);
   if (e == hipSuccess) e = hipDeviceGetAttribute(&ctx->cus, hipDeviceAttributeMultiprocessorCount, device);
   if (e == hipSuccess) e = hipMalloc(&ctx->d_flag, 256);
-  if (e != hipSuccess) {
-    delete ctx;
-    return e == hipErrorOutOfMemory ? BGW_ENOMEM : BGW_ENODEV;
-  }
   ctx->stream = ctx->own_stream;
-  reset_flag(ctx->d_flag, ctx->stream);
-  if (hipStreamSynchronize(ctx->stream) != hipSuccess) {
-    delete ctx;
-    return BGW_ENODEV;
+  if (e == hipSuccess) {
+    reset_flag(ctx->d_flag, ctx->stream);
+    e = hipStreamSynchronize(ctx->stream);
+  }
+  if (e != hipSuccess) {
+    const int code = e == hipErrorOutOfMemory ? BGW_ENOMEM : BGW_ENODEV;
+    bgw_destroy(ctx);  // releases whatever was created
+    return code;
   }
   *out = ctx;
   return BGW_OK;
