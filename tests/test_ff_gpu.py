@@ -72,6 +72,25 @@ def test_large_frequency_grid_vs_oracle(ctx):
     assert rel_err(cor, O.achDtemp_cor(f)) < TOL
 
 
+@pytest.mark.parametrize("shape", [(10, 0, 64, 16, 5, 2), (12, 3, 100, 20, 2, 3), (1, 1, 50, 7, 6, 2),
+                                   (1, 0, 50, 7, 6, 2)])
+def test_edge_shapes_vs_oracle(ctx, shape):
+    # no valence bands (asxDtemp's band loop is empty: exactly zero), the two-point
+    # frequency grid, and a single band (occupied or not)
+    f = bgw.ff_inputs(*shape, 4242)
+    achs, asx, cor = _all(ctx, f)
+    assert abs(achs - O.achsDtemp(f)) <= TOL * abs(O.achsDtemp(f))
+    oa, oc = O.asxDtemp(f), O.achDtemp_cor(f)
+    if shape[1] == 0:
+        assert np.all(asx == 0)
+    else:
+        assert rel_err(asx, oa) < TOL
+    if np.all(oc == 0):
+        assert np.all(cor == 0)
+    else:
+        assert rel_err(cor, oc) < TOL
+
+
 def test_validation(ctx):
     f = bgw.ff_inputs(10, 2, 20, 30, 5, 2, 3)
     with pytest.raises(bgw.BgwError) as e:  # ngpown > ncouls (FF/ff.cpp:492-499)
