@@ -7,6 +7,25 @@
  * Per-term arithmetic follows the reference expressions (cited per function);
  * sums are compensated: TwoSum lanes over the G-vector axis and double-double
  * accumulation over bands and G'-columns, combined in a fixed order.
+ *
+ * How each function is pinned (tests/test_oracle.py):
+ *   oracle_gpp_forward, achtemp     the reference's own noflagOCC_solver built from
+ *                                   its sources (oracle/Makefile -> _ref/), whole and
+ *                                   per band; its printed goldens (GPP/gpp.cpp:119-125);
+ *                                   closed forms of the constant inputs; exact
+ *                                   rational fixtures; committed reference vectors
+ *     acht_n1_loc                   per band against the reference build; band-sum
+ *     achstemp, asxtemp             no reference counterpart (parity unpinned against
+ *                                   the reference): exact rational fixtures reaching
+ *                                   every SX branch, closed forms, a numpy
+ *                                   restatement of the regular SX branch
+ *   oracle_gpp_grad_entries,        no reference counterpart: central finite
+ *   oracle_gpp_dsigma_domega        differences of oracle_gpp_forward, including the
+ *                                   near-pole and cut-off SX branches
+ *   oracle_compute_fact             every branch and quirk against exact rationals
+ *   oracle_achsDtemp, _asxDtemp,    the reference's FF kernels built from its sources;
+ *   _achDtemp_cor                   its printed goldens (FF/ff.cpp:72-93); closed forms;
+ *                                   exact rational fixtures; committed reference vectors
  */
 #include "bgw_oracle.h"
 
