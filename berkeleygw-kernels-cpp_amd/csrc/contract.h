@@ -35,19 +35,21 @@ struct ContractArgs {
   int ksplit;              // set by contract_launch
   int tiles;               // set by contract_launch: row-tile slots per (G' tile, K split)
   int njobs;               // set by contract_launch
-  const int* tile_start;   // set by contract_launch: [njobs+1] prefix of ceil(cnt/BM)
+  const int* tile_start;   // set by contract_launch: [njobs+1] prefix of ceil(cnt/row tile)
   int items;               // set by contract_launch: workgroup tasks (= partials)
   int chunk;               // set by contract_launch: consecutive items kept on one XCD
   double4* partial;        // set by contract_launch
   double2* out;            // c128[1] result (device)
 };
 
-// max_tiles: a bound on sum_j ceil(cnt[j] / contract_row_tile()) known on the host,
-// exact_tiles: whether the bound is met with equality (identity row lists),
+// max_tiles: a bound on sum_j ceil(cnt[j] / contract_row_tile(f32, !exact_tiles)) known
+// on the host, exact_tiles: whether the bound is met with equality (identity row lists),
 // est_tiles: a tighter estimate of the live tiles, used to size the K split.
 int contract_launch(bgw_ctx* ctx, ContractArgs a, int njobs, int max_tiles, bool exact_tiles, int est_tiles,
                     const char* tname);
-int contract_row_tile();
+// Rows (bands) per contraction tile: FP64 (f32 = false) or FP32 form, for a sparse
+// plan (frequency-slice band lists, exact_tiles = false) or a dense row list.
+int contract_row_tile(bool f32, bool sparse);
 
 // rows[i] = i, wts[i] = w, cnt[0] = n, slice[0] = slice.
 void launch_identity_rows(int* rows, double2* wts, int* cnt, int* slice, int n, double2 w, int slice_idx,

@@ -29,28 +29,41 @@ namespace {
 typedef double f64x4 __attribute__((ext_vector_type(4)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
-constexpr int BM = 64;   // rows (bands) per workgroup tile
+// Workgroup tile: BM rows (bands) x BN columns (G'), 4 waves as WR x WC, each wave
+// RB x CB 16x16 MFMA blocks.  Square form (dense row lists, and the FP32 form):
+// 64 x 64 as 2 x 2 waves of 32 x 32, the fewest LDS operand reads per MFMA.  Tall
+// form (FP64 sparse plans): 80 x 64 with each wave owning one 16-column block of
+// all 80 rows -- a frequency slice's band list (two neighbouring frequency groups,
+// ~60-80 bands) then fits one tile and its B tile is streamed once.
+template <bool TALL>
+struct TileShape {
+  static constexpr int WR = TALL ? 1 : 2, WC = TALL ? 4 : 2;
+  static constexpr int RB = TALL ? 5 : 2, CB = TALL ? 1 : 2;
+  static constexpr int BM = WR * RB * 16;
+  static constexpr int LW = (BM + 31) / 32 * 32;  // LDS row width: the swizzle permutes within 32
+};
 constexpr int BN = 64;   // columns (G') per workgroup tile
 constexpr int BK = 16;   // G-vectors per LDS stage (double-buffered)
-constexpr int NT = 256;  // 4 waves (2 x 2), each a 32x32 sub-tile (2x2 MFMA blocks)
+constexpr int NT = 256;  // 4 waves
 constexpr int kXcds = 8;  // MI355X: 8 XCDs, workgroups dealt round-robin by linear id
 
 // F32 = false: FP64 MFMA throughout.
 // F32 = true (precision form): operands rounded to FP32 in LDS, products on
 // v_mfma_f32_16x16x4f32 with FP32 accumulation over one LDS stage (16 G-vectors),
 // folded into the FP64 accumulators after every stage.
-template <int MODE, bool F32>
+template <int MODE, bool F32, bool TALL>
 __global__ __launch_bounds__(NT, 2) void k_contract(ContractArgs a) {
   typedef typename std::conditional<F32, float, double>::type T;
+  typedef TileShape<TALL> TS;
+  constexpr int BM = TS::BM, RB = TS::RB, CB = TS::CB;
   constexpr int AE = BM * BK / NT;  // A elements staged per thread and stage
   constexpr int BE = BN * BK / NT;  // B elements staged per thread and stage
   // [buffer][re/im][k][row ^ swz(k)]: two stages in flight, one barrier per stage.
   // The XOR swizzle swz(k) = 16*(k&1) + k keeps both the staging writes (16 k x 2
   // rows per half-wave) and the MFMA operand reads (16 rows x 2 k per half-wave)
   // on distinct LDS banks.
-  __shared__ T As[2][2][BK][BM];
-  __shared__ T Bs[2][2][BK][BN];
-  __shared__ cdd red[NT / 64];
+  __shared__ __align__(16) T As[2][2][BK][TS::LW];
+  __shared__ __align__(16) T Bs[2][2][BK][BN];
 
   // Work items are (G' tile ct, live row tile t, K split ks), ct fastest.  The live
   // row tiles of all jobs are numbered densely by tile_start (prefix of
@@ -93,15 +106,15 @@ __global__ __launch_bounds__(NT, 2) void k_contract(ContractArgs a) {
   const int k0 = ks * kper, k1 = min(K, k0 + kper);
 
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int wr = (wave >> 1) * 32, wc = (wave & 1) * 32;  // waves 2 (rows) x 2 (cols)
+  const int wr = (wave / TS::WC) * (RB * 16), wc = (wave % TS::WC) * (CB * 16);
 
   // complex blocks by the 3M scheme: e = Ar*Br, f = Ai*Bi, h = (Ar+Ai)*(Br+Bi), so
   // re = e - f and im = h - e - f take three real MFMAs instead of four
-  f64x4 de[2][2], df[2][2], dh[2][2];
+  f64x4 de[RB][CB], df[RB][CB], dh[RB][CB];
 #pragma unroll
-  for (int i = 0; i < 2; ++i)
+  for (int i = 0; i < RB; ++i)
 #pragma unroll
-    for (int j = 0; j < 2; ++j) {
+    for (int j = 0; j < CB; ++j) {
       de[i][j] = (f64x4){0.0, 0.0, 0.0, 0.0};
       df[i][j] = (f64x4){0.0, 0.0, 0.0, 0.0};
       dh[i][j] = (f64x4){0.0, 0.0, 0.0, 0.0};
@@ -128,12 +141,11 @@ __global__ __launch_bounds__(NT, 2) void k_contract(ContractArgs a) {
 
   // 16x16 blocks of this wave that hold at least one live row / column: the
   // others (row-tile tails of short band lists, the last G' tile) skip their MFMAs
-  bool live_r[2], live_c[2];
+  bool live_r[RB], live_c[CB];
 #pragma unroll
-  for (int i = 0; i < 2; ++i) {
-    live_r[i] = rt * BM + wr + i * 16 < nrows;
-    live_c[i] = ct * BN + wc + i * 16 < a.N;
-  }
+  for (int i = 0; i < RB; ++i) live_r[i] = rt * BM + wr + i * 16 < nrows;
+#pragma unroll
+  for (int j = 0; j < CB; ++j) live_c[j] = ct * BN + wc + j * 16 < a.N;
 
   // software pipeline: the global loads of stage kb+BK are in flight while the
   // MFMAs of stage kb run.  The loads are unconditional, from clamped in-range
@@ -194,20 +206,23 @@ __global__ __launch_bounds__(NT, 2) void k_contract(ContractArgs a) {
 #pragma unroll
       for (int kk = 0; kk < BK; kk += 4) {
         const int kr = kk + (lane >> 4), cl = lane & 15, sk = swz(kr);
-        double ar[2], ai[2], sa[2], br[2], bi[2], sb[2];
+        double ar[RB], ai[RB], sa[RB], br[CB], bi[CB], sb[CB];
 #pragma unroll
-        for (int i = 0; i < 2; ++i) {
+        for (int i = 0; i < RB; ++i) {
           ar[i] = As[buf][0][kr][(wr + i * 16 + cl) ^ sk];
           ai[i] = As[buf][1][kr][(wr + i * 16 + cl) ^ sk];
           sa[i] = ar[i] + ai[i];
-          br[i] = Bs[buf][0][kr][(wc + i * 16 + cl) ^ sk];
-          bi[i] = Bs[buf][1][kr][(wc + i * 16 + cl) ^ sk];
-          sb[i] = br[i] + bi[i];
         }
 #pragma unroll
-        for (int i = 0; i < 2; ++i)
+        for (int j = 0; j < CB; ++j) {
+          br[j] = Bs[buf][0][kr][(wc + j * 16 + cl) ^ sk];
+          bi[j] = Bs[buf][1][kr][(wc + j * 16 + cl) ^ sk];
+          sb[j] = br[j] + bi[j];
+        }
 #pragma unroll
-          for (int j = 0; j < 2; ++j) {
+        for (int i = 0; i < RB; ++i)
+#pragma unroll
+          for (int j = 0; j < CB; ++j) {
             if (!live_r[i] || !live_c[j]) continue;
             de[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[i], br[j], de[i][j], 0, 0, 0);
             df[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai[i], bi[j], df[i][j], 0, 0, 0);
@@ -215,11 +230,11 @@ __global__ __launch_bounds__(NT, 2) void k_contract(ContractArgs a) {
           }
       }
     } else {
-      f32x4 se[2][2], sf[2][2], sh[2][2];
+      f32x4 se[RB][CB], sf[RB][CB], sh[RB][CB];
 #pragma unroll
-      for (int i = 0; i < 2; ++i)
+      for (int i = 0; i < RB; ++i)
 #pragma unroll
-        for (int j = 0; j < 2; ++j) {
+        for (int j = 0; j < CB; ++j) {
           se[i][j] = (f32x4){0.f, 0.f, 0.f, 0.f};
           sf[i][j] = (f32x4){0.f, 0.f, 0.f, 0.f};
           sh[i][j] = (f32x4){0.f, 0.f, 0.f, 0.f};
@@ -227,20 +242,23 @@ __global__ __launch_bounds__(NT, 2) void k_contract(ContractArgs a) {
 #pragma unroll
       for (int kk = 0; kk < BK; kk += 4) {
         const int kr = kk + (lane >> 4), cl = lane & 15, sk = swz(kr);
-        float ar[2], ai[2], sa[2], br[2], bi[2], sb[2];
+        float ar[RB], ai[RB], sa[RB], br[CB], bi[CB], sb[CB];
 #pragma unroll
-        for (int i = 0; i < 2; ++i) {
+        for (int i = 0; i < RB; ++i) {
           ar[i] = As[buf][0][kr][(wr + i * 16 + cl) ^ sk];
           ai[i] = As[buf][1][kr][(wr + i * 16 + cl) ^ sk];
           sa[i] = ar[i] + ai[i];
-          br[i] = Bs[buf][0][kr][(wc + i * 16 + cl) ^ sk];
-          bi[i] = Bs[buf][1][kr][(wc + i * 16 + cl) ^ sk];
-          sb[i] = br[i] + bi[i];
         }
 #pragma unroll
-        for (int i = 0; i < 2; ++i)
+        for (int j = 0; j < CB; ++j) {
+          br[j] = Bs[buf][0][kr][(wc + j * 16 + cl) ^ sk];
+          bi[j] = Bs[buf][1][kr][(wc + j * 16 + cl) ^ sk];
+          sb[j] = br[j] + bi[j];
+        }
 #pragma unroll
-          for (int j = 0; j < 2; ++j) {
+        for (int i = 0; i < RB; ++i)
+#pragma unroll
+          for (int j = 0; j < CB; ++j) {
             if (!live_r[i] || !live_c[j]) continue;
             se[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(ar[i], br[j], se[i][j], 0, 0, 0);
             sf[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(ai[i], bi[j], sf[i][j], 0, 0, 0);
@@ -248,9 +266,9 @@ __global__ __launch_bounds__(NT, 2) void k_contract(ContractArgs a) {
           }
       }
 #pragma unroll
-      for (int i = 0; i < 2; ++i)
+      for (int i = 0; i < RB; ++i)
 #pragma unroll
-        for (int j = 0; j < 2; ++j)
+        for (int j = 0; j < CB; ++j)
 #pragma unroll
           for (int v = 0; v < 4; ++v) {
             de[i][j][v] += (double)se[i][j][v];
@@ -269,13 +287,13 @@ __global__ __launch_bounds__(NT, 2) void k_contract(ContractArgs a) {
   // path folds its accumulators element-wise, so it keeps the FP32 layout.
   cdd acc = cdd_zero();
 #pragma unroll
-  for (int j = 0; j < 2; ++j) {
+  for (int j = 0; j < CB; ++j) {
     const int p = ct * BN + wc + j * 16 + (lane & 15);
     if (p >= a.N) continue;
     const int igp = a.gpp_index ? gpp_igp(a.inv, a.indinv, p, a.ncouls) : ff_igp(a.inv, a.indinv, p, a.ncouls);
     const double vc = a.vcoul[igp];
 #pragma unroll
-    for (int i = 0; i < 2; ++i) {
+    for (int i = 0; i < RB; ++i) {
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
         const int r = rt * BM + wr + i * 16 + (F32 ? 4 * (lane >> 4) + v : (lane >> 4) + 4 * v);
@@ -289,20 +307,22 @@ __global__ __launch_bounds__(NT, 2) void k_contract(ContractArgs a) {
       }
     }
   }
-  cdd tot = cdd_block_reduce(acc, red);
+  // the stage buffers are free after the loop's last barrier: the reduction
+  // scratch lives in them (two FP64 workgroups fill the 160 KB of LDS of a CU)
+  cdd tot = cdd_block_reduce(acc, reinterpret_cast<cdd*>(&As[0][0][0][0]));
   if (tid == 0) cdd_store(&a.partial[pid], tot);
 }
 
-// tile_start[j] = sum_{i<j} ceil(cnt[i]/BM), j <= njobs: each thread sums a run
+// tile_start[j] = sum_{i<j} ceil(cnt[i]/bm), j <= njobs: each thread sums a run
 // of consecutive jobs, then one LDS scan over the runs.
 constexpr int kPrefixThreads = 1024;
-__global__ __launch_bounds__(kPrefixThreads) void k_tile_prefix(const int* __restrict__ cnt, int njobs,
+__global__ __launch_bounds__(kPrefixThreads) void k_tile_prefix(const int* __restrict__ cnt, int njobs, int bm,
                                                                int* __restrict__ tile_start) {
   __shared__ int part[kPrefixThreads];
   const int per = (njobs + kPrefixThreads - 1) / kPrefixThreads;
   const int j0 = min(njobs, (int)threadIdx.x * per), j1 = min(njobs, j0 + per);
   int s = 0;
-  for (int j = j0; j < j1; ++j) s += (cnt[j] + BM - 1) / BM;
+  for (int j = j0; j < j1; ++j) s += (cnt[j] + bm - 1) / bm;
   part[threadIdx.x] = s;
   __syncthreads();
   for (int off = 1; off < kPrefixThreads; off <<= 1) {
@@ -314,7 +334,7 @@ __global__ __launch_bounds__(kPrefixThreads) void k_tile_prefix(const int* __res
   int run = part[threadIdx.x] - s;  // exclusive prefix of this thread's run
   for (int j = j0; j < j1; ++j) {
     tile_start[j] = run;
-    run += (cnt[j] + BM - 1) / BM;
+    run += (cnt[j] + bm - 1) / bm;
   }
   if (threadIdx.x == kPrefixThreads - 1) tile_start[njobs] = part[threadIdx.x];
 }
@@ -352,15 +372,20 @@ int contract_launch(bgw_ctx* ctx, ContractArgs a, int njobs, int max_tiles, bool
   const int ntn = (a.N + BN - 1) / BN;
   const bool f32 = ctx->precision == BGW_FP32_BULK;
   const int mode = a.Y ? kModeDiff : (a.thr2 > 0.0 ? kModeMask : kModePlain);
+  // the tall tile for FP64 sparse plans (frequency-slice band lists), square otherwise
+  const bool tall = !f32 && !exact_tiles;
   void (*kern)(ContractArgs) = nullptr;
   if (mode == kModeDiff)
-    kern = f32 ? k_contract<kModeDiff, true> : k_contract<kModeDiff, false>;
+    kern = f32 ? k_contract<kModeDiff, true, false>
+               : (tall ? k_contract<kModeDiff, false, true> : k_contract<kModeDiff, false, false>);
   else if (mode == kModeMask)
-    kern = f32 ? k_contract<kModeMask, true> : k_contract<kModeMask, false>;
+    kern = f32 ? k_contract<kModeMask, true, false>
+               : (tall ? k_contract<kModeMask, false, true> : k_contract<kModeMask, false, false>);
   else
-    kern = f32 ? k_contract<kModePlain, true> : k_contract<kModePlain, false>;
+    kern = f32 ? k_contract<kModePlain, true, false>
+               : (tall ? k_contract<kModePlain, false, true> : k_contract<kModePlain, false, false>);
   // Row counts per job are only known on the device (plan kernel), so the grid
-  // covers max_tiles row tiles, a host bound on sum_j ceil(cnt[j]/BM); slots
+  // covers max_tiles row tiles, a host bound on sum_j ceil(cnt[j]/bm); slots
   // beyond the live tiles exit after a zero partial.  When max_tiles is exact the
   // K split is chosen for whole rounds of resident workgroups.  Otherwise each
   // workgroup takes at most ~1024 G-vectors, and plans with few live tiles
@@ -392,7 +417,7 @@ int contract_launch(bgw_ctx* ctx, ContractArgs a, int njobs, int max_tiles, bool
   int rc = pool_get(ctx, "ws_tile_start", (size_t)(njobs + 1) * sizeof(int), &pts);
   if (rc) return rc;
   a.tile_start = static_cast<const int*>(pts);
-  hipLaunchKernelGGL(k_tile_prefix, dim3(1), dim3(kPrefixThreads), 0, ctx->stream, a.cnt, njobs,
+  hipLaunchKernelGGL(k_tile_prefix, dim3(1), dim3(kPrefixThreads), 0, ctx->stream, a.cnt, njobs, contract_row_tile(f32, !exact_tiles),
                      static_cast<int*>(pts));
   if ((rc = check_launch(ctx, "k_tile_prefix"))) return rc;
   void* pp;
@@ -439,10 +464,13 @@ int contract_achstemp(bgw_ctx* ctx, int nb, int ngpown, int ncouls, const int* i
   a.ncouls = ncouls;
   a.thr2 = kTo1 * kTo1;  // |schs| > to1 cut of the static COH term
   a.out = achstemp;
-  const int tiles = (nb + BM - 1) / BM;
+  const int bm = contract_row_tile(ctx->precision == BGW_FP32_BULK, false);
+  const int tiles = (nb + bm - 1) / bm;
   return contract_launch(ctx, a, 1, tiles, true, tiles, "contract_achs");
 }
 
-int contract_row_tile() { return BM; }
+int contract_row_tile(bool f32, bool sparse) {
+  return !f32 && sparse ? TileShape<true>::BM : TileShape<false>::BM;
+}
 
 }  // namespace bgw

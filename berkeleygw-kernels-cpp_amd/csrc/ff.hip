@@ -107,7 +107,7 @@ int ff_device(bgw_ctx* ctx, int op, int nb, int nvband, int nfreqeval, int ncoul
   a.ncouls = ncouls;
   a.thr2 = 0.0;
   a.out = reinterpret_cast<double2*>(out);
-  const int BMr = contract_row_tile();
+  const bool f32 = ctx->precision == BGW_FP32_BULK;
   void *prow, *pw, *pcnt, *pslice;
   const int njobs = op == 0 ? 1 : (op == 1 ? 2 * nFreq : nFreq);
   const int maxrows = op == 0 ? nb : (op == 1 ? nvband : nb);
@@ -127,7 +127,8 @@ int ff_device(bgw_ctx* ctx, int op, int nb, int nvband, int nfreqeval, int ncoul
                          static_cast<int*>(pslice), nb, make_double2(-0.5, 0.0), 1, s);
     a.Xa = reinterpret_cast<const double2*>(I_epsR);
     a.jobs_a = 1;
-    const int tiles = (nb + BMr - 1) / BMr;
+    const int BMd = contract_row_tile(f32, false);
+    const int tiles = (nb + BMd - 1) / BMd;
     return contract_launch(ctx, a, 1, tiles, true, tiles, "contract_ff_achs");
   }
   void *pbj, *pbw;
@@ -139,6 +140,7 @@ int ff_device(bgw_ctx* ctx, int op, int nb, int nvband, int nfreqeval, int ncoul
   if ((rc = check_launch(ctx, "k_ff_plan"))) return rc;
   // every listed band is a row of exactly two jobs, so sum_j ceil(cnt_j/BM) is at
   // most njobs + ceil(2*maxrows/BM), and never above njobs * ceil(maxrows/BM)
+  const int BMr = contract_row_tile(f32, true);
   const long long tiles_dense = (long long)njobs * ((maxrows + BMr - 1) / BMr);
   const long long tiles_rows = (long long)njobs + (2LL * maxrows + BMr - 1) / BMr;
   const int max_tiles = (int)(tiles_dense < tiles_rows ? tiles_dense : tiles_rows);
