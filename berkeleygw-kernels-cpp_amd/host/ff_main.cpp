@@ -6,6 +6,8 @@
 // selects the seeded benchmark distributions.  The reference's one-sided golden
 // checks (FF/ff.cpp:68-101, one of which has the wrong sign) are replaced by
 // two-sided checks against the exact closed forms of the constant inputs.
+// --dump FILE writes the generated inputs as an input set (see bgw.h); --input FILE
+// reads one instead of generating inputs (no size arguments).
 #include <chrono>
 #include <cmath>
 #include <complex>
@@ -17,12 +19,57 @@
 #include "bgw.h"
 #include "bgw_reference_api.hpp"
 #include "host_array.hpp"
+#include "input_set.hpp"
 
 using cplx = std::complex<double>;
 using clk = std::chrono::steady_clock;
 
 static void usage() {
-  std::printf("./ff.ex number_bands nvband ncouls ngpown nFreq nfreqeval [--random SEED] [--precision fp64|fp32]\n");
+  std::printf("./ff.ex number_bands nvband ncouls ngpown nFreq nfreqeval [--random SEED] [--precision fp64|fp32] "
+              "[--dump FILE]\n./ff.ex --input FILE [--precision fp64|fp32]\n");
+}
+
+static double since(clk::time_point t) { return std::chrono::duration<double>(clk::now() - t).count(); }
+
+// --input FILE: the arrays stay in the file's read-only mapping and go to the
+// host-pointer entry points from there.
+static int run_input_set(const char* path, int precision) {
+  bgw::InputSetFile in(path, BGW_INPUTS_FF);
+  const int number_bands = (int)in.i64("number_bands"), nvband = (int)in.i64("nvband");
+  const int ncouls = (int)in.i64("ncouls"), ngpown = (int)in.i64("ngpown");
+  const int nFreq = (int)in.i64("nFreq"), nfreqeval = (int)in.i64("nfreqeval");
+  const double freqevalmin = in.f64("freqevalmin"), freqevalstep = in.f64("freqevalstep"), occ = in.f64("occ");
+  std::printf(" number_bands = %d\n nvband = %d\n ncouls = %d\n ngpown = %d\n nFreq = %d\n nfreqeval = %d\n"
+              " (input set %s)\n", number_bands, nvband, ncouls, ngpown, nFreq, nfreqeval, path);
+  bgw_ctx* ctx = bgw::default_context();
+  bgw::check(bgw_set_precision(ctx, precision));
+  const int* inv = in.array<int>("inv_igp_index");
+  const int* indinv = in.array<int>("indinv");
+  const double *aqsm = in.array<double>("aqsmtemp"), *aqsn = in.array<double>("aqsntemp");
+  const double *vcoul = in.array<double>("vcoul"), *ekq = in.array<double>("ekq");
+  const double* grid = in.array<double>("dFreqGrid");
+  const double *R = in.array<double>("I_epsR_array"), *A = in.array<double>("I_epsA_array");
+  std::vector<cplx> asx(nfreqeval), cor(nfreqeval);
+  cplx achs;
+  auto tk = clk::now();
+  auto t = clk::now();
+  bgw::check(bgw_achsDtemp_Kernel(ctx, number_bands, ngpown, ncouls, inv, indinv, aqsn, aqsm, R, vcoul,
+                                  reinterpret_cast<double*>(&achs)));
+  std::printf("********** achsDtemp Time Taken **********= %g secs\n", since(t));
+  t = clk::now();
+  bgw::check(bgw_asxDtemp_Kernel(ctx, nvband, nfreqeval, ncouls, ngpown, nFreq, freqevalmin, freqevalstep, occ, ekq,
+                                 grid, inv, indinv, aqsm, aqsn, vcoul, R, A, reinterpret_cast<double*>(asx.data())));
+  std::printf("********** asxDtemp Time Taken **********= %g secs\n", since(t));
+  t = clk::now();
+  bgw::check(bgw_achDtemp_cor_Kernel(ctx, number_bands, nvband, nfreqeval, ncouls, ngpown, nFreq, freqevalmin,
+                                     freqevalstep, ekq, grid, inv, indinv, aqsm, aqsn, vcoul, R, A,
+                                     reinterpret_cast<double*>(cor.data())));
+  std::printf("********** achDtemp_cor Time Taken **********= %g secs\n", since(t));
+  const double kernel_s = since(tk);
+  std::printf("achsDtemp = (%f,%f) \nasxDtemp = (%f,%f) \nachDtemp_cor = (%f,%f) \n", achs.real(), achs.imag(),
+              asx[0].real(), asx[0].imag(), cor[0].real(), cor[0].imag());
+  std::printf("********** Kernel  Time Taken (Includes data allocation) **********= %g secs\n", kernel_s);
+  return 0;
 }
 
 static void verdict(const char* name, cplx got, cplx exact, double tol) {
@@ -37,9 +84,14 @@ int main(int argc, char** argv) {
   std::vector<std::string> pos;
   long long seed = -1;
   int precision = BGW_FP64;
+  std::string input_path, dump_path;
   for (int i = 1; i < argc; ++i) {
     std::string a = argv[i];
-    if (a == "--random" && i + 1 < argc)
+    if (a == "--input" && i + 1 < argc)
+      input_path = argv[++i];
+    else if (a == "--dump" && i + 1 < argc)
+      dump_path = argv[++i];
+    else if (a == "--random" && i + 1 < argc)
       seed = std::atoll(argv[++i]);
     else if (a == "--precision" && i + 1 < argc)
       precision = std::string(argv[++i]) == "fp32" ? BGW_FP32_BULK : BGW_FP64;
@@ -48,6 +100,13 @@ int main(int argc, char** argv) {
       return 0;
     } else
       pos.push_back(a);
+  }
+  if (!input_path.empty()) {
+    if (!pos.empty() || seed >= 0 || !dump_path.empty()) {
+      usage();
+      return 1;
+    }
+    return run_input_set(input_path.c_str(), precision);
   }
   int number_bands = 15023, nvband = 1998, ncouls = 22401, ngpown = 66, nFreq = 15, nfreqeval = 10;
   if (pos.size() == 6) {
@@ -103,6 +162,30 @@ int main(int argc, char** argv) {
     bgw_synth_complex(s + 9, 0, 0.5, -0.1, 0.1, reinterpret_cast<double*>(I_epsR_array.dptr), I_epsR_array.size);
     bgw_synth_complex(s + 10, 0, 0.5, -0.1, 0.1, reinterpret_cast<double*>(I_epsA_array.dptr), I_epsA_array.size);
     for (int f = 0; f < nFreq; ++f) dFreqGrid(f) = 2.0 * f / (nFreq - 1);
+  }
+  if (!dump_path.empty()) {
+    bgw::InputSetWriter w;
+    w.scalar_i64("number_bands", number_bands);
+    w.scalar_i64("nvband", nvband);
+    w.scalar_i64("ncouls", ncouls);
+    w.scalar_i64("ngpown", ngpown);
+    w.scalar_i64("nFreq", nFreq);
+    w.scalar_i64("nfreqeval", nfreqeval);
+    w.scalar_f64("freqevalmin", freqevalmin);
+    w.scalar_f64("freqevalstep", freqevalstep);
+    w.scalar_f64("occ", occ);
+    const size_t B = number_bands, C = ncouls, G = ngpown, F = nFreq;
+    w.array("inv_igp_index", BGW_DT_I32, inv_igp_index.dptr, {G});
+    w.array("indinv", BGW_DT_I32, indinv.dptr, {C});
+    w.array("ekq", BGW_DT_F64, ekq.dptr, {B});
+    w.array("dFreqGrid", BGW_DT_F64, dFreqGrid.dptr, {F});
+    w.array("aqsmtemp", BGW_DT_C128, aqsmtemp.dptr, {B, C});
+    w.array("aqsntemp", BGW_DT_C128, aqsntemp.dptr, {B, C});
+    w.array("vcoul", BGW_DT_F64, vcoul.dptr, {C});
+    w.array("I_epsR_array", BGW_DT_C128, I_epsR_array.dptr, {F, G, C});
+    w.array("I_epsA_array", BGW_DT_C128, I_epsA_array.dptr, {F, G, C});
+    w.write(dump_path.c_str(), BGW_INPUTS_FF);
+    std::printf("inputs written to %s\n", dump_path.c_str());
   }
   bgw::check(bgw_set_precision(bgw::default_context(), precision));
   std::printf("pre kernel time taken = %g secs\nstarting Kernels\n",

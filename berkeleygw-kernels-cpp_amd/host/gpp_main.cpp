@@ -6,7 +6,9 @@
 // selects the seeded benchmark distributions.  The one-sided golden check of the
 // reference (GPP/gpp.cpp:117-131) is replaced by a two-sided check against the
 // exact closed form of the constant inputs.  Extra flags: --random SEED,
-// --precision fp64|fp32, --all (also prints acht_n1_loc sum, achstemp, asxtemp).
+// --precision fp64|fp32, --all (also prints acht_n1_loc sum, achstemp, asxtemp),
+// --dump FILE (writes the generated inputs as an input set, see bgw.h) and
+// --input FILE (reads an input set instead of generating inputs; no size arguments).
 #include <chrono>
 #include <cmath>
 #include <complex>
@@ -19,13 +21,65 @@
 #include "bgw.h"
 #include "bgw_reference_api.hpp"
 #include "host_array.hpp"
+#include "input_set.hpp"
 
 using cplx = std::complex<double>;
 using clk = std::chrono::steady_clock;
 
 static void usage() {
   std::printf("The correct form of input is :\n ./gpp.ex <number_bands> <number_valence_bands> "
-              "<number_plane_waves> <nodes_per_mpi_group> [--random SEED] [--precision fp64|fp32] [--all]\n");
+              "<number_plane_waves> <nodes_per_mpi_group> [--random SEED] [--precision fp64|fp32] [--all] "
+              "[--dump FILE]\n ./gpp.ex --input FILE [--precision fp64|fp32] [--all]\n");
+}
+
+static void print_extras(int number_bands, int nvband, int ngpown, int ncouls, const int* inv, const int* indinv,
+                         const double* wx_all, const double* wtilde, const double* aqsm, const double* aqsn,
+                         const double* I_eps, const double* vcoul, double occ) {
+  std::vector<cplx> loc(number_bands * 3);
+  cplx ach[3], achs, asx[3];
+  bgw::check(bgw_gpp_solver(bgw::default_context(), number_bands, nvband, ngpown, ncouls, inv, indinv, wx_all,
+                            wtilde, aqsm, aqsn, I_eps, vcoul, occ, reinterpret_cast<double*>(ach),
+                            reinterpret_cast<double*>(loc.data()), reinterpret_cast<double*>(&achs),
+                            reinterpret_cast<double*>(asx)));
+  cplx sum = 0;
+  for (int n1 = 0; n1 < number_bands; ++n1) sum += loc[n1 * 3];
+  std::printf(" sum_n1 acht_n1_loc[n1][0] = (%f,%f)\n achstemp = (%f,%f)\n asxtemp[0] = (%f,%f)\n", sum.real(),
+              sum.imag(), achs.real(), achs.imag(), asx[0].real(), asx[0].imag());
+}
+
+// --input FILE: the arrays stay in the file's read-only mapping and go to the
+// host-pointer entry point from there.
+static int run_input_set(const char* path, int precision, bool all) {
+  auto t_start = clk::now();
+  bgw::InputSetFile in(path, BGW_INPUTS_GPP);
+  const int number_bands = (int)in.i64("number_bands"), nvband = (int)in.i64("nvband");
+  const int ncouls = (int)in.i64("ncouls"), ngpown = (int)in.i64("ngpown");
+  const double occ = in.f64("occ");
+  std::printf("number_bands = %d\t nvband = %d\t ncouls = %d\t ngpown = %d\t nend = 3\t nstart = 0\t (input set %s)\n",
+              number_bands, nvband, ncouls, ngpown, path);
+  const bgw_array_desc wx = in.entry("wx_array");
+  std::vector<double> wx_all((size_t)number_bands * 3);
+  const double* w = static_cast<const double*>(wx.data);
+  for (int n1 = 0; n1 < number_bands; ++n1)
+    for (int iw = 0; iw < 3; ++iw) wx_all[n1 * 3 + iw] = wx.ndim == 1 ? w[iw] : w[n1 * 3 + iw];
+  bgw::check(bgw_set_precision(bgw::default_context(), precision));
+  cplx achtemp[3];
+  auto k_start = clk::now();
+  bgw::check(bgw_gpp_solver(bgw::default_context(), number_bands, nvband, ngpown, ncouls,
+                            in.array<int>("inv_igp_index"), in.array<int>("indinv"), wx_all.data(),
+                            in.array<double>("wtilde_array"), in.array<double>("aqsmtemp"),
+                            in.array<double>("aqsntemp"), in.array<double>("I_eps_array"), in.array<double>("vcoul"),
+                            occ, reinterpret_cast<double*>(achtemp), nullptr, nullptr, nullptr));
+  const double kernel_s = std::chrono::duration<double>(clk::now() - k_start).count();
+  std::printf("\n Final achtemp\n(%f,%f) \n", achtemp[0].real(), achtemp[0].imag());
+  if (all)
+    print_extras(number_bands, nvband, ngpown, ncouls, in.array<int>("inv_igp_index"), in.array<int>("indinv"),
+                 wx_all.data(), in.array<double>("wtilde_array"), in.array<double>("aqsmtemp"),
+                 in.array<double>("aqsntemp"), in.array<double>("I_eps_array"), in.array<double>("vcoul"), occ);
+  std::printf("********** Kernel Time Taken **********= %g secs\n", kernel_s);
+  std::printf("********** Total Time Taken **********= %g secs\n",
+              std::chrono::duration<double>(clk::now() - t_start).count());
+  return 0;
 }
 
 int main(int argc, char** argv) {
@@ -33,9 +87,14 @@ int main(int argc, char** argv) {
   long long seed = -1;
   int precision = BGW_FP64;
   bool all = false;
+  std::string input_path, dump_path;
   for (int i = 1; i < argc; ++i) {
     std::string a = argv[i];
-    if (a == "--random" && i + 1 < argc) {
+    if (a == "--input" && i + 1 < argc) {
+      input_path = argv[++i];
+    } else if (a == "--dump" && i + 1 < argc) {
+      dump_path = argv[++i];
+    } else if (a == "--random" && i + 1 < argc) {
       seed = std::atoll(argv[++i]);
     } else if (a == "--precision" && i + 1 < argc) {
       std::string p = argv[++i];
@@ -48,6 +107,13 @@ int main(int argc, char** argv) {
     } else {
       pos.push_back(a);
     }
+  }
+  if (!input_path.empty()) {
+    if (!pos.empty() || seed >= 0 || !dump_path.empty()) {
+      usage();
+      return 1;
+    }
+    return run_input_set(input_path.c_str(), precision, all);
   }
   int number_bands = 512, nvband = 2, ncouls = 32768, nodes_per_group = 20;
   const int npes = 1;
@@ -117,6 +183,24 @@ int main(int argc, char** argv) {
     for (int iw = 0; iw < 3; ++iw) wx3(iw) = wx_all(iw);
   }
 
+  if (!dump_path.empty()) {
+    bgw::InputSetWriter w;
+    w.scalar_i64("number_bands", number_bands);
+    w.scalar_i64("nvband", nvband);
+    w.scalar_i64("ncouls", ncouls);
+    w.scalar_i64("ngpown", ngpown);
+    w.scalar_f64("occ", 1.0);
+    w.array("inv_igp_index", BGW_DT_I32, inv_igp_index.dptr, {(size_t)ngpown});
+    w.array("indinv", BGW_DT_I32, indinv.dptr, {(size_t)ncouls + 1});
+    w.array("wx_array", BGW_DT_F64, wx_all.dptr, {(size_t)number_bands, 3});
+    w.array("wtilde_array", BGW_DT_C128, wtilde_array.dptr, {(size_t)ngpown, (size_t)ncouls});
+    w.array("aqsmtemp", BGW_DT_C128, aqsmtemp.dptr, {(size_t)number_bands, (size_t)ncouls});
+    w.array("aqsntemp", BGW_DT_C128, aqsntemp.dptr, {(size_t)number_bands, (size_t)ncouls});
+    w.array("I_eps_array", BGW_DT_C128, I_eps_array.dptr, {(size_t)ngpown, (size_t)ncouls});
+    w.array("vcoul", BGW_DT_F64, vcoul.dptr, {(size_t)ncouls});
+    w.write(dump_path.c_str(), BGW_INPUTS_GPP);
+    std::printf("inputs written to %s\n", dump_path.c_str());
+  }
   bgw::check(bgw_set_precision(bgw::default_context(), precision));
   auto k_start = clk::now();
   if (seed < 0) {
@@ -142,19 +226,10 @@ int main(int argc, char** argv) {
       std::printf("\n!!!! FAILURE - Correctness test failed :-( :-(  (rel err %.3e vs exact)\n", rel);
   }
   std::printf("\n Final achtemp\n(%f,%f) \n", achtemp(0).real(), achtemp(0).imag());
-  if (all) {
-    std::vector<cplx> loc(number_bands * 3);
-    cplx achs, asx[3];
-    bgw::check(bgw_gpp_solver(bgw::default_context(), number_bands, nvband, ngpown, ncouls, inv_igp_index.dptr,
-                              indinv.dptr, wx_all.dptr, bgw::cptr(wtilde_array), bgw::cptr(aqsmtemp),
-                              bgw::cptr(aqsntemp), bgw::cptr(I_eps_array), vcoul.dptr, 1.0, bgw::mptr(achtemp),
-                              reinterpret_cast<double*>(loc.data()), reinterpret_cast<double*>(&achs),
-                              reinterpret_cast<double*>(asx)));
-    cplx sum = 0;
-    for (int n1 = 0; n1 < number_bands; ++n1) sum += loc[n1 * 3];
-    std::printf(" sum_n1 acht_n1_loc[n1][0] = (%f,%f)\n achstemp = (%f,%f)\n asxtemp[0] = (%f,%f)\n", sum.real(),
-                sum.imag(), achs.real(), achs.imag(), asx[0].real(), asx[0].imag());
-  }
+  if (all)
+    print_extras(number_bands, nvband, ngpown, ncouls, inv_igp_index.dptr, indinv.dptr, wx_all.dptr,
+                 bgw::cptr(wtilde_array), bgw::cptr(aqsmtemp), bgw::cptr(aqsntemp), bgw::cptr(I_eps_array),
+                 vcoul.dptr, 1.0);
   const double total_s = std::chrono::duration<double>(clk::now() - t_start).count();
   std::printf("********** Kernel Time Taken **********= %g secs\n", kernel_s);
   std::printf("********** Total Time Taken **********= %g secs\n", total_s);

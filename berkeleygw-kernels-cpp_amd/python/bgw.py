@@ -17,10 +17,13 @@ PKG = Path(__file__).resolve().parents[1]
 LIB_PATH = PKG / "build" / "libbgw.so"
 HEADER = PKG.parent / "include" / "bgw.h"
 
-BGW_OK, BGW_EINVAL, BGW_ENOMEM, BGW_EHIP, BGW_ENODEV = 0, -22, -12, -5, -19
+BGW_OK, BGW_EINVAL, BGW_ENOMEM, BGW_EHIP, BGW_ENODEV, BGW_EIO = 0, -22, -12, -5, -19, -74
 BGW_FP64, BGW_FP32_BULK = 0, 1
+BGW_INPUTS_GPP, BGW_INPUTS_FF, BGW_INPUTS_OTHER = 1, 2, 3
+BGW_DT_F64, BGW_DT_C128, BGW_DT_I32, BGW_DT_I64 = 1, 2, 3, 4
 
-_ERRNAMES = {BGW_EINVAL: "BGW_EINVAL", BGW_ENOMEM: "BGW_ENOMEM", BGW_EHIP: "BGW_EHIP", BGW_ENODEV: "BGW_ENODEV"}
+_ERRNAMES = {BGW_EINVAL: "BGW_EINVAL", BGW_ENOMEM: "BGW_ENOMEM", BGW_EHIP: "BGW_EHIP", BGW_ENODEV: "BGW_ENODEV",
+             BGW_EIO: "BGW_EIO"}
 
 _lib = None
 
@@ -58,6 +61,11 @@ _P = C.c_void_p
 _SZ = C.c_size_t
 _D = C.c_double
 _I = C.c_int
+
+
+class _ArrayDesc(C.Structure):
+    _fields_ = [("name", C.c_char_p), ("dtype", C.c_int), ("ndim", C.c_int), ("dims", C.c_size_t * 4),
+                ("data", C.c_void_p)]
 
 
 def _declare(L: C.CDLL) -> None:
@@ -100,6 +108,16 @@ def _declare(L: C.CDLL) -> None:
     L.bgw_synth_index.restype = None
     L.bgw_synth_permutation.argtypes = [C.c_ulonglong, _P, _SZ]
     L.bgw_synth_permutation.restype = None
+    L.bgw_inputs_write.argtypes = [C.c_char_p, _I, C.POINTER(_ArrayDesc), _I]
+    L.bgw_inputs_open.argtypes = [C.c_char_p, _I, C.POINTER(_P)]
+    L.bgw_inputs_kind.argtypes = [_P]
+    L.bgw_inputs_count.argtypes = [_P]
+    L.bgw_inputs_entry.argtypes = [_P, _I, C.POINTER(_ArrayDesc)]
+    L.bgw_inputs_get.argtypes = [_P, C.c_char_p, C.POINTER(_ArrayDesc)]
+    L.bgw_inputs_close.argtypes = [_P]
+    L.bgw_inputs_close.restype = None
+    L.bgw_io_last_error.argtypes = []
+    L.bgw_io_last_error.restype = C.c_char_p
 
 
 def header_symbols() -> list[str]:
@@ -240,6 +258,128 @@ def ff_config4_inputs(nb: int | None = None, ngpown: int | None = None, ncouls: 
     nc = ncouls or c["ncouls"]
     return ff_inputs(nb or c["nb"], c["nvband"], nc, ngpown or c["ncouls"] // c["nodes_per_group"], c["nFreq"],
                      c["nfreqeval"], seed_base(4))
+
+
+# ----------------------------------------------------------------------- input sets on disk
+_DT = {BGW_DT_F64: np.float64, BGW_DT_C128: np.complex128, BGW_DT_I32: np.int32, BGW_DT_I64: np.int64}
+_DT_OF = {np.dtype(v): k for k, v in _DT.items()}
+
+# file entry name -> key of the dicts made by gpp_inputs / ff_inputs
+_GPP_FILE = {"aqsmtemp": "aqsm", "aqsntemp": "aqsn", "I_eps_array": "I_eps", "wtilde_array": "wtilde",
+             "vcoul": "vcoul", "inv_igp_index": "inv", "indinv": "indinv", "wx_array": "wx"}
+_FF_FILE = {"aqsmtemp": "aqsm", "aqsntemp": "aqsn", "I_epsR_array": "I_epsR", "I_epsA_array": "I_epsA",
+            "vcoul": "vcoul", "inv_igp_index": "inv", "indinv": "indinv", "ekq": "ekq", "dFreqGrid": "dFreqGrid"}
+
+
+def save_inputs(path, d: dict, kind: str) -> None:
+    """Writes a GPP or FF input dict (the form gpp_inputs / ff_inputs return) as an
+    input set (bgw_inputs_write).  Complex arrays get their 2-D / 3-D shapes."""
+    nb, nc, ng = int(d["nb"]), int(d["ncouls"]), int(d["ngpown"])
+    arrays = {"number_bands": np.int64(nb), "nvband": np.int64(d["nvband"]), "ncouls": np.int64(nc),
+              "ngpown": np.int64(ng), "occ": np.float64(d.get("occ", 1.0))}
+    if kind == "gpp":
+        code, names = BGW_INPUTS_GPP, _GPP_FILE
+        shapes = {"aqsm": (nb, nc), "aqsn": (nb, nc), "I_eps": (ng, nc), "wtilde": (ng, nc)}
+        wx = np.asarray(d["wx"], np.float64)
+        shapes["wx"] = (3,) if wx.size == 3 else (nb, 3)
+    elif kind == "ff":
+        code, names = BGW_INPUTS_FF, _FF_FILE
+        nf = int(d["nFreq"])
+        arrays.update(nFreq=np.int64(nf), nfreqeval=np.int64(d["nfreqeval"]),
+                      freqevalmin=np.float64(d["freqevalmin"]), freqevalstep=np.float64(d["freqevalstep"]))
+        shapes = {"aqsm": (nb, nc), "aqsn": (nb, nc), "I_epsR": (nf, ng, nc), "I_epsA": (nf, ng, nc)}
+    else:
+        raise ValueError("kind must be 'gpp' or 'ff'")
+    for fname, key in names.items():
+        a = np.asarray(d[key])
+        arrays[fname] = a.reshape(shapes[key]) if key in shapes else a
+    write_input_set(path, code, arrays)
+
+
+def write_input_set(path, kind: int, arrays: dict) -> None:
+    """Low-level writer: `arrays` maps entry names to numpy arrays or numpy scalars
+    (float64, complex128, int32 or int64; a numpy scalar becomes a rank-0 entry)."""
+    keep, descs = [], []
+    for name, a in arrays.items():
+        rank0 = np.ndim(a) == 0
+        a = np.ascontiguousarray(a)
+        if a.dtype not in _DT_OF:
+            raise TypeError(f"{name}: unsupported dtype {a.dtype}")
+        if a.ndim > 4:
+            raise ValueError(f"{name}: rank above 4")
+        keep.append(a)
+        dims = (C.c_size_t * 4)(*(list(a.shape) + [1] * (4 - a.ndim)))
+        descs.append(_ArrayDesc(name.encode(), _DT_OF[a.dtype], 0 if rank0 else a.ndim, dims, a.ctypes.data))
+    arr = (_ArrayDesc * len(descs))(*descs)
+    rc = lib().bgw_inputs_write(str(path).encode(), kind, arr, len(descs))
+    if rc != BGW_OK:
+        raise BgwError(rc, lib().bgw_io_last_error().decode())
+
+
+class InputSet:
+    """A memory-mapped input set (bgw_inputs_open).  `arrays` maps entry names to
+    read-only numpy views of the mapping, valid until close()."""
+
+    def __init__(self, path, verify: bool = True):
+        self._h = _P()
+        rc = lib().bgw_inputs_open(str(path).encode(), int(verify), C.byref(self._h))
+        if rc != BGW_OK:
+            raise BgwError(rc, lib().bgw_io_last_error().decode())
+        self.kind = lib().bgw_inputs_kind(self._h)
+        self.arrays = {}
+        for i in range(lib().bgw_inputs_count(self._h)):
+            e = _ArrayDesc()
+            lib().bgw_inputs_entry(self._h, i, C.byref(e))
+            shape = tuple(e.dims[:e.ndim])
+            dt = np.dtype(_DT[e.dtype])
+            n = int(np.prod(shape)) if shape else 1
+            buf = (C.c_char * (n * dt.itemsize)).from_address(e.data)
+            v = np.frombuffer(buf, dtype=dt, count=n)
+            v = v.reshape(shape) if shape else v[0]
+            if isinstance(v, np.ndarray):
+                v.flags.writeable = False
+            self.arrays[e.name.decode()] = v
+
+    def close(self) -> None:
+        if self._h:
+            self.arrays = {}
+            lib().bgw_inputs_close(self._h)
+            self._h = _P()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def to_dict(self) -> dict:
+        """Copies the set into the dict form gpp_inputs / ff_inputs return."""
+        a = self.arrays
+        d = dict(nb=int(a["number_bands"]), nvband=int(a["nvband"]), ncouls=int(a["ncouls"]),
+                 ngpown=int(a["ngpown"]), occ=float(a["occ"]))
+        if self.kind == BGW_INPUTS_GPP:
+            names = _GPP_FILE
+        elif self.kind == BGW_INPUTS_FF:
+            names = _FF_FILE
+            d.update(nFreq=int(a["nFreq"]), nfreqeval=int(a["nfreqeval"]), freqevalmin=float(a["freqevalmin"]),
+                     freqevalstep=float(a["freqevalstep"]))
+        else:
+            raise ValueError("to_dict needs a GPP or FF input set")
+        for fname, key in names.items():
+            d[key] = np.array(a[fname]).reshape(-1)
+        return d
+
+
+def load_inputs(path, verify: bool = True) -> dict:
+    """Reads a GPP or FF input set into the dict form gpp_inputs / ff_inputs return."""
+    with InputSet(path, verify) as s:
+        return s.to_dict()
 
 
 # ----------------------------------------------------------------------- context

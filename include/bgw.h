@@ -41,6 +41,8 @@ extern "C" {
 #define BGW_ENOMEM  (-12) /* device allocation failed                                   */
 #define BGW_EHIP    (-5)  /* any other HIP runtime error (text in bgw_last_error)       */
 #define BGW_ENODEV  (-19) /* no GPU / HIP runtime not usable                            */
+#define BGW_EIO     (-74) /* input-set file missing, unreadable or malformed
+                             (text in bgw_io_last_error)                              */
 
 /* ---- arithmetic forms (bgw_set_precision) ------------------------------- */
 #define BGW_FP64       0  /* plain FP64 throughout (default)                           */
@@ -197,6 +199,64 @@ void bgw_synth_complex(unsigned long long seed, double re_lo, double re_hi,
                        double im_lo, double im_hi, double* out, size_t n);
 void bgw_synth_index(unsigned long long seed, int n_max, int* out, size_t n);
 void bgw_synth_permutation(unsigned long long seed, int* out, size_t n);
+
+/* ---- input sets on disk (host only) --------------------------------------
+ * The reference drivers build their inputs in memory (GPP/gpp.cpp:220-276,
+ * FF/ff.cpp:552-633) and read no files; these calls store and load the same
+ * arrays as a self-describing container, memory-mapped on read so the pointers
+ * of a loaded set go to the solver entry points above without a copy.
+ * Layout (little-endian):
+ *   bytes [0,32)  "BGWINPUT", u32 version = 1, u32 kind, u32 count, u32 0,
+ *                 u64 directory offset = 32
+ *   directory     count entries of 96 bytes: char name[32] (NUL-padded),
+ *                 u32 dtype, u32 ndim (0..4, 0 = one-element scalar),
+ *                 u64 dims[4] (unused = 1), u64 offset (multiple of 4096),
+ *                 u64 nbytes, u64 checksum
+ *   payload       each array at its offset, row-major, complex as (re,im) pairs
+ *   checksum      sum_i mix64(w_i ^ (i * 0x9E3779B97F4A7C15)) mod 2^64 over the
+ *                 payload's 8-byte little-endian words (zero-padded), mix64 the
+ *                 SplitMix64 finalizer of bgw_synth_*.
+ * Kinds and the entries bgw_inputs_open checks for them (names = the entry
+ * points' argument names; int64 scalars number_bands, nvband, ncouls, ngpown and
+ * f64 scalar occ in both):
+ *   GPP: inv_igp_index i32[ngpown], indinv i32[ncouls+1], wx_array f64[3] or
+ *        [number_bands,3], wtilde_array, I_eps_array c128[ngpown,ncouls],
+ *        aqsmtemp, aqsntemp c128[number_bands,ncouls], vcoul f64[ncouls]
+ *   FF:  int64 nFreq, nfreqeval; f64 freqevalmin, freqevalstep; inv_igp_index
+ *        i32[ngpown], indinv i32[ncouls], ekq f64[number_bands], dFreqGrid
+ *        f64[nFreq], aqsmtemp, aqsntemp c128[number_bands,ncouls], vcoul
+ *        f64[ncouls], I_epsR_array, I_epsA_array c128[nFreq,ngpown,ncouls]
+ *   OTHER: any entries, no check.
+ * Errors: BGW_EINVAL for bad arguments, BGW_EIO for files (bgw_io_last_error,
+ * per thread, has the text).  No reference counterpart. */
+#define BGW_INPUTS_GPP    1
+#define BGW_INPUTS_FF     2
+#define BGW_INPUTS_OTHER  3
+#define BGW_DT_F64   1
+#define BGW_DT_C128  2
+#define BGW_DT_I32   3
+#define BGW_DT_I64   4
+
+typedef struct bgw_array_desc {
+  const char* name;
+  int         dtype;
+  int         ndim;
+  size_t      dims[4];
+  const void* data;
+} bgw_array_desc;
+typedef struct bgw_inputs bgw_inputs;
+
+int         bgw_inputs_write(const char* path, int kind, const bgw_array_desc* arrays, int count);
+/* Maps `path` read-only and checks its structure and, for GPP/FF, its entries;
+ * verify != 0 also checks every checksum.  Descriptors returned by the getters
+ * point into the mapping and stay valid until bgw_inputs_close. */
+int         bgw_inputs_open(const char* path, int verify, bgw_inputs** out);
+int         bgw_inputs_kind(const bgw_inputs* in);
+int         bgw_inputs_count(const bgw_inputs* in);
+int         bgw_inputs_entry(const bgw_inputs* in, int i, bgw_array_desc* desc);
+int         bgw_inputs_get(const bgw_inputs* in, const char* name, bgw_array_desc* desc);
+void        bgw_inputs_close(bgw_inputs* in);
+const char* bgw_io_last_error(void);
 
 #ifdef __cplusplus
 }

@@ -4,7 +4,8 @@ gpp.ex calls noflagOCC_solver through the reference-signature C++ shim
 (host/bgw_reference_api.hpp) and ff.ex calls the three FF kernels the same way, so
 these tests also cover that shim.  The drivers check the reference constants against
 exact closed forms; with --random SEED they generate the same inputs as bgw.gpp_inputs /
-bgw.ff_inputs, which the tests compare with the oracle."""
+bgw.ff_inputs, which the tests compare with the oracle.  --dump FILE and --input FILE
+write and read input sets (bgw_inputs_write / bgw_inputs_open)."""
 import re
 import subprocess
 from pathlib import Path
@@ -81,3 +82,49 @@ def test_ff_driver_random_inputs_vs_oracle():
 def test_ff_driver_rejects_ngpown_above_ncouls():
     rc, out = _run("ff.ex", 10, 2, 20, 30, 5, 2)
     assert rc == 1 and "ngpown cannot be greater than ncouls" in out
+
+
+def test_gpp_driver_dump_then_input_reproduces_random_run(tmp_path):
+    f = tmp_path / "gpp.bgwin"
+    rc, out1 = _run("gpp.ex", 24, 2, 800, 100, "--random", 99, "--all", "--dump", f)
+    assert rc == 0 and "inputs written" in out1
+    rc, out2 = _run("gpp.ex", "--input", f, "--all")
+    assert rc == 0, out2
+    for label in ("Final achtemp\n", "sum_n1 acht_n1_loc[n1][0]", "achstemp", "asxtemp[0]"):
+        assert _pair(out1, label) == _pair(out2, label), label
+    d = bgw.load_inputs(f)
+    o = O.gpp_forward(d)
+    assert abs(_pair(out2, "Final achtemp\n") - o["achtemp"][0]) <= 1e-6 + 1e-9 * abs(o["achtemp"][0])
+
+
+def test_gpp_driver_reads_the_constant_inputs_back(tmp_path):
+    f = tmp_path / "c.bgwin"
+    rc, out1 = _run("gpp.ex", 16, 2, 512, 8, "--dump", f)
+    assert rc == 0 and "SUCCESS" in out1
+    rc, out2 = _run("gpp.ex", "--input", f)
+    assert rc == 0 and _pair(out1, "Final achtemp\n") == _pair(out2, "Final achtemp\n")
+
+
+def test_ff_driver_dump_then_input_reproduces_random_run(tmp_path):
+    f = tmp_path / "ff.bgwin"
+    shape = (40, 3, 300, 20, 15, 2)
+    rc, out1 = _run("ff.ex", *shape, "--random", 555, "--dump", f)
+    assert rc == 0 and "inputs written" in out1
+    rc, out2 = _run("ff.ex", "--input", f)
+    assert rc == 0, out2
+    d = bgw.load_inputs(f)
+    for label, ref in (("achsDtemp", O.achsDtemp(d)), ("asxDtemp", O.asxDtemp(d)[0]),
+                       ("achDtemp_cor", O.achDtemp_cor(d)[0])):
+        assert _pair(out1, label + " ") == _pair(out2, label + " "), label
+        assert abs(_pair(out2, label + " ") - ref) <= 1e-6 + 1e-9 * abs(ref), label
+
+
+def test_drivers_refuse_bad_input_sets(tmp_path):
+    rc, out = _run("gpp.ex", "--input", tmp_path / "missing.bgwin")
+    assert rc == 1 and "cannot read input set" in out
+    f = tmp_path / "ff.bgwin"
+    bgw.save_inputs(f, bgw.ff_inputs(6, 2, 30, 4, 5, 3, 77), "ff")
+    rc, out = _run("gpp.ex", "--input", f)
+    assert rc == 1 and "not a GPP input set" in out
+    rc, out = _run("gpp.ex", 8, 2, 64, 4, "--input", f)
+    assert rc == 1
