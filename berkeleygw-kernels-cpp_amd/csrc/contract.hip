@@ -69,7 +69,8 @@ __global__ __launch_bounds__(NT, 2) void k_contract(ContractArgs a) {
   // row tiles of all jobs are numbered densely by tile_start (prefix of
   // ceil(cnt/BM)), so empty jobs and empty row-tile tails take no workgroup.
   // XCD-aware order: the hardware deals workgroup L to XCD L % 8; chunks of
-  // a.chunk = 2*ntn consecutive items are dealt round-robin over the XCDs.  The
+  // a.chunk consecutive items (the G' tiles of one row tile for dense row lists,
+  // of two row tiles for sparse plans) are dealt round-robin over the XCDs.  The
   // G' tiles of a row tile (sharing its A rows) and the row tiles of a slice
   // (sharing its B tile) then run side by side on one XCD and meet in its L2,
   // while every XCD gets an even share of the work.
@@ -407,7 +408,9 @@ int contract_launch(bgw_ctx* ctx, ContractArgs a, int njobs, int max_tiles, bool
   a.tiles = max_tiles;
   a.njobs = njobs;
   const size_t W = (size_t)ntn * ksplit * max_tiles;
-  a.chunk = 2 * ntn;
+  // dense lists: one row tile's G' tiles per chunk (measured 6-8% faster for
+  // achsDtemp than two); sparse plans: two row tiles (one or two frequency slices)
+  a.chunk = (exact_tiles ? 1 : 2) * ntn;
   const size_t per_round = (size_t)kXcds * a.chunk;
   const size_t G = (W + per_round - 1) / per_round * per_round;
   if (G > (size_t)INT32_MAX) return fail(ctx, BGW_EINVAL, "contraction grid too large");
