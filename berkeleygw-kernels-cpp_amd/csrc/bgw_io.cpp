@@ -279,6 +279,7 @@ int bgw_inputs_open(const char* path, int verify, bgw_inputs** out) {
   if (in->kind != BGW_INPUTS_GPP && in->kind != BGW_INPUTS_FF && in->kind != BGW_INPUTS_OTHER)
     return bad("unknown kind " + std::to_string(in->kind));
   if (count == 0 || count > kMaxEntries) return bad("bad entry count");
+  if (get<uint32_t>(b, 20) != 0) return bad("reserved header word is not zero");
   if (dir != kHeader || dir + (uint64_t)count * kEntry > in->size) return bad("directory outside the file");
   for (uint32_t i = 0; i < count; ++i) {
     const size_t e = dir + (size_t)i * kEntry;
@@ -291,6 +292,8 @@ int bgw_inputs_open(const char* path, int verify, bgw_inputs** out) {
     d.dtype = (int)get<uint32_t>(b, e + 32);
     d.ndim = (int)get<uint32_t>(b, e + 36);
     for (int k = 0; k < 4; ++k) d.dims[k] = (size_t)get<uint64_t>(b, e + 40 + 8 * k);
+    for (int k = d.ndim; k >= 0 && k < 4; ++k)
+      if (d.dims[k] != 1) return bad(std::string("entry '") + name + "': unused dims must be 1");
     const uint64_t off = get<uint64_t>(b, e + 72), nbytes = get<uint64_t>(b, e + 80);
     size_t n = 0;
     const size_t es = elem_size(d.dtype);

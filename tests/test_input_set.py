@@ -146,6 +146,8 @@ def test_malformed_files_are_refused(tmp_path):
         "magic": b"BGWINPUX" + good[8:],
         "version": good[:8] + struct.pack("<I", 2) + good[12:],
         "kind": good[:12] + struct.pack("<I", 9) + good[16:],
+        "reserved": good[:20] + struct.pack("<I", 1) + good[24:],
+        "unused_dim": good[:32 + 40 + 24] + struct.pack("<Q", 2) + good[32 + 40 + 32:],
         "truncated_directory": good[:40],
         "truncated_payload": good[:len(good) - 100],
         "short": good[:16],
