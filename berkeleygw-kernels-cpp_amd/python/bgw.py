@@ -316,24 +316,40 @@ def write_input_set(path, kind: int, arrays: dict) -> None:
         raise BgwError(rc, lib().bgw_io_last_error().decode())
 
 
+class _Mapping:
+    """Owns a bgw_inputs handle; unmapped when the last view of it is gone."""
+
+    def __init__(self, h):
+        self.h = h
+
+    def __del__(self):
+        if self.h and _lib is not None:
+            _lib.bgw_inputs_close(self.h)
+            self.h = None
+
+
 class InputSet:
     """A memory-mapped input set (bgw_inputs_open).  `arrays` maps entry names to
-    read-only numpy views of the mapping, valid until close()."""
+    read-only numpy views of the mapping.  Each view keeps the mapping alive, so
+    views stay valid after close(); the file is unmapped once the set and every
+    view of it are gone."""
 
     def __init__(self, path, verify: bool = True):
-        self._h = _P()
-        rc = lib().bgw_inputs_open(str(path).encode(), int(verify), C.byref(self._h))
+        h = _P()
+        rc = lib().bgw_inputs_open(str(path).encode(), int(verify), C.byref(h))
         if rc != BGW_OK:
             raise BgwError(rc, lib().bgw_io_last_error().decode())
-        self.kind = lib().bgw_inputs_kind(self._h)
+        self._m = _Mapping(h)
+        self.kind = lib().bgw_inputs_kind(h)
         self.arrays = {}
-        for i in range(lib().bgw_inputs_count(self._h)):
+        for i in range(lib().bgw_inputs_count(h)):
             e = _ArrayDesc()
-            lib().bgw_inputs_entry(self._h, i, C.byref(e))
+            lib().bgw_inputs_entry(h, i, C.byref(e))
             shape = tuple(e.dims[:e.ndim])
             dt = np.dtype(_DT[e.dtype])
             n = int(np.prod(shape)) if shape else 1
             buf = (C.c_char * (n * dt.itemsize)).from_address(e.data)
+            buf._mapping = self._m  # the numpy view's base chain holds the mapping
             v = np.frombuffer(buf, dtype=dt, count=n)
             v = v.reshape(shape) if shape else v[0]
             if isinstance(v, np.ndarray):
@@ -341,22 +357,15 @@ class InputSet:
             self.arrays[e.name.decode()] = v
 
     def close(self) -> None:
-        if self._h:
-            self.arrays = {}
-            lib().bgw_inputs_close(self._h)
-            self._h = _P()
+        """Drops this set's references; the mapping goes with the last view."""
+        self.arrays = {}
+        self._m = None
 
     def __enter__(self):
         return self
 
     def __exit__(self, *exc):
         self.close()
-
-    def __del__(self):
-        try:
-            self.close()
-        except Exception:
-            pass
 
     def to_dict(self) -> dict:
         """Copies the set into the dict form gpp_inputs / ff_inputs return."""

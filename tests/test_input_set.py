@@ -110,6 +110,25 @@ def test_mapped_views_are_read_only_and_zero_copy(tmp_path):
             a[0, 0] = 0
 
 
+def test_views_outlive_close(tmp_path):
+    import gc
+
+    d = bgw.gpp_inputs(3, 1, 16, 2, 9)
+    p = tmp_path / "g.bgwin"
+    bgw.save_inputs(p, d, "gpp")
+    s = bgw.InputSet(p)
+    v = s.arrays["wtilde_array"]
+    s.close()
+    del s
+    gc.collect()
+    assert np.array_equal(v.reshape(-1), d["wtilde"])  # still mapped through the view
+    maps = lambda: open("/proc/self/maps").read().count(str(p))  # noqa: E731
+    assert maps() == 1
+    del v
+    gc.collect()
+    assert maps() == 0  # unmapped with the last view
+
+
 def _gpp_file(tmp_path, name="g.bgwin"):
     p = tmp_path / name
     bgw.save_inputs(p, bgw.gpp_inputs(4, 1, 24, 3, 5), "gpp")
