@@ -3,6 +3,9 @@
 // their arguments on the host, stage arrays into context-owned device buffers,
 // run the `_device` path and copy results back, so a caller sees the
 // reference's blocking semantics (GPP/gpp.cpp:283-285, FF/ff.cpp:700-728).
+#include <omp.h>
+
+#include <algorithm>
 #include <cstdio>
 #include <cstring>
 
@@ -76,6 +79,124 @@ bool mul_ok(size_t a, size_t b, size_t* out) {
   return true;
 }
 
+// ------------------------------------------------------------ pinned staging ring
+namespace {
+constexpr size_t kSlotBytes = size_t(32) << 20;  // 32 MiB per pinned buffer
+constexpr int kSlots = 4;
+constexpr size_t kDirectBytes = size_t(4) << 20;  // smaller copies go straight from pageable memory
+
+// memcpy split over the host threads (pageable <-> pinned runs at host memory bandwidth)
+void parallel_copy(void* dst, const void* src, size_t n) {
+  const int nt = std::max(1, std::min(16, omp_get_max_threads()));
+  const size_t piece = (n + nt - 1) / nt;
+#pragma omp parallel for num_threads(nt) schedule(static)
+  for (int t = 0; t < nt; ++t) {
+    const size_t b = std::min(n, (size_t)t * piece), e = std::min(n, b + piece);
+    if (e > b) std::memcpy(static_cast<char*>(dst) + b, static_cast<const char*>(src) + b, e - b);
+  }
+}
+
+// Makes the ring usable; false if pinned memory is unavailable (copies then go direct).
+bool ring_ready(bgw_ctx* ctx) {
+  if (!ctx->ring.empty()) return true;
+  std::vector<PinnedSlot> r(kSlots);
+  for (auto& s : r) {
+    if (hipHostMalloc(&s.host, kSlotBytes, hipHostMallocDefault) != hipSuccess ||
+        hipEventCreateWithFlags(&s.done, hipEventDisableTiming) != hipSuccess) {
+      for (auto& t : r) {
+        if (t.host) (void)hipHostFree(t.host);
+        if (t.done) (void)hipEventDestroy(t.done);
+      }
+      return false;
+    }
+  }
+  ctx->ring = std::move(r);
+  return true;
+}
+
+// Next ring slot, after the DMA that last used it has finished.
+int take_slot(bgw_ctx* ctx, PinnedSlot** out) {
+  PinnedSlot& s = ctx->ring[ctx->ring_next++ % ctx->ring.size()];
+  if (s.busy) {
+    hipError_t e = hipEventSynchronize(s.done);
+    if (e != hipSuccess) return hip_fail(ctx, e, "staging event");
+    s.busy = false;
+  }
+  *out = &s;
+  return BGW_OK;
+}
+}  // namespace
+
+int upload_host(bgw_ctx* ctx, void* dst, const void* src, size_t bytes) {
+  if (bytes < kDirectBytes || !ring_ready(ctx)) {
+    hipError_t e = hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream);
+    return e == hipSuccess ? BGW_OK : hip_fail(ctx, e, "upload");
+  }
+  for (size_t off = 0; off < bytes; off += kSlotBytes) {
+    const size_t n = std::min(kSlotBytes, bytes - off);
+    PinnedSlot* s;
+    int rc = take_slot(ctx, &s);
+    if (rc) return rc;
+    parallel_copy(s->host, static_cast<const char*>(src) + off, n);
+    hipError_t e = hipMemcpyAsync(static_cast<char*>(dst) + off, s->host, n, hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess) e = hipEventRecord(s->done, ctx->stream);
+    if (e != hipSuccess) return hip_fail(ctx, e, "staged upload");
+    s->busy = true;
+  }
+  return BGW_OK;
+}
+
+int download_host(bgw_ctx* ctx, void* dst, const void* src, size_t bytes) {
+  if (bytes < kDirectBytes || !ring_ready(ctx)) {
+    hipError_t e = hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    return e == hipSuccess ? BGW_OK : hip_fail(ctx, e, "download");
+  }
+  // DMA of chunk c runs while the host copies chunk c - (slots - 1) out of its buffer
+  struct Pending {
+    PinnedSlot* s;
+    size_t off, n;
+  };
+  std::vector<Pending> q;
+  auto drain_one = [&]() -> int {
+    Pending p = q.front();
+    q.erase(q.begin());
+    hipError_t e = hipEventSynchronize(p.s->done);
+    if (e != hipSuccess) return hip_fail(ctx, e, "staging event");
+    p.s->busy = false;
+    parallel_copy(static_cast<char*>(dst) + p.off, p.s->host, p.n);
+    return BGW_OK;
+  };
+  for (size_t off = 0; off < bytes; off += kSlotBytes) {
+    if (q.size() + 1 >= ctx->ring.size()) {
+      int rc = drain_one();
+      if (rc) return rc;
+    }
+    const size_t n = std::min(kSlotBytes, bytes - off);
+    PinnedSlot* s;
+    int rc = take_slot(ctx, &s);
+    if (rc) return rc;
+    hipError_t e = hipMemcpyAsync(s->host, static_cast<const char*>(src) + off, n, hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipEventRecord(s->done, ctx->stream);
+    if (e != hipSuccess) return hip_fail(ctx, e, "staged download");
+    s->busy = true;
+    q.push_back({s, off, n});
+  }
+  while (!q.empty()) {
+    int rc = drain_one();
+    if (rc) return rc;
+  }
+  return BGW_OK;
+}
+
+void ring_release(bgw_ctx* ctx) {
+  for (auto& s : ctx->ring) {
+    if (s.host) (void)hipHostFree(s.host);
+    if (s.done) (void)hipEventDestroy(s.done);
+  }
+  ctx->ring.clear();
+}
+
 }  // namespace bgw
 
 using namespace bgw;
@@ -119,6 +240,7 @@ void bgw_destroy(bgw_ctx* ctx) {
     (void)hipEventDestroy(tl.stop);
   }
   for (hipEvent_t ev : ctx->free_events) (void)hipEventDestroy(ev);
+  ring_release(ctx);
   if (ctx->d_flag) (void)hipFree(ctx->d_flag);
   if (ctx->own_stream) (void)hipStreamDestroy(ctx->own_stream);
   delete ctx;
@@ -222,12 +344,8 @@ struct Stage {
     void* d = nullptr;
     rc = pool_get(ctx, name, bytes, &d);
     if (rc) return nullptr;
-    hipError_t e = hipMemcpyAsync(d, src, bytes, hipMemcpyHostToDevice, ctx->stream);
-    if (e != hipSuccess) {
-      rc = hip_fail(ctx, e, "upload");
-      return nullptr;
-    }
-    return static_cast<T*>(d);
+    rc = upload_host(ctx, d, src, bytes);
+    return rc ? nullptr : static_cast<T*>(d);
   }
   double* out(const char* name, size_t count) {
     if (rc) return nullptr;
@@ -237,8 +355,7 @@ struct Stage {
   }
   void down(double* host, const double* dev, size_t count) {
     if (rc || !host) return;
-    hipError_t e = hipMemcpyAsync(host, dev, count * sizeof(double), hipMemcpyDeviceToHost, ctx->stream);
-    if (e != hipSuccess) rc = hip_fail(ctx, e, "download");
+    rc = download_host(ctx, host, dev, count * sizeof(double));
   }
 };
 

@@ -25,6 +25,13 @@ struct TimedLaunch {
   hipEvent_t start, stop;
 };
 
+// One buffer of the pinned staging ring of the host-pointer entry points.
+struct PinnedSlot {
+  void* host = nullptr;
+  hipEvent_t done = nullptr;  // recorded after the DMA that reads or fills `host`
+  bool busy = false;
+};
+
 }  // namespace bgw
 
 struct bgw_ctx {
@@ -42,6 +49,9 @@ struct bgw_ctx {
   std::vector<bgw::TimedLaunch> pending;
   std::vector<hipEvent_t> free_events;
   std::map<std::string, std::pair<double, long>> ktime;
+  // pinned staging ring for large host<->device copies (allocated on first use)
+  std::vector<bgw::PinnedSlot> ring;
+  size_t ring_next = 0;
 };
 
 namespace bgw {
@@ -65,6 +75,16 @@ int check_launch(bgw_ctx* ctx, const char* what);
 
 // Overflow-checked product of sizes; returns false on overflow.
 bool mul_ok(size_t a, size_t b, size_t* out);
+
+// Copies between caller (pageable) host memory and device memory on ctx->stream.
+// Large copies go through the context's pinned ring: a multi-threaded host copy
+// fills one pinned buffer while the DMA engine drains the previous one.  Uploads
+// return once the source has been consumed (the caller may reuse it); downloads
+// return with the data in `dst`.
+int upload_host(bgw_ctx* ctx, void* dst, const void* src, size_t bytes);
+int download_host(bgw_ctx* ctx, void* dst, const void* src, size_t bytes);
+// Releases the pinned ring (the stream must be idle).
+void ring_release(bgw_ctx* ctx);
 
 // Zero-fill kernels usable inside captured graphs (no memset nodes).
 void fill_zero(void* p, size_t bytes, hipStream_t s);

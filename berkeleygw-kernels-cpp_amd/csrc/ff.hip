@@ -201,8 +201,7 @@ int upload(bgw_ctx* ctx, const char* slot, const T* src, size_t count, const T**
   void* d;
   int rc = pool_get(ctx, slot, count * sizeof(T), &d);
   if (rc) return rc;
-  hipError_t e = hipMemcpyAsync(d, src, count * sizeof(T), hipMemcpyHostToDevice, ctx->stream);
-  if (e != hipSuccess) return hip_fail(ctx, e, "upload");
+  if ((rc = upload_host(ctx, d, src, count * sizeof(T)))) return rc;
   *dst = static_cast<const T*>(d);
   return BGW_OK;
 }
@@ -225,9 +224,7 @@ int ff_host(bgw_ctx* ctx, int op, int nb, int nvband, int nfreqeval, int ncouls,
     // only slice 1 of I_epsR is read (FF/ff.cpp:200): stage it as slice 1 of a 2-slice array
     void* d;
     if ((rc = pool_get(ctx, "h_epsR", 2 * npg * 16, &d))) return rc;
-    hipError_t e = hipMemcpyAsync(static_cast<char*>(d) + npg * 16, R + 2 * npg, npg * 16,
-                                  hipMemcpyHostToDevice, ctx->stream);
-    if (e != hipSuccess) return hip_fail(ctx, e, "upload");
+    if ((rc = upload_host(ctx, static_cast<char*>(d) + npg * 16, R + 2 * npg, npg * 16))) return rc;
     d_R = static_cast<const double*>(d);
   } else {
     if ((rc = upload(ctx, "h_ekq", ekq, (size_t)(op == 1 ? nvband : nb), &d_ekq))) return rc;

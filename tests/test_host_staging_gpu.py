@@ -1,0 +1,73 @@
+"""Host-pointer entry points with arrays large enough to go through the context's
+pinned staging ring (several 32 MiB chunks, the last one partial): their results
+must equal, bit for bit, those of the device-pointer entry points on the same
+data, for uploads (GPP forward, FF achDtemp_cor) and downloads (GPP adjoint
+gradients of ngpown x ncouls entries)."""
+import numpy as np
+import pytest
+
+import bgw
+
+pytestmark = pytest.mark.gpu
+
+
+def _dev(torch, a):
+    return torch.from_numpy(np.ascontiguousarray(a)).to("cuda:0")
+
+
+def test_staged_gpp_forward_matches_device_api(ctx):
+    import torch
+
+    # I_eps and wtilde 1000 x 24000 complex = 384 MB each (12 chunks); aqs 96 x 24000 (2 chunks)
+    nb, nvband, ncouls, ngpown = 96, 2, 24000, 1000
+    d = bgw.gpp_inputs(nb, nvband, ncouls, ngpown, 2024)
+    h = ctx.gpp_solver(d)
+    T = {k: _dev(torch, d[k]) for k in ("inv", "indinv", "wx", "wtilde", "aqsm", "aqsn", "I_eps", "vcoul")}
+    out = {k: torch.zeros(n, dtype=torch.complex128, device="cuda:0") for k, n in
+           (("achtemp", 3), ("acht_n1_loc", nb * 3), ("achstemp", 1), ("asxtemp", 3))}
+    torch.cuda.synchronize()
+    P = {k: v.data_ptr() for k, v in T.items()}
+    ctx.gpp_solver_device(nb, nvband, ngpown, ncouls, P["inv"], P["indinv"], P["wx"], P["wtilde"], P["aqsm"],
+                          P["aqsn"], P["I_eps"], P["vcoul"], 1.0, out["achtemp"].data_ptr(),
+                          out["acht_n1_loc"].data_ptr(), out["achstemp"].data_ptr(), out["asxtemp"].data_ptr())
+    ctx.synchronize()
+    for k in h:
+        assert np.array_equal(out[k].cpu().numpy().reshape(h[k].shape), h[k]), k
+
+
+def test_staged_gpp_backward_downloads_match_device_api(ctx):
+    import torch
+
+    # dL/dwtilde and dL/dI_eps: 400 x 24000 complex = 154 MB each (5 chunks, the last partial)
+    nb, nvband, ncouls, ngpown = 32, 2, 24000, 400
+    d = bgw.gpp_inputs(nb, nvband, ncouls, ngpown, 77)
+    h = ctx.gpp_backward(d)
+    T = {k: _dev(torch, d[k]) for k in ("inv", "indinv", "wx", "wtilde", "aqsm", "aqsn", "I_eps", "vcoul")}
+    npg = ngpown * ncouls
+    ds = torch.zeros(3, dtype=torch.complex128, device="cuda:0")
+    dw = torch.zeros(npg, dtype=torch.complex128, device="cuda:0")
+    dI = torch.zeros(npg, dtype=torch.complex128, device="cuda:0")
+    torch.cuda.synchronize()
+    P = {k: v.data_ptr() for k, v in T.items()}
+    ctx.gpp_backward_device(nb, nvband, ngpown, ncouls, P["inv"], P["indinv"], P["wx"], P["wtilde"], P["aqsm"],
+                            P["aqsn"], P["I_eps"], P["vcoul"], 1.0, ds.data_ptr(), dw.data_ptr(), dI.data_ptr())
+    ctx.synchronize()
+    assert np.array_equal(ds.cpu().numpy(), h["dsigma_domega"])
+    assert np.array_equal(dw.cpu().numpy(), h["dL_dwtilde"])
+    assert np.array_equal(dI.cpu().numpy(), h["dL_dI_eps"])
+    assert np.count_nonzero(h["dL_dI_eps"]) == npg
+
+
+def test_staged_ff_upload_matches_device_api(ctx):
+    import torch
+
+    # I_epsR / I_epsA: 6 x 200 x 20000 complex = 384 MB each
+    f = bgw.ff_inputs(64, 2, 20000, 200, 6, 3, 9)
+    h = ctx.achDtemp_cor_Kernel(f)
+    keys = ("inv", "indinv", "ekq", "aqsm", "aqsn", "vcoul", "I_epsR", "I_epsA", "dFreqGrid")
+    T = {k: _dev(torch, f[k]) for k in keys}
+    out = torch.zeros(f["nfreqeval"], dtype=torch.complex128, device="cuda:0")
+    torch.cuda.synchronize()
+    ctx.ff_device("achDtemp_cor", f, {k: v.data_ptr() for k, v in T.items()}, out.data_ptr())
+    ctx.synchronize()
+    assert np.array_equal(out.cpu().numpy(), h)
