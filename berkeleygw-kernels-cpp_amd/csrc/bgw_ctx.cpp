@@ -128,8 +128,12 @@ int take_slot(bgw_ctx* ctx, PinnedSlot** out) {
 }  // namespace
 
 int upload_host(bgw_ctx* ctx, void* dst, const void* src, size_t bytes) {
+  return upload_host(ctx, dst, src, bytes, ctx->stream);
+}
+
+int upload_host(bgw_ctx* ctx, void* dst, const void* src, size_t bytes, hipStream_t stream) {
   if (bytes < kDirectBytes || !ring_ready(ctx)) {
-    hipError_t e = hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream);
+    hipError_t e = hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, stream);
     return e == hipSuccess ? BGW_OK : hip_fail(ctx, e, "upload");
   }
   for (size_t off = 0; off < bytes; off += kSlotBytes) {
@@ -138,8 +142,8 @@ int upload_host(bgw_ctx* ctx, void* dst, const void* src, size_t bytes) {
     int rc = take_slot(ctx, &s);
     if (rc) return rc;
     parallel_copy(s->host, static_cast<const char*>(src) + off, n);
-    hipError_t e = hipMemcpyAsync(static_cast<char*>(dst) + off, s->host, n, hipMemcpyHostToDevice, ctx->stream);
-    if (e == hipSuccess) e = hipEventRecord(s->done, ctx->stream);
+    hipError_t e = hipMemcpyAsync(static_cast<char*>(dst) + off, s->host, n, hipMemcpyHostToDevice, stream);
+    if (e == hipSuccess) e = hipEventRecord(s->done, stream);
     if (e != hipSuccess) return hip_fail(ctx, e, "staged upload");
     s->busy = true;
   }
@@ -187,6 +191,38 @@ int download_host(bgw_ctx* ctx, void* dst, const void* src, size_t bytes) {
     if (rc) return rc;
   }
   return BGW_OK;
+}
+
+int copy_stream(bgw_ctx* ctx, hipStream_t* out) {
+  if (!ctx->copy_stream) {
+    hipError_t e = hipStreamCreateWithFlags(&ctx->copy_stream, hipStreamNonBlocking);
+    if (e == hipSuccess && !ctx->xfer_event) e = hipEventCreateWithFlags(&ctx->xfer_event, hipEventDisableTiming);
+    if (e != hipSuccess) return hip_fail(ctx, e, "copy stream");
+  }
+  *out = ctx->copy_stream;
+  return BGW_OK;
+}
+
+int cross_stream_wait(bgw_ctx* ctx, hipStream_t from, hipStream_t to) {
+  hipError_t e = hipEventRecord(ctx->xfer_event, from);
+  if (e == hipSuccess) e = hipStreamWaitEvent(to, ctx->xfer_event, 0);
+  return e == hipSuccess ? BGW_OK : hip_fail(ctx, e, "stream hand-over");
+}
+
+void gather_igp_columns(const double2* am, const double2* an, const int* inv, const int* indinv, int nb,
+                        int ngpown, int ncouls, double2* gm, double2* gn) {
+  std::vector<int> igp(ngpown);
+  for (int p = 0; p < ngpown; ++p) igp[p] = indinv[inv[p]];
+  const int nt = std::max(1, std::min(16, omp_get_max_threads()));
+#pragma omp parallel for num_threads(nt) schedule(static)
+  for (int n1 = 0; n1 < nb; ++n1) {
+    const double2 *rm = am + (size_t)n1 * ncouls, *rn = an + (size_t)n1 * ncouls;
+    double2 *om = gm + (size_t)n1 * ngpown, *on = gn + (size_t)n1 * ngpown;
+    for (int p = 0; p < ngpown; ++p) {
+      om[p] = rm[igp[p]];
+      on[p] = rn[igp[p]];
+    }
+  }
 }
 
 void ring_release(bgw_ctx* ctx) {
@@ -240,6 +276,11 @@ void bgw_destroy(bgw_ctx* ctx) {
     (void)hipEventDestroy(tl.stop);
   }
   for (hipEvent_t ev : ctx->free_events) (void)hipEventDestroy(ev);
+  if (ctx->copy_stream) {
+    (void)hipStreamSynchronize(ctx->copy_stream);
+    (void)hipStreamDestroy(ctx->copy_stream);
+  }
+  if (ctx->xfer_event) (void)hipEventDestroy(ctx->xfer_event);
   ring_release(ctx);
   if (ctx->d_flag) (void)hipFree(ctx->d_flag);
   if (ctx->own_stream) (void)hipStreamDestroy(ctx->own_stream);
@@ -347,6 +388,12 @@ struct Stage {
     rc = upload_host(ctx, d, src, bytes);
     return rc ? nullptr : static_cast<T*>(d);
   }
+  void* buf(const char* name, size_t bytes) {
+    if (rc) return nullptr;
+    void* d = nullptr;
+    rc = pool_get(ctx, name, bytes, &d);
+    return rc ? nullptr : d;
+  }
   double* out(const char* name, size_t count) {
     if (rc) return nullptr;
     void* d = nullptr;
@@ -388,6 +435,7 @@ int gpp_sizes(bgw_ctx* ctx, size_t nb, size_t ngpown, size_t ncouls, size_t* n_b
 }
 
 int finish(bgw_ctx* ctx, int rc) {
+  if (ctx->copy_stream) (void)hipStreamSynchronize(ctx->copy_stream);
   if (rc) {
     (void)hipStreamSynchronize(ctx->stream);
     return rc;
@@ -407,22 +455,25 @@ int gpp_host(bgw_ctx* ctx, size_t nb, size_t nvband, size_t ngpown, size_t ncoul
   if (!rc) rc = check_gpp_indices(ctx, ngpown, ncouls, inv, indinv);
   if (rc) return rc;
   if (hipSetDevice(ctx->device) != hipSuccess) return fail(ctx, BGW_ENODEV, "hipSetDevice");
+  // device buffers (grow-only pools), filled by the overlapped staging of gpp_forward_staged
   Stage st(ctx);
-  const int* d_inv = st.up("h_inv", inv, ngpown);
-  const int* d_indinv = st.up("h_indinv", indinv, ncouls + 1);
-  const double* d_wx = st.up("h_wx", wx, wx_per_band ? nb * 3 : 3);
-  const double* d_wt = st.up("h_wtilde", wtilde, 2 * n_pg);
-  const double* d_am = st.up("h_aqsm", aqsm, 2 * n_bg);
-  const double* d_an = st.up("h_aqsn", aqsn, 2 * n_bg);
-  const double* d_I = st.up("h_ieps", I_eps, 2 * n_pg);
-  const double* d_vc = st.up("h_vcoul", vcoul, ncouls);
+  GppDevBufs d{};
+  d.inv = static_cast<int*>(st.buf("h_inv", (size_t)ngpown * sizeof(int)));
+  d.indinv = static_cast<int*>(st.buf("h_indinv", (ncouls + 1) * sizeof(int)));
+  d.wx = static_cast<double*>(st.buf("h_wx", (wx_per_band ? nb * 3 : 3) * sizeof(double)));
+  d.wtilde = static_cast<double*>(st.buf("h_wtilde", 16 * n_pg));
+  d.aqsm = static_cast<double*>(st.buf("h_aqsm", 16 * n_bg));
+  d.aqsn = static_cast<double*>(st.buf("h_aqsn", 16 * n_bg));
+  d.I_eps = static_cast<double*>(st.buf("h_ieps", 16 * n_pg));
+  d.vcoul = static_cast<double*>(st.buf("h_vcoul", ncouls * sizeof(double)));
   double* d_ach = st.out("h_achtemp", 6);
   double* d_loc = acht_n1_loc ? st.out("h_acht_n1_loc", 6 * nb) : nullptr;
   double* d_achs = achstemp ? st.out("h_achstemp", 2) : nullptr;
   double* d_asx = asxtemp ? st.out("h_asxtemp", 6) : nullptr;
   if (st.rc) return finish(ctx, st.rc);
-  rc = gpp_forward_device(ctx, nb, nvband, ngpown, ncouls, d_inv, d_indinv, d_wx, wx_per_band, d_wt,
-                          d_am, d_an, d_I, d_vc, occ, d_ach, d_loc, d_achs, d_asx);
+  const GppHostArgs h{inv, indinv, wx, wx_per_band, wtilde, aqsm, aqsn, I_eps, vcoul};
+  rc = gpp_forward_staged(ctx, (int)nb, (int)nvband, (int)ngpown, (int)ncouls, h, d, occ, d_ach, d_loc, d_achs,
+                          d_asx);
   if (rc) return finish(ctx, rc);
   st.down(achtemp, d_ach, 6);
   st.down(acht_n1_loc, d_loc, 6 * nb);

@@ -52,6 +52,9 @@ struct bgw_ctx {
   // pinned staging ring for large host<->device copies (allocated on first use)
   std::vector<bgw::PinnedSlot> ring;
   size_t ring_next = 0;
+  // copy stream of the overlapped host-pointer GPP path and its hand-over event
+  hipStream_t copy_stream = nullptr;
+  hipEvent_t xfer_event = nullptr;
 };
 
 namespace bgw {
@@ -82,9 +85,18 @@ bool mul_ok(size_t a, size_t b, size_t* out);
 // return once the source has been consumed (the caller may reuse it); downloads
 // return with the data in `dst`.
 int upload_host(bgw_ctx* ctx, void* dst, const void* src, size_t bytes);
+int upload_host(bgw_ctx* ctx, void* dst, const void* src, size_t bytes, hipStream_t s);
 int download_host(bgw_ctx* ctx, void* dst, const void* src, size_t bytes);
 // Releases the pinned ring (the stream must be idle).
 void ring_release(bgw_ctx* ctx);
+// The context's copy stream (created on first use, default priority).
+int copy_stream(bgw_ctx* ctx, hipStream_t* out);
+// Makes `to` wait for the work enqueued so far on `from`.
+int cross_stream_wait(bgw_ctx* ctx, hipStream_t from, hipStream_t to);
+// Host gather of the columns igp(p) = indinv[inv[p]] of two [nb][ncouls] complex
+// arrays into [nb][ngpown] (indices already checked), on the host threads.
+void gather_igp_columns(const double2* am, const double2* an, const int* inv, const int* indinv, int nb,
+                        int ngpown, int ncouls, double2* gm, double2* gn);
 
 // Zero-fill kernels usable inside captured graphs (no memset nodes).
 void fill_zero(void* p, size_t bytes, hipStream_t s);
@@ -99,6 +111,20 @@ void launch_check_indices(const int* inv_igp_index, size_t ngpown, const int* in
 
 // Internal entry points implemented in the kernel translation units.
 namespace bgw {
+// Host arrays of a GPP call and the device buffers they are staged into.
+struct GppHostArgs {
+  const int *inv, *indinv;
+  const double* wx;
+  int wx_per_band;
+  const double *wtilde, *aqsm, *aqsn, *I_eps, *vcoul;
+};
+struct GppDevBufs {
+  int *inv, *indinv;
+  double *wx, *wtilde, *aqsm, *aqsn, *I_eps, *vcoul;
+};
+int gpp_forward_staged(bgw_ctx* ctx, int nb, int nvband, int ngpown, int ncouls, const GppHostArgs& h,
+                       const GppDevBufs& d, double occ, double* d_ach, double* d_loc, double* d_achs,
+                       double* d_asx);
 int gpp_forward_device(bgw_ctx* ctx, size_t nb, size_t nvband, size_t ngpown, size_t ncouls,
                        const int* inv, const int* indinv, const double* wx, int wx_per_band,
                        const double* wtilde, const double* aqsm, const double* aqsn,

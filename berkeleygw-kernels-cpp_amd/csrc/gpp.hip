@@ -14,7 +14,9 @@
 // Partial sums stay in registers per lane and per work item, then go to
 // per-workgroup partials [W][nb][3]; a fixed-order double-double pass produces
 // acht_n1_loc and achtemp (run-to-run bitwise reproducible).
+#include <algorithm>
 #include <cmath>
+#include <vector>
 
 #include "bgw_dd.h"
 #include "bgw_internal.h"
@@ -30,15 +32,17 @@ namespace {
 __global__ void k_gpp_prep(const int* __restrict__ inv, const int* __restrict__ indinv,
                            const double2* __restrict__ wt, const double2* __restrict__ aqsm,
                            const double2* __restrict__ aqsn, const double* __restrict__ vcoul,
-                           const double* __restrict__ wx, int wx_per_band, int nb, int ngpown,
-                           int ncouls, double2* __restrict__ S, double2* __restrict__ C,
-                           double* __restrict__ wxb) {
+                           const double* __restrict__ wx, int wx_per_band, int nb, int ncouls,
+                           int p0, int gathered, int ngpown, double2* __restrict__ S,
+                           double2* __restrict__ C, double* __restrict__ wxb) {
   const int n1 = blockIdx.x * blockDim.x + threadIdx.x;
-  const int p = blockIdx.y;
+  const int p = p0 + blockIdx.y;  // rows [p0, p0 + gridDim.y)
   if (n1 >= nb) return;
   const int igp = gpp_igp(inv, indinv, p, ncouls);
-  const double2 am = aqsm[(size_t)n1 * ncouls + igp];
-  const double2 an = aqsn[(size_t)n1 * ncouls + igp];
+  // gathered: aqsm/aqsn hold only the columns igp(p), as [nb][ngpown]
+  const size_t at = gathered ? (size_t)n1 * ngpown + p : (size_t)n1 * ncouls + igp;
+  const double2 am = aqsm[at];
+  const double2 an = aqsn[at];
   const double vc = vcoul[igp];
   const double2 w = wt[(size_t)p * ncouls + igp];
   // conj(am) * an, then * 0.5, * vcoul, * wtilde(p,igp), in the reference's order
@@ -268,6 +272,7 @@ __global__ __launch_bounds__(NT, F32 ? 6 : 4) void k_gpp_main(const double2* __r
                                                  const double* __restrict__ wxb,
                                                  const double* __restrict__ wxrange, int nb,
                                                  int ngpown, int ncouls, int chunk, int nchunks,
+                                                 long item0, long item1, int carry,
                                                  double2* __restrict__ partial) {
   __shared__ TileEntry tile[NT];
   __shared__ TileEntry32 tile32[F32 ? NT : 1];
@@ -280,21 +285,29 @@ __global__ __launch_bounds__(NT, F32 ? 6 : 4) void k_gpp_main(const double2* __r
   for (int bb = 0; bb < BPL; ++bb) {
     n1[bb] = band0 + tid + bb * NT;
     const int nc = n1[bb] < nb ? n1[bb] : nb - 1;  // idle lanes mirror the last band
+    // carry: a launch continuing an earlier one (same grid) resumes from the totals
+    // that launch left in this workgroup's partial slot
+    const double2* src = partial + (size_t)blockIdx.x * nb * 3 + (size_t)nc * 3;
 #pragma unroll
     for (int iw = 0; iw < 3; ++iw) {
       x[bb][iw] = wxb[nc * 3 + iw];
-      tot[bb][iw][0] = tot[bb][iw][1] = 0.0;
+      const double2 t0 = carry && n1[bb] < nb ? src[iw] : make_double2(0.0, 0.0);
+      tot[bb][iw][0] = t0.x;
+      tot[bb][iw][1] = t0.y;
     }
   }
   const double wxmin = wxrange[0], wxmax = wxrange[1];
-  const long nitems = (long)ngpown * nchunks;
+  // items [item0, item1): workgroup b takes item0 + b, item0 + b + gridDim.x, ... so
+  // that launches over consecutive ranges starting at multiples of gridDim.x give
+  // every workgroup the items, in the order, of one launch over all of them
+  const long nitems = item1;
   float x32[BPL][3];
 #pragma unroll
   for (int bb = 0; bb < BPL; ++bb)
 #pragma unroll
     for (int iw = 0; iw < 3; ++iw) x32[bb][iw] = (float)x[bb][iw];
 
-  for (long item = blockIdx.x; item < nitems; item += gridDim.x) {
+  for (long item = item0 + blockIdx.x; item < nitems; item += gridDim.x) {
     const int p = (int)(item / nchunks);
     const int c = (int)(item - (long)p * nchunks);
     const int gb = c * chunk;
@@ -477,7 +490,7 @@ __global__ __launch_bounds__(NT) void k_gpp_asx(const int* __restrict__ inv, con
 template <int NT, int BPL, bool F32>
 int launch_main(bgw_ctx* ctx, const double2* wt, const double2* I, const double2* S,
                 const double* wxb, const double* range, int nb, int ngpown, int ncouls,
-                double2* partial_ws, int* W_out, int chunk) {
+                double2* partial_ws, int* W_out, int chunk, long item0, long item1) {
   const int nchunks = (ncouls + chunk - 1) / chunk;
   const int btiles = (nb + NT * BPL - 1) / (NT * BPL);
   int per_cu = 0;
@@ -491,8 +504,10 @@ int launch_main(bgw_ctx* ctx, const double2* wt, const double2* I, const double2
   *W_out = (int)W;
   if (!partial_ws) return BGW_OK;  // sizing query
   KernelTimer kt(ctx, "gpp_main");
+  if (item1 < 0) item1 = items;
   hipLaunchKernelGGL((k_gpp_main<NT, BPL, F32>), dim3((unsigned)W, btiles), dim3(NT), 0, ctx->stream, wt, I, S,
-                     wxb, range, nb, ngpown, ncouls, chunk, nchunks, partial_ws);
+                     wxb, range, nb, ngpown, ncouls, chunk, nchunks, item0, item1, item0 > 0 ? 1 : 0,
+                     partial_ws);
   return check_launch(ctx, "k_gpp_main");
 }
 
@@ -507,85 +522,97 @@ MainCfg pick_main(int nb) {
   return {64, 1};
 }
 
+constexpr int kGChunk = 2048;  // G-vectors per work item
+
 template <bool F32>
 int run_main_t(bgw_ctx* ctx, MainCfg cfg, const double2* wt, const double2* I, const double2* S, const double* wxb,
-               const double* range, int nb, int ngpown, int ncouls, double2* ws, int* W) {
-  const int chunk = 2048;
+               const double* range, int nb, int ngpown, int ncouls, double2* ws, int* W, long item0, long item1) {
+  const int chunk = kGChunk;
   if (cfg.nt == 256 && cfg.bpl == 2)
-    return launch_main<256, 2, F32>(ctx, wt, I, S, wxb, range, nb, ngpown, ncouls, ws, W, chunk);
+    return launch_main<256, 2, F32>(ctx, wt, I, S, wxb, range, nb, ngpown, ncouls, ws, W, chunk, item0, item1);
   if (cfg.nt == 128 && cfg.bpl == 2)
-    return launch_main<128, 2, F32>(ctx, wt, I, S, wxb, range, nb, ngpown, ncouls, ws, W, chunk);
+    return launch_main<128, 2, F32>(ctx, wt, I, S, wxb, range, nb, ngpown, ncouls, ws, W, chunk, item0, item1);
   if (cfg.nt == 128)
-    return launch_main<128, 1, F32>(ctx, wt, I, S, wxb, range, nb, ngpown, ncouls, ws, W, chunk);
-  return launch_main<64, 1, F32>(ctx, wt, I, S, wxb, range, nb, ngpown, ncouls, ws, W, chunk);
+    return launch_main<128, 1, F32>(ctx, wt, I, S, wxb, range, nb, ngpown, ncouls, ws, W, chunk, item0, item1);
+  return launch_main<64, 1, F32>(ctx, wt, I, S, wxb, range, nb, ngpown, ncouls, ws, W, chunk, item0, item1);
 }
 
+// Launches the main kernel over work items [item0, item1) (item1 < 0: all).  A
+// launch with item0 > 0 continues the per-workgroup totals of the launches before
+// it; item0 must then be a multiple of the grid width *W.
 int run_main(bgw_ctx* ctx, MainCfg cfg, const double2* wt, const double2* I, const double2* S,
              const double* wxb, const double* range, int nb, int ngpown, int ncouls, double2* ws,
-             int* W) {
+             int* W, long item0 = 0, long item1 = -1) {
   if (ctx->precision == BGW_FP32_BULK)
-    return run_main_t<true>(ctx, cfg, wt, I, S, wxb, range, nb, ngpown, ncouls, ws, W);
-  return run_main_t<false>(ctx, cfg, wt, I, S, wxb, range, nb, ngpown, ncouls, ws, W);
+    return run_main_t<true>(ctx, cfg, wt, I, S, wxb, range, nb, ngpown, ncouls, ws, W, item0, item1);
+  return run_main_t<false>(ctx, cfg, wt, I, S, wxb, range, nb, ngpown, ncouls, ws, W, item0, item1);
 }
 
 }  // namespace
 
 // S[p][n1], C[p][n1] (sch_store1 without its wtilde(p,igp) factor), the band
-// frequencies wxb[n1][3] and their [min,max]; buffers owned by the context.
-int gpp_prep_tables(bgw_ctx* ctx, int nb, int ngpown, int ncouls, const int* inv, const int* indinv,
-                    const double* wx, int wx_per_band, const double2* wt, const double2* am, const double2* an,
-                    const double* vcoul, double2** S, double2** C, double** wxb, double** range) {
+// frequencies wxb[n1][iw] and their [min,max]; buffers owned by the context.
+namespace {
+
+struct PrepTables {
+  double2 *S, *C;
+  double *wxb, *range;
+};
+
+int prep_alloc(bgw_ctx* ctx, int nb, int ngpown, PrepTables* t) {
   void *pS, *pC, *pwx, *prange;
   int rc = pool_get(ctx, "ws_S", (size_t)nb * ngpown * 16, &pS);
   if (!rc) rc = pool_get(ctx, "ws_C", (size_t)nb * ngpown * 16, &pC);
   if (!rc) rc = pool_get(ctx, "ws_wxb", (size_t)nb * 3 * 8, &pwx);
   if (!rc) rc = pool_get(ctx, "ws_range", 64, &prange);
   if (rc) return rc;
-  *S = static_cast<double2*>(pS);
-  *C = static_cast<double2*>(pC);
-  *wxb = static_cast<double*>(pwx);
-  *range = static_cast<double*>(prange);
-  hipLaunchKernelGGL(k_gpp_prep, dim3((nb + 255) / 256, ngpown), dim3(256), 0, ctx->stream, inv, indinv, wt, am, an,
-                     vcoul, wx, wx_per_band, nb, ngpown, ncouls, *S, *C, *wxb);
-  hipLaunchKernelGGL(k_wx_range, dim3(1), dim3(256), 0, ctx->stream, *wxb, nb * 3, *range);
-  return check_launch(ctx, "k_gpp_prep");
+  t->S = static_cast<double2*>(pS);
+  t->C = static_cast<double2*>(pC);
+  t->wxb = static_cast<double*>(pwx);
+  t->range = static_cast<double*>(prange);
+  return BGW_OK;
 }
 
-// Forward driver on device pointers; enqueues on ctx->stream.
-int gpp_forward_device(bgw_ctx* ctx, size_t nb_, size_t nvband_, size_t ngpown_, size_t ncouls_,
-                       const int* inv, const int* indinv, const double* wx, int wx_per_band,
-                       const double* wtilde, const double* aqsm, const double* aqsn,
-                       const double* I_eps, const double* vcoul, double occ, double* achtemp,
-                       double* acht_n1_loc, double* achstemp, double* asxtemp) {
-  const int nb = (int)nb_, nvband = (int)nvband_, ngpown = (int)ngpown_, ncouls = (int)ncouls_;
-  hipStream_t s = ctx->stream;
-  const double2* wt = reinterpret_cast<const double2*>(wtilde);
-  const double2* I = reinterpret_cast<const double2*>(I_eps);
-  const double2* am = reinterpret_cast<const double2*>(aqsm);
-  const double2* an = reinterpret_cast<const double2*>(aqsn);
+// S and C of rows [p0, p1); the rows containing p = 0 also write wxb.
+// gathered != 0: am/an are the [nb][ngpown] gathers of the columns igp(p).
+void prep_rows(bgw_ctx* ctx, const PrepTables& t, int nb, int p0, int p1, int ncouls, const int* inv,
+               const int* indinv, const double* wx, int wx_per_band, const double2* wt, const double2* am,
+               const double2* an, const double* vcoul, int gathered = 0, int ngpown = 0) {
+  hipLaunchKernelGGL(k_gpp_prep, dim3((nb + 255) / 256, p1 - p0), dim3(256), 0, ctx->stream, inv, indinv, wt, am,
+                     an, vcoul, wx, wx_per_band, nb, ncouls, p0, gathered, ngpown, t.S, t.C, t.wxb);
+}
 
-  const MainCfg cfg = pick_main(nb);
+// Work-item bookkeeping of one forward call: launch shape and the partial buffers.
+struct Fwd {
+  MainCfg cfg;
   int W = 0;
-  run_main(ctx, cfg, nullptr, nullptr, nullptr, nullptr, nullptr, nb, ngpown, ncouls, nullptr, &W);
+  double2* part = nullptr;
+  double4* band = nullptr;
+};
+
+int fwd_begin(bgw_ctx* ctx, int nb, int ngpown, int ncouls, Fwd* f) {
+  f->cfg = pick_main(nb);
+  run_main(ctx, f->cfg, nullptr, nullptr, nullptr, nullptr, nullptr, nb, ngpown, ncouls, nullptr, &f->W);
   void *ppart, *pband;
-  int rc = pool_get(ctx, "ws_partial", (size_t)W * nb * 3 * 16, &ppart);
+  int rc = pool_get(ctx, "ws_partial", (size_t)f->W * nb * 3 * 16, &ppart);
   if (!rc) rc = pool_get(ctx, "ws_band_dd", (size_t)nb * 3 * 32, &pband);
   if (rc) return rc;
-  launch_check_indices(inv, ngpown_, indinv, ncouls_ + 1, ncouls, ncouls, ctx->d_flag, s);
-  double2 *S, *C;
-  double *wxb, *range;
-  rc = gpp_prep_tables(ctx, nb, ngpown, ncouls, inv, indinv, wx, wx_per_band, wt, am, an, vcoul, &S, &C, &wxb, &range);
-  if (rc) return rc;
+  f->part = static_cast<double2*>(ppart);
+  f->band = static_cast<double4*>(pband);
+  return BGW_OK;
+}
 
-  rc = run_main(ctx, cfg, wt, I, S, wxb, range, nb, ngpown, ncouls, static_cast<double2*>(ppart), &W);
-  if (rc) return rc;
-  hipLaunchKernelGGL(k_reduce_bands, dim3((nb * 3 + 15) / 16), dim3(256), 0, s,
-                     static_cast<const double2*>(ppart), W, nb, reinterpret_cast<double2*>(acht_n1_loc),
-                     static_cast<double4*>(pband));
-  hipLaunchKernelGGL(k_reduce_achtemp, dim3(1), dim3(256), 0, s, static_cast<const double4*>(pband), nb,
-                     reinterpret_cast<double2*>(achtemp));
+// Band and frequency reductions, then the optional asxtemp and achstemp terms.
+int fwd_end(bgw_ctx* ctx, const Fwd& f, int nb, int nvband, int ngpown, int ncouls, const int* inv,
+            const int* indinv, const double2* wt, const double2* I, const double2* am, const double2* an,
+            const double* vcoul, const double* wxb, double occ, double* achtemp, double* acht_n1_loc,
+            double* achstemp, double* asxtemp) {
+  hipStream_t s = ctx->stream;
+  hipLaunchKernelGGL(k_reduce_bands, dim3((nb * 3 + 15) / 16), dim3(256), 0, s, f.part, f.W, nb,
+                     reinterpret_cast<double2*>(acht_n1_loc), f.band);
+  hipLaunchKernelGGL(k_reduce_achtemp, dim3(1), dim3(256), 0, s, f.band, nb, reinterpret_cast<double2*>(achtemp));
+  int rc;
   if ((rc = check_launch(ctx, "k_reduce"))) return rc;
-
   if (asxtemp) {
     if (nvband == 0) {
       fill_zero(asxtemp, 6 * sizeof(double), s);
@@ -612,6 +639,123 @@ int gpp_forward_device(bgw_ctx* ctx, size_t nb_, size_t nvband_, size_t ngpown_,
     if (rc) return rc;
   }
   return BGW_OK;
+}
+
+}  // namespace
+
+int gpp_prep_tables(bgw_ctx* ctx, int nb, int ngpown, int ncouls, const int* inv, const int* indinv,
+                    const double* wx, int wx_per_band, const double2* wt, const double2* am, const double2* an,
+                    const double* vcoul, double2** S, double2** C, double** wxb, double** range) {
+  PrepTables t;
+  int rc = prep_alloc(ctx, nb, ngpown, &t);
+  if (rc) return rc;
+  *S = t.S;
+  *C = t.C;
+  *wxb = t.wxb;
+  *range = t.range;
+  prep_rows(ctx, t, nb, 0, ngpown, ncouls, inv, indinv, wx, wx_per_band, wt, am, an, vcoul);
+  hipLaunchKernelGGL(k_wx_range, dim3(1), dim3(256), 0, ctx->stream, t.wxb, nb * 3, t.range);
+  return check_launch(ctx, "k_gpp_prep");
+}
+
+// Forward driver on device pointers; enqueues on ctx->stream.
+int gpp_forward_device(bgw_ctx* ctx, size_t nb_, size_t nvband_, size_t ngpown_, size_t ncouls_,
+                       const int* inv, const int* indinv, const double* wx, int wx_per_band,
+                       const double* wtilde, const double* aqsm, const double* aqsn,
+                       const double* I_eps, const double* vcoul, double occ, double* achtemp,
+                       double* acht_n1_loc, double* achstemp, double* asxtemp) {
+  const int nb = (int)nb_, nvband = (int)nvband_, ngpown = (int)ngpown_, ncouls = (int)ncouls_;
+  const double2* wt = reinterpret_cast<const double2*>(wtilde);
+  const double2* I = reinterpret_cast<const double2*>(I_eps);
+  const double2* am = reinterpret_cast<const double2*>(aqsm);
+  const double2* an = reinterpret_cast<const double2*>(aqsn);
+  Fwd f;
+  int rc = fwd_begin(ctx, nb, ngpown, ncouls, &f);
+  if (rc) return rc;
+  launch_check_indices(inv, ngpown_, indinv, ncouls_ + 1, ncouls, ncouls, ctx->d_flag, ctx->stream);
+  double2 *S, *C;
+  double *wxb, *range;
+  rc = gpp_prep_tables(ctx, nb, ngpown, ncouls, inv, indinv, wx, wx_per_band, wt, am, an, vcoul, &S, &C, &wxb, &range);
+  if (rc) return rc;
+  rc = run_main(ctx, f.cfg, wt, I, S, wxb, range, nb, ngpown, ncouls, f.part, &f.W);
+  if (rc) return rc;
+  return fwd_end(ctx, f, nb, nvband, ngpown, ncouls, inv, indinv, wt, I, am, an, vcoul, wxb, occ, achtemp,
+                 acht_n1_loc, achstemp, asxtemp);
+}
+
+// Forward driver on HOST pointers with the transfer of wtilde and I_eps overlapped:
+// their rows go up in ranges on the context's copy stream while the main kernel
+// runs over the work items of the rows already resident (one launch per range,
+// each continuing the per-workgroup totals of the last, so the results equal
+// gpp_forward_device's bit for bit).  `d` holds the device buffers the arrays are
+// staged into; the other inputs are uploaded first.
+int gpp_forward_staged(bgw_ctx* ctx, int nb, int nvband, int ngpown, int ncouls, const GppHostArgs& h,
+                       const GppDevBufs& d, double occ, double* d_ach, double* d_loc, double* d_achs,
+                       double* d_asx) {
+  int rc;
+  hipStream_t cs;
+  if ((rc = copy_stream(ctx, &cs))) return rc;
+  const size_t row = (size_t)ncouls * 16;
+  // inputs other than wtilde / I_eps, then the compute stream waits for them
+  if ((rc = upload_host(ctx, d.inv, h.inv, (size_t)ngpown * sizeof(int), cs))) return rc;
+  if ((rc = upload_host(ctx, d.indinv, h.indinv, (size_t)(ncouls + 1) * sizeof(int), cs))) return rc;
+  if ((rc = upload_host(ctx, d.wx, h.wx, (size_t)(h.wx_per_band ? nb * 3 : 3) * sizeof(double), cs))) return rc;
+  if ((rc = upload_host(ctx, d.vcoul, h.vcoul, (size_t)ncouls * sizeof(double), cs))) return rc;
+  // aqsm/aqsn: whole, when asxtemp or achstemp read them; otherwise only the columns
+  // igp(p) that sch_store1 reads, gathered on the host as [nb][ngpown]
+  const int gathered = !d_achs && !d_asx;
+  std::vector<double2> gm, gn;
+  if (gathered) {
+    gm.resize((size_t)nb * ngpown);
+    gn.resize((size_t)nb * ngpown);
+    const double2* hm = reinterpret_cast<const double2*>(h.aqsm);
+    const double2* hn = reinterpret_cast<const double2*>(h.aqsn);
+    gather_igp_columns(hm, hn, h.inv, h.indinv, nb, ngpown, ncouls, gm.data(), gn.data());
+    if ((rc = upload_host(ctx, d.aqsm, gm.data(), gm.size() * 16, cs))) return rc;
+    if ((rc = upload_host(ctx, d.aqsn, gn.data(), gn.size() * 16, cs))) return rc;
+  } else {
+    if ((rc = upload_host(ctx, d.aqsm, h.aqsm, (size_t)nb * row, cs))) return rc;
+    if ((rc = upload_host(ctx, d.aqsn, h.aqsn, (size_t)nb * row, cs))) return rc;
+  }
+  if ((rc = cross_stream_wait(ctx, cs, ctx->stream))) return rc;
+  Fwd f;
+  if ((rc = fwd_begin(ctx, nb, ngpown, ncouls, &f))) return rc;
+  launch_check_indices(d.inv, ngpown, d.indinv, (size_t)ncouls + 1, ncouls, ncouls, ctx->d_flag, ctx->stream);
+  PrepTables t;
+  if ((rc = prep_alloc(ctx, nb, ngpown, &t))) return rc;
+  const double2* wt = reinterpret_cast<const double2*>(d.wtilde);
+  const double2* I = reinterpret_cast<const double2*>(d.I_eps);
+  const double2* am = reinterpret_cast<const double2*>(d.aqsm);
+  const double2* an = reinterpret_cast<const double2*>(d.aqsn);
+  // ranges of work items: multiples of the grid width W, about 8 ranges
+  const int nchunks = (ncouls + kGChunk - 1) / kGChunk;
+  const long items = (long)ngpown * nchunks;
+  const long per = std::max<long>(1, (items + 8L * f.W - 1) / (8L * f.W)) * f.W;
+  int rows_up = 0;
+  for (long i0 = 0; i0 < items; i0 += per) {
+    const long i1 = std::min(items, i0 + per);
+    const int rows_need = (int)((i1 + nchunks - 1) / nchunks);  // rows holding items < i1
+    if (rows_need > rows_up) {
+      const size_t off = (size_t)rows_up * row, n = (size_t)(rows_need - rows_up) * row;
+      if ((rc = upload_host(ctx, static_cast<char*>(static_cast<void*>(d.wtilde)) + off,
+                            static_cast<const char*>(static_cast<const void*>(h.wtilde)) + off, n, cs)))
+        return rc;
+      if ((rc = upload_host(ctx, static_cast<char*>(static_cast<void*>(d.I_eps)) + off,
+                            static_cast<const char*>(static_cast<const void*>(h.I_eps)) + off, n, cs)))
+        return rc;
+      if ((rc = cross_stream_wait(ctx, cs, ctx->stream))) return rc;
+      prep_rows(ctx, t, nb, rows_up, rows_need, ncouls, d.inv, d.indinv, d.wx, h.wx_per_band, wt, am, an, d.vcoul,
+                gathered, ngpown);
+      if (rows_up == 0)
+        hipLaunchKernelGGL(k_wx_range, dim3(1), dim3(256), 0, ctx->stream, t.wxb, nb * 3, t.range);
+      if ((rc = check_launch(ctx, "k_gpp_prep"))) return rc;
+      rows_up = rows_need;
+    }
+    if ((rc = run_main(ctx, f.cfg, wt, I, t.S, t.wxb, t.range, nb, ngpown, ncouls, f.part, &f.W, i0, i1)))
+      return rc;
+  }
+  return fwd_end(ctx, f, nb, nvband, ngpown, ncouls, d.inv, d.indinv, wt, I, am, an, d.vcoul, t.wxb, occ, d_ach,
+                 d_loc, d_achs, d_asx);
 }
 
 }  // namespace bgw

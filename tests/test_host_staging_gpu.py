@@ -2,7 +2,10 @@
 pinned staging ring (several 32 MiB chunks, the last one partial): their results
 must equal, bit for bit, those of the device-pointer entry points on the same
 data, for uploads (GPP forward, FF achDtemp_cor) and downloads (GPP adjoint
-gradients of ngpown x ncouls entries)."""
+gradients of ngpown x ncouls entries).  The GPP host path also overlaps the upload
+of wtilde / I_eps with main-kernel launches over row ranges (each continuing the
+per-workgroup totals of the last), and without achstemp / asxtemp uploads only the
+columns of aqsm / aqsn that sch_store1 reads."""
 import numpy as np
 import pytest
 
@@ -33,6 +36,36 @@ def test_staged_gpp_forward_matches_device_api(ctx):
     ctx.synchronize()
     for k in h:
         assert np.array_equal(out[k].cpu().numpy().reshape(h[k].shape), h[k]), k
+
+
+def test_gathered_columns_path_matches_device_api(ctx):
+    """Without achstemp/asxtemp the host path uploads only the columns igp(p) of
+    aqsm/aqsn (gathered on the host); noflagOCC_solver takes that path too."""
+    import torch
+
+    nb, nvband, ncouls, ngpown = 80, 2, 20000, 900
+    d = bgw.gpp_inputs(nb, nvband, ncouls, ngpown, 515)
+    h = ctx.gpp_solver(d, outputs=("achtemp", "acht_n1_loc"))
+    T = {k: _dev(torch, d[k]) for k in ("inv", "indinv", "wx", "wtilde", "aqsm", "aqsn", "I_eps", "vcoul")}
+    ach = torch.zeros(3, dtype=torch.complex128, device="cuda:0")
+    loc = torch.zeros(nb * 3, dtype=torch.complex128, device="cuda:0")
+    torch.cuda.synchronize()
+    P = {k: v.data_ptr() for k, v in T.items()}
+    ctx.gpp_solver_device(nb, nvband, ngpown, ncouls, P["inv"], P["indinv"], P["wx"], P["wtilde"], P["aqsm"],
+                          P["aqsn"], P["I_eps"], P["vcoul"], 1.0, ach.data_ptr(), loc.data_ptr())
+    ctx.synchronize()
+    assert np.array_equal(ach.cpu().numpy(), h["achtemp"])
+    assert np.array_equal(loc.cpu().numpy().reshape(nb, 3), h["acht_n1_loc"])
+    # the reference signature (wx[3], achtemp only)
+    d3 = dict(d, wx=np.tile(d["wx"][:3], nb))
+    T3 = _dev(torch, d3["wx"])
+    torch.cuda.synchronize()
+    ctx.gpp_solver_device(nb, nvband, ngpown, ncouls, P["inv"], P["indinv"], T3.data_ptr(), P["wtilde"],
+                          P["aqsm"], P["aqsn"], P["I_eps"], P["vcoul"], 1.0, ach.data_ptr())
+    ctx.synchronize()
+    r = ctx.noflagOCC_solver(nb, ngpown, ncouls, d["inv"], d["indinv"], d["wx"][:3], d["wtilde"], d["aqsm"],
+                             d["aqsn"], d["I_eps"], d["vcoul"])
+    assert np.array_equal(ach.cpu().numpy(), r)
 
 
 def test_staged_gpp_backward_downloads_match_device_api(ctx):
