@@ -11,6 +11,7 @@
 #include <algorithm>
 #include <cstring>
 #include <string>
+#include <vector>
 
 #include "bgw_fact.h"
 #include "bgw_internal.h"
@@ -229,8 +230,36 @@ int ff_host(bgw_ctx* ctx, int op, int nb, int nvband, int nfreqeval, int ncouls,
   } else {
     if ((rc = upload(ctx, "h_ekq", ekq, (size_t)(op == 1 ? nvband : nb), &d_ekq))) return rc;
     if ((rc = upload(ctx, "h_grid", grid, (size_t)nFreq, &d_grid))) return rc;
-    if ((rc = upload(ctx, "h_epsR", R, 2 * npg * nFreq, &d_R))) return rc;
-    if ((rc = upload(ctx, "h_epsA", A, 2 * npg * nFreq, &d_A))) return rc;
+    // Only the slices the device plan (k_ff_plan) will read are transferred.  The
+    // slice of a band is ifreq (and ifreq + 1) of compute_fact, the same function
+    // on host and device, and wx is formed with the same two exact-order operations,
+    // so the sets agree; untouched slices of the device buffer are never read.
+    std::vector<char> needR(nFreq, 0), needA(nFreq, 0);
+    const int nbands = op == 1 ? nvband : nb;
+    for (int n1 = 0; n1 < nbands; ++n1) {
+      const double wx = fmin - ekq[n1] + fstep;
+      double f1 = 0.0, f2 = 0.0;
+      int ifreq = 0;
+      if (op == 1) {
+        compute_fact(wx, nFreq, grid, f1, f2, ifreq, 1, false);
+        std::vector<char>& need = wx > 0 ? needR : needA;
+        need[ifreq] = need[ifreq + 1] = 1;
+      } else {
+        const bool occd = n1 < nvband;
+        compute_fact(wx, nFreq, grid, f1, f2, ifreq, 2, occd);
+        if ((wx > 0 && !occd) || (!(wx > 0) && occd)) needR[ifreq] = needR[ifreq + 1] = needA[ifreq] = needA[ifreq + 1] = 1;
+      }
+    }
+    void *pR, *pA;
+    if ((rc = pool_get(ctx, "h_epsR", 2 * npg * nFreq * sizeof(double), &pR))) return rc;
+    if ((rc = pool_get(ctx, "h_epsA", 2 * npg * nFreq * sizeof(double), &pA))) return rc;
+    const size_t slice = npg * 16;
+    for (int f = 0; f < nFreq; ++f) {
+      if (needR[f] && (rc = upload_host(ctx, static_cast<char*>(pR) + f * slice, R + 2 * npg * f, slice))) return rc;
+      if (needA[f] && (rc = upload_host(ctx, static_cast<char*>(pA) + f * slice, A + 2 * npg * f, slice))) return rc;
+    }
+    d_R = static_cast<const double*>(pR);
+    d_A = static_cast<const double*>(pA);
   }
   void* d_out;
   if ((rc = pool_get(ctx, "h_ffout", (size_t)(nfreqeval > 1 ? nfreqeval : 1) * 16, &d_out))) return rc;

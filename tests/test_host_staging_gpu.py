@@ -104,3 +104,36 @@ def test_staged_ff_upload_matches_device_api(ctx):
     ctx.ff_device("achDtemp_cor", f, {k: v.data_ptr() for k, v in T.items()}, out.data_ptr())
     ctx.synchronize()
     assert np.array_equal(out.cpu().numpy(), h)
+
+
+def test_ff_host_transfers_only_the_slices_its_plan_reads(ctx):
+    """The FF host path uploads only the frequency slices its bands interpolate
+    between.  A first call fills every slice of the staging buffers with NaN (its
+    bands touch all of them); a second call of the same shape with finite data must
+    then read nothing but the slices it uploaded."""
+    import torch
+
+    nb, nvband, ncouls, ngpown, nFreq, nfe = 20, 2, 4000, 50, 6, 2
+    f = bgw.ff_inputs(nb, nvband, ncouls, ngpown, nFreq, nfe, 31)
+    # call 1: unoccupied bands with wx inside every interval of the grid (loop 2, wx > 0)
+    grid = f["dFreqGrid"]
+    mids = 0.5 * (grid[:-1] + grid[1:])
+    nan = dict(f)
+    nan["ekq"] = f["ekq"].copy()
+    nan["ekq"][nvband:nvband + len(mids)] = f["freqevalmin"] + f["freqevalstep"] - mids
+    nan["I_epsR"] = np.full_like(f["I_epsR"], np.nan)
+    nan["I_epsA"] = np.full_like(f["I_epsA"], np.nan)
+    ctx.achDtemp_cor_Kernel(nan)
+    # call 2: finite data, two valence bands touching at most four slices
+    h = ctx.asxDtemp_Kernel(f)
+    assert np.all(np.isfinite(h))
+    keys = ("inv", "indinv", "ekq", "aqsm", "aqsn", "vcoul", "I_epsR", "I_epsA", "dFreqGrid")
+    T = {k: _dev(torch, f[k]) for k in keys}
+    out = torch.zeros(nfe, dtype=torch.complex128, device="cuda:0")
+    torch.cuda.synchronize()
+    ctx.ff_device("asxDtemp", f, {k: v.data_ptr() for k, v in T.items()}, out.data_ptr())
+    ctx.synchronize()
+    assert np.array_equal(out.cpu().numpy(), h)
+    # and achDtemp_cor on finite data after the NaN call
+    h2 = ctx.achDtemp_cor_Kernel(f)
+    assert np.all(np.isfinite(h2))
