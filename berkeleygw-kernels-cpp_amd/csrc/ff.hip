@@ -213,13 +213,16 @@ int ff_host(bgw_ctx* ctx, int op, int nb, int nvband, int nfreqeval, int ncouls,
             const double* R, const double* A, double* out) {
   int rc = ff_host_indices(ctx, ngpown, ncouls, inv, indinv);
   if (rc) return rc;
-  const size_t nbg = (size_t)nb * ncouls, npg = (size_t)ngpown * ncouls;
+  const size_t npg = (size_t)ngpown * ncouls;
   const int *d_inv, *d_indinv;
   const double *d_ekq = nullptr, *d_grid = nullptr, *d_am, *d_an, *d_vc, *d_R = nullptr, *d_A = nullptr;
   if ((rc = upload(ctx, "h_inv", inv, (size_t)ngpown, &d_inv))) return rc;
   if ((rc = upload(ctx, "h_indinv", indinv, (size_t)ncouls, &d_indinv))) return rc;
-  if ((rc = upload(ctx, "h_aqsm", aqsm, 2 * nbg, &d_am))) return rc;
-  if ((rc = upload(ctx, "h_aqsn", aqsn, 2 * nbg, &d_an))) return rc;
+  // asxDtemp reads only the valence rows n1 < nvband of aqsm / aqsn (its row lists
+  // hold those bands only); the other kernels read every band
+  const size_t arows = op == 1 ? (size_t)nvband : (size_t)nb;
+  if ((rc = upload(ctx, "h_aqsm", aqsm, 2 * arows * ncouls, &d_am))) return rc;
+  if ((rc = upload(ctx, "h_aqsn", aqsn, 2 * arows * ncouls, &d_an))) return rc;
   if ((rc = upload(ctx, "h_vcoul", vcoul, (size_t)ncouls, &d_vc))) return rc;
   if (op == 0) {
     // only slice 1 of I_epsR is read (FF/ff.cpp:200): stage it as slice 1 of a 2-slice array

@@ -108,9 +108,10 @@ def test_staged_ff_upload_matches_device_api(ctx):
 
 def test_ff_host_transfers_only_the_slices_its_plan_reads(ctx):
     """The FF host path uploads only the frequency slices its bands interpolate
-    between.  A first call fills every slice of the staging buffers with NaN (its
-    bands touch all of them); a second call of the same shape with finite data must
-    then read nothing but the slices it uploaded."""
+    between (and, for asxDtemp, only the valence rows of aqsm / aqsn).  A first call
+    fills every slice of the staging buffers, and the non-valence rows of aqsm /
+    aqsn, with NaN; a second call of the same shape with finite data must then read
+    nothing but what it uploaded."""
     import torch
 
     nb, nvband, ncouls, ngpown, nFreq, nfe = 20, 2, 4000, 50, 6, 2
@@ -123,6 +124,10 @@ def test_ff_host_transfers_only_the_slices_its_plan_reads(ctx):
     nan["ekq"][nvband:nvband + len(mids)] = f["freqevalmin"] + f["freqevalstep"] - mids
     nan["I_epsR"] = np.full_like(f["I_epsR"], np.nan)
     nan["I_epsA"] = np.full_like(f["I_epsA"], np.nan)
+    # and the rows of aqsm / aqsn beyond the valence bands, which asxDtemp does not transfer
+    for k in ("aqsm", "aqsn"):
+        nan[k] = f[k].copy()
+        nan[k][nvband * ncouls:] = np.nan
     ctx.achDtemp_cor_Kernel(nan)
     # call 2: finite data, two valence bands touching at most four slices
     h = ctx.asxDtemp_Kernel(f)
