@@ -142,3 +142,32 @@ def test_ff_host_transfers_only_the_slices_its_plan_reads(ctx):
     # and achDtemp_cor on finite data after the NaN call
     h2 = ctx.achDtemp_cor_Kernel(f)
     assert np.all(np.isfinite(h2))
+
+
+def test_staged_fp32_form_and_caller_stream(ctx):
+    """The FP32-bulk main kernel through the ranged, carried launches of the host
+    path equals its single device launch bit for bit, also when the context runs on
+    a caller's stream and the host call follows an asynchronous device call."""
+    import torch
+
+    nb, nvband, ncouls, ngpown = 96, 2, 24000, 700
+    d = bgw.gpp_inputs(nb, nvband, ncouls, ngpown, 99)
+    T = {k: _dev(torch, d[k]) for k in ("inv", "indinv", "wx", "wtilde", "aqsm", "aqsn", "I_eps", "vcoul")}
+    P = {k: v.data_ptr() for k, v in T.items()}
+    s = torch.cuda.Stream()
+    ctx.set_stream(s.cuda_stream)
+    ctx.set_precision(bgw.BGW_FP32_BULK)
+    try:
+        ach = torch.zeros(3, dtype=torch.complex128, device="cuda:0")
+        loc = torch.zeros(nb * 3, dtype=torch.complex128, device="cuda:0")
+        torch.cuda.synchronize()
+        ctx.gpp_solver_device(nb, nvband, ngpown, ncouls, P["inv"], P["indinv"], P["wx"], P["wtilde"], P["aqsm"],
+                              P["aqsn"], P["I_eps"], P["vcoul"], 1.0, ach.data_ptr(), loc.data_ptr())
+        # no synchronisation here: the host call is ordered after it on the same stream
+        h = ctx.gpp_solver(d, outputs=("achtemp", "acht_n1_loc"))
+        ctx.synchronize()
+        assert np.array_equal(ach.cpu().numpy(), h["achtemp"])
+        assert np.array_equal(loc.cpu().numpy().reshape(nb, 3), h["acht_n1_loc"])
+    finally:
+        ctx.set_precision(bgw.BGW_FP64)
+        ctx.set_stream(None)
