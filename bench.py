@@ -173,6 +173,38 @@ def adjoint_leg(bgw, torch, dev, stream, ctx, reps=3):
     return res
 
 
+def host_api_leg(bgw, reps=3):
+    """Host-pointer calls as a drop-in caller makes them (pageable numpy inputs, results
+    back on the host): GPP noflagOCC-style (achtemp + acht_n1_loc) at the benchmark
+    workload, and the FF asxDtemp / achDtemp_cor kernels at config 4.  One warm-up call
+    each (it sizes the pools and the pinned ring), then the best of `reps`."""
+    res = {}
+    ctx = bgw.Context(0)
+    log("[host api] GPP inputs")
+    d = bgw.gpp_config_inputs(1)
+    flops = bgw.gpp_flops(d["nb"], d["ngpown"], d["ncouls"])
+
+    def best(fn):
+        fn()
+        ts = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            fn()
+            ts.append((time.perf_counter() - t0) * 1e3)
+        return min(ts)
+
+    ms = best(lambda: ctx.gpp_solver(d, outputs=("achtemp", "acht_n1_loc")))
+    res.update(gpp_ms=ms, gpp_tflops=flops / (ms * 1e-3) / 1e12,
+               gpp_input_bytes=int(sum(np.asarray(d[k]).nbytes for k in ("wtilde", "I_eps", "aqsm", "aqsn"))))
+    del d
+    log("[host api] FF inputs")
+    f = bgw.ff_config4_inputs()
+    res["ff_asxDtemp_ms"] = best(lambda: ctx.asxDtemp_Kernel(f))
+    res["ff_achDtemp_cor_ms"] = best(lambda: ctx.achDtemp_cor_Kernel(f))
+    ctx.close()
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -184,6 +216,9 @@ def main():
     ap.add_argument("--no-extras", action="store_true", help="skip the FP32-bulk, full-output, FF and adjoint legs")
     ap.add_argument("--ff-only", action="store_true",
                     help="only run the FF leg (config 4) and print its results as one JSON line")
+    ap.add_argument("--host-api-only", action="store_true",
+                    help="only time the host-pointer entry points (GPP benchmark workload, FF config 4), "
+                         "PCIe transfers included, and print one JSON line")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run a device-to-device copy of BYTES (reference for rocprofv3 FETCH/WRITE_SIZE)")
     args = ap.parse_args()
@@ -208,6 +243,10 @@ def main():
         b.copy_(a)
         torch.cuda.synchronize()
         print(json.dumps({"copy_bytes": n * 8}), flush=True)
+        return
+
+    if args.host_api_only:
+        print(json.dumps({"host_api": host_api_leg(bgw)}), flush=True)
         return
 
     if args.ff_only:
