@@ -11,14 +11,20 @@ namespace bgw {
 //    search is remapped to nFreq-2, so the first interval extrapolates from the top;
 //  * loop 2 with wx>0: weights times -1/2 and no remap (its test is ifreq==-1);
 //  * loop 2, unoccupied, wx<=0: nothing is written.
-// fact1, fact2, ifreq are in/out, like the reference's reference arguments.
-__host__ __device__ inline void compute_fact(double wx, int nFreq, const double* grid, double& fact1,
+// fact1, fact2, ifreq are in/out, like the reference's reference arguments.  When no
+// interval brackets wx, the incoming ifreq survives the search; the reference then
+// reads the grid at it even when it lies outside [0, nFreq-2].  Here that case
+// returns false before any read (fact1, fact2 unchanged); the device plan and the
+// host slice selection always start from ifreq = 0 and never meet it.
+__host__ __device__ inline bool compute_fact(double wx, int nFreq, const double* grid, double& fact1,
                                              double& fact2, int& ifreq, int loop, bool flag_occ) {
+  auto in_grid = [&]() { return ifreq >= 0 && ifreq <= nFreq - 2; };
   if (loop == 1) {
     const double w = wx > 0.0 ? wx : -wx;
     for (int k = 0; k < nFreq - 1; ++k)
       if (w > grid[k] && w < grid[k + 1]) ifreq = k;
     if (ifreq == 0) ifreq = nFreq - 2;
+    if (!in_grid()) return false;
     const double den = grid[ifreq + 1] - grid[ifreq];
     if (wx > 0.0) {
       fact1 = (grid[ifreq + 1] - wx) / den;
@@ -27,12 +33,13 @@ __host__ __device__ inline void compute_fact(double wx, int nFreq, const double*
       fact1 = (grid[ifreq + 1] + wx) / den;
       fact2 = (-grid[ifreq] - wx) / den;
     }
-    return;
+    return true;
   }
   if (loop == 2 && wx > 0.0) {
     for (int k = 0; k < nFreq - 1; ++k)
       if (wx > grid[k] && wx < grid[k + 1]) ifreq = k;
     if (ifreq == -1) ifreq = nFreq - 2;
+    if (!in_grid()) return false;
     const double den = grid[ifreq + 1] - grid[ifreq];
     fact1 = -0.5 * (grid[ifreq + 1] - wx) / den;
     fact2 = -0.5 * (wx - grid[ifreq]) / den;
@@ -46,6 +53,7 @@ __host__ __device__ inline void compute_fact(double wx, int nFreq, const double*
     fact1 = (grid[ifreq + 1] - w) / den;
     fact2 = (w - grid[ifreq]) / den;
   }
+  return true;
 }
 
 }  // namespace bgw

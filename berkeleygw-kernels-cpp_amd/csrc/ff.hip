@@ -285,8 +285,7 @@ extern "C" {
 int bgw_compute_fact(double wx, int nFreq, const double* dFreqGrid, double* fact1, double* fact2, int* ifreq,
                      int loop, int flag_occ) {
   if (!dFreqGrid || !fact1 || !fact2 || !ifreq || nFreq < 2) return BGW_EINVAL;
-  compute_fact(wx, nFreq, dFreqGrid, *fact1, *fact2, *ifreq, loop, flag_occ != 0);
-  return BGW_OK;
+  return compute_fact(wx, nFreq, dFreqGrid, *fact1, *fact2, *ifreq, loop, flag_occ != 0) ? BGW_OK : BGW_EINVAL;
 }
 
 int bgw_achsDtemp_Kernel(bgw_ctx* ctx, int number_bands, int ngpown, int ncouls, const int* inv_igp_index,

@@ -136,7 +136,9 @@ int bgw_gpp_backward_device(bgw_ctx* ctx, size_t number_bands, size_t nvband, si
 /* Replaces compute_fact (FF/ff.cpp:103-152), including its quirks: strict '<'
  * bracketing, ifreq==0 -> nFreq-2 remap for loop 1 and loop-2 occupied, no remap
  * for loop 2 with wx>0, nothing set for loop 2 unoccupied with wx<=0.  `ifreq`,
- * `fact1`, `fact2` are in/out exactly like the reference's references.  Host only. */
+ * `fact1`, `fact2` are in/out exactly like the reference's references.  Host only.
+ * Returns BGW_EINVAL, without reading the grid, when no interval brackets wx and
+ * the incoming ifreq lies outside [0, nFreq-2] (the reference reads out of bounds). */
 int bgw_compute_fact(double wx, int nFreq, const double* dFreqGrid, double* fact1,
                      double* fact2, int* ifreq, int loop, int flag_occ);
 

@@ -540,21 +540,25 @@ void oracle_gpp_dsigma_domega(size_t nb, size_t nvband, size_t ngpown, size_t nc
 
 /* =========================================================================== FF */
 
-void oracle_compute_fact(double wx, int nFreq, const double* g, double* fact1, double* fact2,
-                         int* ifreq, int loop, int flag_occ) {
+int oracle_compute_fact(double wx, int nFreq, const double* g, double* fact1, double* fact2,
+                        int* ifreq, int loop, int flag_occ) {
   /* FF/ff.cpp:105-127: loop 1 brackets wx (or -wx) with strict '<' on both sides,
    * then remaps ifreq==0 to nFreq-2 (so wx in the first interval, or on a grid
-   * point, uses the top interval). */
+   * point, uses the top interval).  A stale incoming ifreq outside [0, nFreq-2]
+   * that no bracket overwrites would make the reference read the grid out of
+   * bounds: reported as -1, nothing read or written. */
   if (loop == 1 && wx > 0.0) {
     for (int k = 0; k < nFreq - 1; ++k)
       if (wx > g[k] && wx < g[k + 1]) *ifreq = k;
     if (*ifreq == 0) *ifreq = nFreq - 2;
+    if (*ifreq < 0 || *ifreq > nFreq - 2) return -1;
     *fact1 = (g[*ifreq + 1] - wx) / (g[*ifreq + 1] - g[*ifreq]);
     *fact2 = (wx - g[*ifreq]) / (g[*ifreq + 1] - g[*ifreq]);
   } else if (loop == 1) {
     for (int k = 0; k < nFreq - 1; ++k)
       if (-wx > g[k] && -wx < g[k + 1]) *ifreq = k;
     if (*ifreq == 0) *ifreq = nFreq - 2;
+    if (*ifreq < 0 || *ifreq > nFreq - 2) return -1;
     *fact1 = (g[*ifreq + 1] + wx) / (g[*ifreq + 1] - g[*ifreq]);
     *fact2 = (-g[*ifreq] - wx) / (g[*ifreq + 1] - g[*ifreq]);
   }
@@ -565,6 +569,7 @@ void oracle_compute_fact(double wx, int nFreq, const double* g, double* fact1, d
     for (int k = 0; k < nFreq - 1; ++k)
       if (wx > g[k] && wx < g[k + 1]) *ifreq = k;
     if (*ifreq == -1) *ifreq = nFreq - 2;
+    if (*ifreq < 0 || *ifreq > nFreq - 2) return -1;
     *fact1 = -0.5 * (g[*ifreq + 1] - wx) / (g[*ifreq + 1] - g[*ifreq]);
     *fact2 = -0.5 * (wx - g[*ifreq]) / (g[*ifreq + 1] - g[*ifreq]);
   } else if (loop == 2 && flag_occ) {
@@ -576,6 +581,7 @@ void oracle_compute_fact(double wx, int nFreq, const double* g, double* fact1, d
     *fact1 = (g[*ifreq + 1] - wx) / (g[*ifreq + 1] - g[*ifreq]);
     *fact2 = (wx - g[*ifreq]) / (g[*ifreq + 1] - g[*ifreq]);
   }
+  return 0;
 }
 
 /* sum_g aqsn(n1,g) * (f1*X(f,p,g) + f2*X(f+1,p,g) [- same of Y]) compensated;

@@ -47,7 +47,7 @@ void oracle_gpp_dsigma_domega(size_t nb, size_t nvband, size_t ngpown, size_t nc
                               double* dsigma_domega, int nthreads);
 
 /* FF/ff.cpp:103-152 */
-void oracle_compute_fact(double wx, int nFreq, const double* dFreqGrid, double* fact1,
+int oracle_compute_fact(double wx, int nFreq, const double* dFreqGrid, double* fact1,
                          double* fact2, int* ifreq, int loop, int flag_occ);
 /* FF/ff.cpp:171-209 */
 void oracle_achsDtemp(int nb, int ngpown, int ncouls, const int* inv_igp_index,

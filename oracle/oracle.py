@@ -44,7 +44,7 @@ def olib() -> C.CDLL:
         L.oracle_gpp_dsigma_domega.argtypes = gpp + [_P, _I]
         L.oracle_gpp_dsigma_domega.restype = None
         L.oracle_compute_fact.argtypes = [_D, _I, _P, C.POINTER(_D), C.POINTER(_D), C.POINTER(_I), _I, _I]
-        L.oracle_compute_fact.restype = None
+        L.oracle_compute_fact.restype = _I
         L.oracle_achsDtemp.argtypes = [_I, _I, _I, _P, _P, _P, _P, _P, _P, _P, _I]
         L.oracle_achsDtemp.restype = None
         ff = [_I, _I, _I, _I, _I, _D, _D]
@@ -128,7 +128,8 @@ def gpp_dsigma_domega(d: dict, occ: float = 1.0, nthreads: int | None = None):
 def compute_fact(wx, dFreqGrid, loop, flag_occ, ifreq=0, fact1=0.0, fact2=0.0):
     g = np.ascontiguousarray(dFreqGrid, np.float64)
     f1, f2, i = _D(fact1), _D(fact2), _I(ifreq)
-    olib().oracle_compute_fact(wx, len(g), _p(g), C.byref(f1), C.byref(f2), C.byref(i), loop, int(flag_occ))
+    if olib().oracle_compute_fact(wx, len(g), _p(g), C.byref(f1), C.byref(f2), C.byref(i), loop, int(flag_occ)):
+        return None  # stale ifreq outside the grid, no bracket: the reference would read out of bounds
     return f1.value, f2.value, i.value
 
 

@@ -65,8 +65,27 @@ def test_gpp_inputs_match_benchmark_distributions():
 @pytest.mark.parametrize("wx,loop,occ", FACT_CASES)
 def test_product_compute_fact_matches_oracle(wx, loop, occ):
     grid = np.array([0.0, 0.5, 1.0, 1.5, 2.0])
-    for init in ((0, 0.0, 0.0), (7, 3.0, 4.0)):
-        assert bgw.compute_fact(wx, grid, loop, occ, *init) == O.compute_fact(wx, grid, loop, occ, *init)
+    for init in ((0, 0.0, 0.0), (2, 3.0, 4.0), (7, 3.0, 4.0)):
+        o = O.compute_fact(wx, grid, loop, occ, *init)
+        if o is None:  # unbracketed wx with a stale ifreq beyond the grid: refused, nothing read
+            with pytest.raises(bgw.BgwError) as e:
+                bgw.compute_fact(wx, grid, loop, occ, *init)
+            assert e.value.code == bgw.BGW_EINVAL
+        else:
+            assert bgw.compute_fact(wx, grid, loop, occ, *init) == o
+
+
+def test_compute_fact_never_reads_past_the_grid():
+    """wx on a grid point leaves the incoming ifreq in place; the reference would then
+    index the grid with it.  An out-of-grid ifreq is refused (BGW_EINVAL) every time,
+    an in-grid one is used, like the reference."""
+    grid = np.array([0.0, 0.5, 1.0, 1.5, 2.0])
+    for _ in range(50):
+        with pytest.raises(bgw.BgwError):
+            bgw.compute_fact(1.0, grid, 2, False, 7, 3.0, 4.0)  # loop 2, wx > 0 on a grid point
+        assert O.compute_fact(1.0, grid, 2, False, 7, 3.0, 4.0) is None
+    f1, f2, i = bgw.compute_fact(1.0, grid, 2, False, 1, 3.0, 4.0)
+    assert i == 1 and (f1, f2) == (-0.5 * (1.0 - 1.0) / 0.5, -0.5 * (1.0 - 0.5) / 0.5)
 
 
 def test_product_compute_fact_sweep_matches_oracle():
