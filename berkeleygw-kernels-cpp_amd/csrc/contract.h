@@ -42,14 +42,20 @@ struct ContractArgs {
   double2* out;            // c128[1] result (device)
 };
 
-// max_tiles: a bound on sum_j ceil(cnt[j] / contract_row_tile(f32, !exact_tiles)) known
-// on the host, exact_tiles: whether the bound is met with equality (identity row lists),
-// est_tiles: a tighter estimate of the live tiles, used to size the K split.
-int contract_launch(bgw_ctx* ctx, ContractArgs a, int njobs, int max_tiles, bool exact_tiles, int est_tiles,
-                    const char* tname);
-// Rows (bands) per contraction tile: FP64 (f32 = false) or FP32 form, for a sparse
-// plan (frequency-slice band lists, exact_tiles = false) or a dense row list.
-int contract_row_tile(bool f32, bool sparse);
+// bm: the row tile, a value of contract_row_tile; max_tiles: a bound on
+// sum_j ceil(cnt[j] / bm) known on the host, exact_tiles: whether the bound is met
+// with equality (identity row lists), est_tiles: a tighter estimate of the live
+// tiles, used to size the K split.
+int contract_launch(bgw_ctx* ctx, ContractArgs a, int njobs, int bm, int max_tiles, bool exact_tiles,
+                    int est_tiles, const char* tname);
+// Rows (bands) per contraction tile.  Dense row lists (sparse = false) take the
+// square 64-row tile.  A sparse plan (frequency-slice band lists) of jobs with at
+// most max_job_rows rows takes the tall 80-row tile in FP64, and in the FP32 form
+// (f32) when a job can hold more rows than the square tile.  In the tall tile a
+// slice's band list fits one tile, so the slice streams once, and every wave has
+// MFMA work even for short lists; the FP32 tall tile runs its k-steps two at a time
+// (registers), which makes short FP32 lists faster in the square tile.
+int contract_row_tile(bool f32, bool sparse, int max_job_rows);
 
 // rows[i] = i, wts[i] = w, cnt[0] = n, slice[0] = slice.
 void launch_identity_rows(int* rows, double2* wts, int* cnt, int* slice, int n, double2 w, int slice_idx,

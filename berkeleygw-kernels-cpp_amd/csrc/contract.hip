@@ -30,11 +30,12 @@ typedef double f64x4 __attribute__((ext_vector_type(4)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 // Workgroup tile: BM rows (bands) x BN columns (G'), 4 waves as WR x WC, each wave
-// RB x CB 16x16 MFMA blocks.  Square form (dense row lists, and the FP32 form):
-// 64 x 64 as 2 x 2 waves of 32 x 32, the fewest LDS operand reads per MFMA.  Tall
-// form (FP64 sparse plans): 80 x 64 with each wave owning one 16-column block of
-// all 80 rows -- a frequency slice's band list (two neighbouring frequency groups,
-// ~60-80 bands) then fits one tile and its B tile is streamed once.
+// RB x CB 16x16 MFMA blocks.  Square form (dense row lists, and FP32 sparse plans of
+// short band lists): 64 x 64 as 2 x 2 waves of 32 x 32, the fewest LDS operand reads
+// per MFMA.  Tall form (other sparse plans): 80 x 64 with each wave owning one
+// 16-column block of all 80 rows -- a frequency slice's band list (two neighbouring
+// frequency groups, ~60-80 bands) then fits one tile and its B tile is streamed once.
+// contract_row_tile picks the form.
 template <bool TALL>
 struct TileShape {
   static constexpr int WR = TALL ? 1 : 2, WC = TALL ? 4 : 2;
@@ -110,21 +111,24 @@ __global__ __launch_bounds__(NT, 2) void k_contract(ContractArgs a) {
   const int wr = (wave / TS::WC) * (RB * 16), wc = (wave % TS::WC) * (CB * 16);
 
   // complex blocks by the 3M scheme: e = Ar*Br, f = Ai*Bi, h = (Ar+Ai)*(Br+Bi), so
-  // re = e - f and im = h - e - f take three real MFMAs instead of four
-  f64x4 de[RB][CB], df[RB][CB], dh[RB][CB];
+  // re = e - f and im = h - e - f take three real MFMAs instead of four.  FP64
+  // form: the MFMAs accumulate e, f, h in de, df, dh.  FP32 form: the FP32 stage sums
+  // are folded as re into de and im into df, so dh is not needed (40 VGPRs fewer
+  // in the tall tile)
+  f64x4 de[RB][CB], df[RB][CB], dh[F32 ? 1 : RB][CB];
 #pragma unroll
   for (int i = 0; i < RB; ++i)
 #pragma unroll
     for (int j = 0; j < CB; ++j) {
       de[i][j] = (f64x4){0.0, 0.0, 0.0, 0.0};
       df[i][j] = (f64x4){0.0, 0.0, 0.0, 0.0};
-      dh[i][j] = (f64x4){0.0, 0.0, 0.0, 0.0};
+      if constexpr (!F32) dh[i][j] = (f64x4){0.0, 0.0, 0.0, 0.0};
     }
 
   // each thread stages AE elements of A and BE of B per stage: idx = tid + NT*i -> (row, k)
   int arow[AE];
   bool aok[AE];
-  size_t bbase[BE];
+  int bcol[BE];  // 32-bit: the 64-bit row offset is formed at fetch time (fewer live VGPRs)
   bool bok[BE];
 #pragma unroll
   for (int i = 0; i < AE; ++i) {
@@ -137,7 +141,7 @@ __global__ __launch_bounds__(NT, 2) void k_contract(ContractArgs a) {
   for (int i = 0; i < BE; ++i) {
     const int gc = ct * BN + (tid + NT * i) / BK;
     bok[i] = gc < a.N;
-    bbase[i] = (size_t)(bok[i] ? gc : 0) * K;
+    bcol[i] = bok[i] ? gc : 0;
   }
 
   // 16x16 blocks of this wave that hold at least one live row / column: the
@@ -163,8 +167,9 @@ __global__ __launch_bounds__(NT, 2) void k_contract(ContractArgs a) {
 #pragma unroll
     for (int i = 0; i < BE; ++i) {
       const int k = min(kb + (tid + NT * i) % BK, K - 1);
-      bx[i] = X[bbase[i] + k];
-      if constexpr (MODE == kModeDiff) by[i] = Y[bbase[i] + k];
+      const size_t off = (size_t)bcol[i] * K + k;
+      bx[i] = X[off];
+      if constexpr (MODE == kModeDiff) by[i] = Y[off];
     }
   };
   auto swz = [](int k) { return 16 * (k & 1) + k; };
@@ -240,7 +245,10 @@ __global__ __launch_bounds__(NT, 2) void k_contract(ContractArgs a) {
           sf[i][j] = (f32x4){0.f, 0.f, 0.f, 0.f};
           sh[i][j] = (f32x4){0.f, 0.f, 0.f, 0.f};
         }
-#pragma unroll
+      // the tall FP32 tile fits 256 VGPRs only with the operand reads of at most two
+      // k-steps in flight (fully unrolled, the compiler hoists all four and spills)
+      constexpr int kSteps = TALL ? 2 : 4;
+#pragma unroll kSteps
       for (int kk = 0; kk < BK; kk += 4) {
         const int kr = kk + (lane >> 4), cl = lane & 15, sk = swz(kr);
         float ar[RB], ai[RB], sa[RB], br[CB], bi[CB], sb[CB];
@@ -272,9 +280,9 @@ __global__ __launch_bounds__(NT, 2) void k_contract(ContractArgs a) {
         for (int j = 0; j < CB; ++j)
 #pragma unroll
           for (int v = 0; v < 4; ++v) {
-            de[i][j][v] += (double)se[i][j][v];
-            df[i][j][v] += (double)sf[i][j][v];
-            dh[i][j][v] += (double)sh[i][j][v];
+            const double e = se[i][j][v], f = sf[i][j][v];
+            de[i][j][v] += e - f;
+            df[i][j][v] += ((double)sh[i][j][v] - e) - f;
           }
     }
     if (more) stage_store(buf ^ 1, kb + BK);
@@ -302,8 +310,14 @@ __global__ __launch_bounds__(NT, 2) void k_contract(ContractArgs a) {
         const int n1 = rows[r];
         const double2 am = a.aqsm[(size_t)n1 * a.ncouls + igp];
         const double2 w = cmul(a.wts[(size_t)job * a.max_rows + r], make_double2(vc * am.x, -vc * am.y));
-        const double e = de[i][j][v], f = df[i][j][v];
-        const double2 t = cmul(w, make_double2(e - f, dh[i][j][v] - e - f));
+        double2 s;
+        if constexpr (F32) {
+          s = make_double2(de[i][j][v], df[i][j][v]);
+        } else {
+          const double e = de[i][j][v], f = df[i][j][v];
+          s = make_double2(e - f, dh[i][j][v] - e - f);
+        }
+        const double2 t = cmul(w, s);
         acc = cdd_add(acc, t.x, t.y);
       }
     }
@@ -368,22 +382,22 @@ int pick_ksplit(int K, int BK, size_t tiles_x_ntn, int slots) {
 
 }  // namespace
 
-int contract_launch(bgw_ctx* ctx, ContractArgs a, int njobs, int max_tiles, bool exact_tiles, int est_tiles,
-                    const char* tname) {
+int contract_launch(bgw_ctx* ctx, ContractArgs a, int njobs, int bm, int max_tiles, bool exact_tiles,
+                    int est_tiles, const char* tname) {
   const int ntn = (a.N + BN - 1) / BN;
   const bool f32 = ctx->precision == BGW_FP32_BULK;
   const int mode = a.Y ? kModeDiff : (a.thr2 > 0.0 ? kModeMask : kModePlain);
-  // the tall tile for FP64 sparse plans (frequency-slice band lists), square otherwise
-  const bool tall = !f32 && !exact_tiles;
+  if (bm != TileShape<true>::BM && bm != TileShape<false>::BM) return fail(ctx, BGW_EINVAL, "unsupported row tile");
+  const bool tall = bm == TileShape<true>::BM;
   void (*kern)(ContractArgs) = nullptr;
   if (mode == kModeDiff)
-    kern = f32 ? k_contract<kModeDiff, true, false>
+    kern = f32 ? (tall ? k_contract<kModeDiff, true, true> : k_contract<kModeDiff, true, false>)
                : (tall ? k_contract<kModeDiff, false, true> : k_contract<kModeDiff, false, false>);
   else if (mode == kModeMask)
-    kern = f32 ? k_contract<kModeMask, true, false>
+    kern = f32 ? (tall ? k_contract<kModeMask, true, true> : k_contract<kModeMask, true, false>)
                : (tall ? k_contract<kModeMask, false, true> : k_contract<kModeMask, false, false>);
   else
-    kern = f32 ? k_contract<kModePlain, true, false>
+    kern = f32 ? (tall ? k_contract<kModePlain, true, true> : k_contract<kModePlain, true, false>)
                : (tall ? k_contract<kModePlain, false, true> : k_contract<kModePlain, false, false>);
   // Row counts per job are only known on the device (plan kernel), so the grid
   // covers max_tiles row tiles, a host bound on sum_j ceil(cnt[j]/bm); slots
@@ -420,7 +434,7 @@ int contract_launch(bgw_ctx* ctx, ContractArgs a, int njobs, int max_tiles, bool
   int rc = pool_get(ctx, "ws_tile_start", (size_t)(njobs + 1) * sizeof(int), &pts);
   if (rc) return rc;
   a.tile_start = static_cast<const int*>(pts);
-  hipLaunchKernelGGL(k_tile_prefix, dim3(1), dim3(kPrefixThreads), 0, ctx->stream, a.cnt, njobs, contract_row_tile(f32, !exact_tiles),
+  hipLaunchKernelGGL(k_tile_prefix, dim3(1), dim3(kPrefixThreads), 0, ctx->stream, a.cnt, njobs, bm,
                      static_cast<int*>(pts));
   if ((rc = check_launch(ctx, "k_tile_prefix"))) return rc;
   void* pp;
@@ -467,13 +481,17 @@ int contract_achstemp(bgw_ctx* ctx, int nb, int ngpown, int ncouls, const int* i
   a.ncouls = ncouls;
   a.thr2 = kTo1 * kTo1;  // |schs| > to1 cut of the static COH term
   a.out = achstemp;
-  const int bm = contract_row_tile(ctx->precision == BGW_FP32_BULK, false);
+  const int bm = contract_row_tile(ctx->precision == BGW_FP32_BULK, false, nb);
   const int tiles = (nb + bm - 1) / bm;
-  return contract_launch(ctx, a, 1, tiles, true, tiles, "contract_achs");
+  return contract_launch(ctx, a, 1, bm, tiles, true, tiles, "contract_achs");
 }
 
-int contract_row_tile(bool f32, bool sparse) {
-  return !f32 && sparse ? TileShape<true>::BM : TileShape<false>::BM;
+int contract_row_tile(bool f32, bool sparse, int max_job_rows) {
+  // measured at config 4: dense achsDtemp 10% slower in 80-row tiles; asxDtemp (2 rows
+  // per job) 10% faster tall in FP64 and 11% faster square in FP32; achDtemp_cor (up to
+  // 79 rows per slice) 13% faster tall in FP32
+  const bool tall = sparse && (!f32 || max_job_rows > TileShape<false>::BM);
+  return tall ? TileShape<true>::BM : TileShape<false>::BM;
 }
 
 }  // namespace bgw

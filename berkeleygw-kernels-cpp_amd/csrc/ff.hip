@@ -128,9 +128,9 @@ int ff_device(bgw_ctx* ctx, int op, int nb, int nvband, int nfreqeval, int ncoul
                          static_cast<int*>(pslice), nb, make_double2(-0.5, 0.0), 1, s);
     a.Xa = reinterpret_cast<const double2*>(I_epsR);
     a.jobs_a = 1;
-    const int BMd = contract_row_tile(f32, false);
+    const int BMd = contract_row_tile(f32, false, nb);
     const int tiles = (nb + BMd - 1) / BMd;
-    return contract_launch(ctx, a, 1, tiles, true, tiles, "contract_ff_achs");
+    return contract_launch(ctx, a, 1, BMd, tiles, true, tiles, "contract_ff_achs");
   }
   void *pbj, *pbw;
   if ((rc = pool_get(ctx, "ws_ff_band_job", (size_t)nb * sizeof(int), &pbj))) return rc;
@@ -141,7 +141,7 @@ int ff_device(bgw_ctx* ctx, int op, int nb, int nvband, int nfreqeval, int ncoul
   if ((rc = check_launch(ctx, "k_ff_plan"))) return rc;
   // every listed band is a row of exactly two jobs, so sum_j ceil(cnt_j/BM) is at
   // most njobs + ceil(2*maxrows/BM), and never above njobs * ceil(maxrows/BM)
-  const int BMr = contract_row_tile(f32, true);
+  const int BMr = contract_row_tile(f32, true, maxrows);
   const long long tiles_dense = (long long)njobs * ((maxrows + BMr - 1) / BMr);
   const long long tiles_rows = (long long)njobs + (2LL * maxrows + BMr - 1) / BMr;
   const int max_tiles = (int)(tiles_dense < tiles_rows ? tiles_dense : tiles_rows);
@@ -151,12 +151,12 @@ int ff_device(bgw_ctx* ctx, int op, int nb, int nvband, int nfreqeval, int ncoul
     a.Xa = reinterpret_cast<const double2*>(I_epsR);
     a.Xb = reinterpret_cast<const double2*>(I_epsA);
     a.jobs_a = nFreq;
-    rc = contract_launch(ctx, a, njobs, max_tiles, false, est_tiles, "contract_ff_asx");
+    rc = contract_launch(ctx, a, njobs, BMr, max_tiles, false, est_tiles, "contract_ff_asx");
   } else {
     a.Xa = reinterpret_cast<const double2*>(I_epsR);
     a.Y = reinterpret_cast<const double2*>(I_epsA);
     a.jobs_a = nFreq;
-    rc = contract_launch(ctx, a, njobs, max_tiles, false, est_tiles, "contract_ff_cor");
+    rc = contract_launch(ctx, a, njobs, BMr, max_tiles, false, est_tiles, "contract_ff_cor");
   }
   if (rc) return rc;
   launch_replicate(a.out, nfreqeval, s);

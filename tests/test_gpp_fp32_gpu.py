@@ -73,6 +73,17 @@ def test_ff_split_fp32_vs_oracle(ctx32):
     assert rel_err(cor, O.achDtemp_cor(f)) > 1e-13
 
 
+def test_ff_split_fp32_long_valence_lists(ctx32):
+    """nvband above 64: asxDtemp's slice lists can exceed the square row tile, so the
+    FP32 form takes the 80-row tile for it too (as for achDtemp_cor)."""
+    f = bgw.ff_inputs(100, 70, 600, 40, 15, 2, 4711)
+    asx = ctx32.asxDtemp_Kernel(f)
+    cor = ctx32.achDtemp_cor_Kernel(f)
+    assert rel_err(asx, O.asxDtemp(f)) < TOL32
+    assert rel_err(cor, O.achDtemp_cor(f)) < TOL32
+    assert rel_err(asx, O.asxDtemp(f)) > 1e-13
+
+
 @pytest.mark.slow
 def test_ff_config4_shape_split_fp32(ctx32):
     f = bgw.ff_config4_inputs(ncouls=2048)
