@@ -268,7 +268,9 @@ int bgw_create(bgw_ctx** out, int device) {
 void bgw_destroy(bgw_ctx* ctx) {
   if (!ctx) return;
   (void)hipSetDevice(ctx->device);
+  // both streams drain before any buffer they may still read or fill is freed
   (void)hipStreamSynchronize(ctx->stream);
+  if (ctx->copy_stream) (void)hipStreamSynchronize(ctx->copy_stream);
   for (auto& kv : ctx->pool)
     if (kv.second.ptr) (void)hipFree(kv.second.ptr);
   for (auto& tl : ctx->pending) {
@@ -276,10 +278,7 @@ void bgw_destroy(bgw_ctx* ctx) {
     (void)hipEventDestroy(tl.stop);
   }
   for (hipEvent_t ev : ctx->free_events) (void)hipEventDestroy(ev);
-  if (ctx->copy_stream) {
-    (void)hipStreamSynchronize(ctx->copy_stream);
-    (void)hipStreamDestroy(ctx->copy_stream);
-  }
+  if (ctx->copy_stream) (void)hipStreamDestroy(ctx->copy_stream);
   if (ctx->xfer_event) (void)hipEventDestroy(ctx->xfer_event);
   ring_release(ctx);
   if (ctx->d_flag) (void)hipFree(ctx->d_flag);

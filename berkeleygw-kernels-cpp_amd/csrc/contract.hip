@@ -28,6 +28,13 @@ namespace {
 
 typedef double f64x4 __attribute__((ext_vector_type(4)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef double f64x2 __attribute__((ext_vector_type(2)));
+
+// 16-byte load with the non-temporal hint (global_load_dwordx4 ... nt)
+__device__ inline double2 ntload(const double2* p) {
+  const f64x2 v = __builtin_nontemporal_load(reinterpret_cast<const f64x2*>(p));
+  return make_double2(v.x, v.y);
+}
 
 // Workgroup tile: BM rows (bands) x BN columns (G'), 4 waves as WR x WC, each wave
 // RB x CB 16x16 MFMA blocks.  Square form (dense row lists, and FP32 sparse plans of
@@ -168,8 +175,17 @@ __global__ __launch_bounds__(NT, 2) void k_contract(ContractArgs a) {
     for (int i = 0; i < BE; ++i) {
       const int k = min(kb + (tid + NT * i) % BK, K - 1);
       const size_t off = (size_t)bcol[i] * K + k;
-      bx[i] = X[off];
-      if constexpr (MODE == kModeDiff) by[i] = Y[off];
+      // the tall tile reads each slice once: stream it past L2 (non-temporal), which
+      // leaves the aqsntemp rows that all G' tiles of a row tile share in L2 longer
+      // (achDtemp_cor 2-3% faster); square tiles of dense lists re-read B across
+      // row tiles
+      if constexpr (TALL) {
+        bx[i] = ntload(X + off);
+        if constexpr (MODE == kModeDiff) by[i] = ntload(Y + off);
+      } else {
+        bx[i] = X[off];
+        if constexpr (MODE == kModeDiff) by[i] = Y[off];
+      }
     }
   };
   auto swz = [](int k) { return 16 * (k & 1) + k; };
