@@ -63,12 +63,13 @@ def cpu_baseline(d_full, bands: int):
     kind = "reference" if O.REF_OMP.exists() else "port"
     t0 = time.perf_counter()
     if kind == "reference":
-        O.ref_gpp_per_band(sub, kind="omp")
+        per_band = O.ref_gpp_per_band(sub, kind="omp")
+        dt = O.ref_last_seconds("omp")  # the band loop alone, timed in the adapter
     else:
-        O.gpp_forward(sub, want=("achtemp", "acht_n1_loc"), nthreads=nthr)
-    dt = time.perf_counter() - t0
+        per_band = O.gpp_forward(sub, want=("achtemp", "acht_n1_loc"), nthreads=nthr)["acht_n1_loc"]
+        dt = time.perf_counter() - t0
     flops = 15.0 * bands * sub["ngpown"] * nc * 3
-    return {"value": flops / dt / 1e12, "unit": "TFLOP/s", "cores": nthr, "kind": kind,
+    return per_band, {"value": flops / dt / 1e12, "unit": "TFLOP/s", "cores": nthr, "kind": kind,
             "sample": f"{bands} of the 512 bands of the benchmark workload (ngpown 1638 x ncouls 32768 x 3 iw), "
                       f"{dt:.2f} s wall, the reference noflagOCC_solver (GPP/gpp.cpp:307-402, OpenMP, its GCC "
                       f"flags) called once per band with that band's wx"}
@@ -85,11 +86,43 @@ def _timed_calls(torch, stream, fn, k):
     return e0.elapsed_time(e1) / k
 
 
-def ff_leg(bgw, torch, dev, stream, ctx, reps=3):
+def ff_cpu_reference(f, vals_fp64):
+    """The reference's own FF kernels (oracle/_ref, its GCC flags + OpenMP, FF/ff.cpp:171-451)
+    on the same config-4 inputs, each timed inside the adapter around the kernel alone.
+    asxDtemp and achDtemp_cor run with nfreqeval = 1: their per-frequency loops repeat the
+    same work for every frequency (the frequency term cancels, FF/ff.cpp:245,337), so the
+    nfreqeval = 10 time is ten times this.  Also reports the relative difference between
+    the reference's values and this library's FP64 values at full size."""
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import oracle as O  # checker / baseline only
+
+    if not O.REF_OMP.exists():
+        return None
+    try:
+        nthr = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    except ValueError:
+        nthr = os.cpu_count() or 1
+    f1 = dict(f, nfreqeval=1)
+    out = {"cores": nthr, "kind": "reference",
+           "sample": "config 4 inputs; asxDtemp and achDtemp_cor at nfreqeval 1 (x10 for the "
+                     "reference's nfreqeval 10, whose frequencies repeat the same work)"}
+    for op, fn, arg in (("achsDtemp", O.ref_achsDtemp, f), ("asxDtemp", O.ref_asxDtemp, f1),
+                        ("achDtemp_cor", O.ref_achDtemp_cor, f1)):
+        v = np.atleast_1d(fn(arg, kind="omp"))[0]
+        sec = O.ref_last_seconds("omp")
+        scale = 1 if op == "achsDtemp" else f["nfreqeval"]
+        out[f"{op}_s"] = sec * scale
+        b = complex(vals_fp64[op][0])
+        out[f"{op}_rel_diff_vs_fp64"] = abs(complex(v) - b) / abs(b)
+    return out
+
+
+def ff_leg(bgw, torch, dev, stream, ctx, reps=3, cpu_reference=False):
     """FF config 4 (nb 512, nvband 2, ncouls 16384, ngpown 819, nFreq 15, nfreqeval 10):
     the three FF kernels in FP64 MFMA and split-FP32 MFMA; FLOP = 8 per distinct
     complex MAC (achsDtemp nb*ngpown*ncouls, achDtemp_cor 2*n_active*ngpown*ncouls,
-    asxDtemp 2*nvband*ngpown*ncouls)."""
+    asxDtemp 2*nvband*ngpown*ncouls).  With cpu_reference, the reference's own CPU
+    kernels are timed beside them on the same inputs."""
     log("[ff] generating config 4 inputs")
     f = bgw.ff_config4_inputs()
     keys = ("inv", "indinv", "ekq", "aqsm", "aqsn", "vcoul", "I_epsR", "I_epsA", "dFreqGrid")
@@ -132,6 +165,9 @@ def ff_leg(bgw, torch, dev, stream, ctx, reps=3):
     res["value_fp64"] = {op: complex(vals[("fp64", op)][0]).__repr__() for op in names}
     del T
     torch.cuda.empty_cache()
+    if cpu_reference:
+        log("[ff] reference CPU kernels")
+        res["cpu_reference"] = ff_cpu_reference(f, {op: vals[("fp64", op)] for op in names})
     return res
 
 
@@ -358,7 +394,7 @@ def main():
 
     if not args.no_extras:
         try:
-            extras["ff_config4"] = ff_leg(bgw, torch, dev, stream, ctx)
+            extras["ff_config4"] = ff_leg(bgw, torch, dev, stream, ctx, cpu_reference=not args.no_cpu_baseline)
         except Exception as e:
             extras["ff_config4_error"] = repr(e)
         try:
@@ -369,7 +405,12 @@ def main():
     cpu = None
     if not args.no_cpu_baseline:
         log("cpu baseline")
-        cpu = cpu_baseline(d, args.cpu_bands)
+        per_band, cpu = cpu_baseline(d, args.cpu_bands)
+        # the same bands from the GPU step above, against the CPU path's values
+        nbc = per_band.shape[0]
+        extras[f"cpu_{cpu['kind']}_bands_compared"] = nbc
+        extras[f"cpu_{cpu['kind']}_max_rel_diff"] = float(
+            np.max(np.abs(f64["acht_n1_loc"][:nbc] - per_band) / np.abs(per_band)))
 
     line = {
         "metric": "GPP Σ-kernel s & FP64 TFLOP/s (1 GPU): plain FP64 vs FP32-bulk at ≤1e-6 rel err",

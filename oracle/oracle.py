@@ -77,6 +77,8 @@ def rlib(kind: str = "serial") -> C.CDLL:
         L.ref_asxDtemp.restype = None
         L.ref_achDtemp_cor.argtypes = [_I, _I, _I, _I, _I, _I, _D, _D] + [_P] * 10
         L.ref_achDtemp_cor.restype = None
+        L.ref_last_seconds.argtypes = []
+        L.ref_last_seconds.restype = _D
         _refs[kind] = L
     return _refs[kind]
 
@@ -177,25 +179,31 @@ def ref_gpp_per_band(d: dict, kind: str = "serial") -> np.ndarray:
     return out.reshape(d["nb"], 3)
 
 
-def ref_achsDtemp(f: dict) -> complex:
+def ref_achsDtemp(f: dict, kind: str = "serial") -> complex:
     out = np.zeros(1, np.complex128)
-    rlib().ref_achsDtemp(f["nb"], f["ngpown"], f["ncouls"], f["nFreq"], _p(f["inv"]), _p(f["indinv"]), _p(f["aqsn"]),
+    rlib(kind).ref_achsDtemp(f["nb"], f["ngpown"], f["ncouls"], f["nFreq"], _p(f["inv"]), _p(f["indinv"]), _p(f["aqsn"]),
                          _p(f["aqsm"]), _p(f["I_epsR"]), _p(f["vcoul"]), _p(out))
     return out[0]
 
 
-def ref_asxDtemp(f: dict) -> np.ndarray:
+def ref_asxDtemp(f: dict, kind: str = "serial") -> np.ndarray:
     out = np.zeros(f["nfreqeval"], np.complex128)
-    rlib().ref_asxDtemp(f["nb"], f["nvband"], f["nfreqeval"], f["ncouls"], f["ngpown"], f["nFreq"], f["freqevalmin"],
+    rlib(kind).ref_asxDtemp(f["nb"], f["nvband"], f["nfreqeval"], f["ncouls"], f["ngpown"], f["nFreq"], f["freqevalmin"],
                         f["freqevalstep"], f["occ"], _p(f["ekq"]), _p(f["dFreqGrid"]), _p(f["inv"]), _p(f["indinv"]),
                         _p(f["aqsm"]), _p(f["aqsn"]), _p(f["vcoul"]), _p(f["I_epsR"]), _p(f["I_epsA"]), _p(out))
     return out
 
 
-def ref_achDtemp_cor(f: dict) -> np.ndarray:
+def ref_achDtemp_cor(f: dict, kind: str = "serial") -> np.ndarray:
     out = np.zeros(f["nfreqeval"], np.complex128)
-    rlib().ref_achDtemp_cor(f["nb"], f["nvband"], f["nfreqeval"], f["ncouls"], f["ngpown"], f["nFreq"],
+    rlib(kind).ref_achDtemp_cor(f["nb"], f["nvband"], f["nfreqeval"], f["ncouls"], f["ngpown"], f["nFreq"],
                             f["freqevalmin"], f["freqevalstep"], _p(f["ekq"]), _p(f["dFreqGrid"]), _p(f["inv"]),
                             _p(f["indinv"]), _p(f["aqsm"]), _p(f["aqsn"]), _p(f["vcoul"]), _p(f["I_epsR"]),
                             _p(f["I_epsA"]), _p(out))
     return out
+
+
+def ref_last_seconds(kind: str = "serial") -> float:
+    """Wall time of the last reference kernel call of that build, timed inside the
+    adapter around the kernel alone (not the copies into the reference's containers)."""
+    return float(rlib(kind).ref_last_seconds())

@@ -9,6 +9,7 @@
 // reference functions.
 #include "common/Defines.h"
 
+#include <chrono>
 #include <cstring>
 
 // Prototypes of the reference FF kernels, whose definitions (external linkage)
@@ -38,9 +39,20 @@ template <typename A>
 void fill_c(A &arr, const double *src) {
   std::memcpy(static_cast<void *>(arr.dptr), src, arr.size * sizeof(*arr.dptr));
 }
+
+// wall time of the last reference kernel call alone (without the copies into ArrayMD)
+double g_last_seconds = 0.0;
+template <typename F>
+void timed(F &&f) {
+  const auto t0 = std::chrono::steady_clock::now();
+  f();
+  g_last_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+}
 }  // namespace
 
 extern "C" {
+
+double ref_last_seconds(void) { return g_last_seconds; }
 
 // noflagOCC_solver (GPP/gpp.cpp:307) on caller data; wx3[3] as the reference takes it.
 void ref_noflagOCC_solver(size_t nb, size_t ngpown, size_t ncouls, const int *inv_igp_index,
@@ -83,14 +95,18 @@ void ref_gpp_per_band(size_t nb, size_t ngpown, size_t ncouls, const int *inv_ig
   fill(vcoul_a, vcoul);
   fill(inv_a, inv_igp_index);
   fill(indinv_a, indinv);
-  for (size_t n1 = 0; n1 < nb; ++n1) {
-    fill_c(aqsmtemp, aqsm + 2 * n1 * ncouls);
-    fill_c(aqsntemp, aqsn + 2 * n1 * ncouls);
-    fill(wx_a, wx + 3 * n1);
-    noflagOCC_solver(1, ngpown, ncouls, inv_a, indinv_a, wx_a, wtilde_array, aqsmtemp,
-                     aqsntemp, I_eps_array, vcoul_a, achtemp_a);
-    std::memcpy(acht_n1_loc + 6 * n1, static_cast<void *>(achtemp_a.dptr), 6 * sizeof(double));
-  }
+  // timed: the band loop (its per-band row copies are ~1 MB against the 1.7 GB each
+  // band streams), not the copies of wtilde / I_eps above
+  timed([&] {
+    for (size_t n1 = 0; n1 < nb; ++n1) {
+      fill_c(aqsmtemp, aqsm + 2 * n1 * ncouls);
+      fill_c(aqsntemp, aqsn + 2 * n1 * ncouls);
+      fill(wx_a, wx + 3 * n1);
+      noflagOCC_solver(1, ngpown, ncouls, inv_a, indinv_a, wx_a, wtilde_array, aqsmtemp,
+                       aqsntemp, I_eps_array, vcoul_a, achtemp_a);
+      std::memcpy(acht_n1_loc + 6 * n1, static_cast<void *>(achtemp_a.dptr), 6 * sizeof(double));
+    }
+  });
 }
 
 void ref_achsDtemp(int nb, int ngpown, int ncouls, int nFreq, const int *inv_igp_index,
@@ -107,8 +123,10 @@ void ref_achsDtemp(int nb, int ngpown, int ncouls, int nFreq, const int *inv_igp
   fill_c(I_epsR_array, I_epsR);
   fill(vcoul_a, vcoul);
   ComplexType res(0.0, 0.0);
-  achsDtemp_Kernel(nb, ngpown, ncouls, inv_a, indinv_a, aqsntemp, aqsmtemp, I_epsR_array, vcoul_a,
-                   res);
+  timed([&] {
+    achsDtemp_Kernel(nb, ngpown, ncouls, inv_a, indinv_a, aqsntemp, aqsmtemp, I_epsR_array, vcoul_a,
+                     res);
+  });
   out[0] = res.real();
   out[1] = res.imag();
 }
@@ -132,8 +150,10 @@ void ref_asxDtemp(int nb, int nvband, int nfreqeval, int ncouls, int ngpown, int
   fill_c(aqsntemp, aqsn);
   fill_c(R, I_epsR);
   fill_c(A, I_epsA);
-  asxDtemp_Kernel(nvband, nfreqeval, ncouls, ngpown, nFreq, freqevalmin, freqevalstep, occ, ekq_a,
-                  grid_a, inv_a, indinv_a, aqsmtemp, aqsntemp, vcoul_a, R, A, res);
+  timed([&] {
+    asxDtemp_Kernel(nvband, nfreqeval, ncouls, ngpown, nFreq, freqevalmin, freqevalstep, occ, ekq_a,
+                    grid_a, inv_a, indinv_a, aqsmtemp, aqsntemp, vcoul_a, R, A, res);
+  });
   std::memcpy(out, static_cast<void *>(res.dptr), nfreqeval * 2 * sizeof(double));
 }
 
@@ -156,8 +176,10 @@ void ref_achDtemp_cor(int nb, int nvband, int nfreqeval, int ncouls, int ngpown,
   fill_c(aqsntemp, aqsn);
   fill_c(R, I_epsR);
   fill_c(A, I_epsA);
-  achDtemp_cor_Kernel(nb, nvband, nfreqeval, ncouls, ngpown, nFreq, freqevalmin, freqevalstep,
-                      ekq_a, grid_a, inv_a, indinv_a, aqsmtemp, aqsntemp, vcoul_a, R, A, res);
+  timed([&] {
+    achDtemp_cor_Kernel(nb, nvband, nfreqeval, ncouls, ngpown, nFreq, freqevalmin, freqevalstep,
+                        ekq_a, grid_a, inv_a, indinv_a, aqsmtemp, aqsntemp, vcoul_a, R, A, res);
+  });
   std::memcpy(out, static_cast<void *>(res.dptr), nfreqeval * 2 * sizeof(double));
 }
 

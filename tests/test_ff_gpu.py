@@ -106,3 +106,22 @@ def test_config4_shape_vs_oracle(ctx):
     assert abs(achs - O.achsDtemp(f)) <= TOL * abs(O.achsDtemp(f))
     assert rel_err(asx, O.asxDtemp(f)) < TOL
     assert rel_err(cor, O.achDtemp_cor(f)) < TOL
+
+
+@pytest.mark.skipif(not O.REF_OMP.exists(), reason="reference build (oracle/_ref) absent")
+def test_full_config4_vs_reference_build(ctx):
+    """The full config-4 inputs (nb 512, ncouls 16384, ngpown 819, nFreq 15, nfreqeval 10)
+    through the host entry points, against the reference's own FF kernels
+    (FF/ff.cpp:171-451, built from its sources into oracle/_ref) on the same inputs.
+    The reference sums naively in FP64, about 1e-13 relative from the exact sum here;
+    its asxDtemp / achDtemp_cor repeat one value for every frequency, so it runs them
+    with nfreqeval 1."""
+    f = bgw.ff_config4_inputs()
+    f1 = dict(f, nfreqeval=1)
+    achs, asx, cor = _all(ctx, f)
+    r_achs = O.ref_achsDtemp(f, kind="omp")
+    r_asx = O.ref_asxDtemp(f1, kind="omp")[0]
+    r_cor = O.ref_achDtemp_cor(f1, kind="omp")[0]
+    assert abs(achs - r_achs) <= 1e-11 * abs(r_achs)
+    assert np.all(np.abs(asx - r_asx) <= 1e-11 * abs(r_asx))
+    assert np.all(np.abs(cor - r_cor) <= 1e-11 * abs(r_cor))

@@ -187,3 +187,18 @@ def test_config1_full_properties(ctx):
                wx=d["wx"][:36].copy())
     o = O.gpp_forward(sub, want=("achtemp", "acht_n1_loc"))
     assert rel_err(a["acht_n1_loc"][:12], o["acht_n1_loc"]) < TOL
+
+
+@pytest.mark.slow
+@pytest.mark.skipif(not O.REF_OMP.exists(), reason="reference build (oracle/_ref) absent")
+def test_config1_full_vs_reference_build(ctx):
+    """The benchmark workload at full size (512 bands, ngpown 1638, ncouls 32768, wx per
+    band): every band's acht_n1_loc against the reference's own noflagOCC_solver
+    (GPP/gpp.cpp:307-402, built from its sources into oracle/_ref, OpenMP) called once
+    per band on the same inputs.  The reference sums 5.4e7 terms per band naively, about
+    1e-13 relative from the exact sum."""
+    d = bgw.gpp_config_inputs(1)
+    a = ctx.gpp_solver(d, outputs=("achtemp", "acht_n1_loc"))
+    r = O.ref_gpp_per_band(d, kind="omp")
+    assert np.max(np.abs(a["acht_n1_loc"] - r) / np.abs(r)) < 1e-11
+    assert rel_err(a["achtemp"], r.sum(0)) < 1e-11
