@@ -87,6 +87,19 @@ __global__ __launch_bounds__(256) void k_ff_plan(int op, int nb, int nvband, int
   }
 }
 
+// One band's term of achDtemp_cor (schDttt_corKernel1/2, FF/ff.cpp:381-451) as two
+// one-row jobs: band n1 on slice ifreq with weight w1 and on slice ifreq + 1 with w2.
+__global__ void k_cor_band_plan(int* __restrict__ rows, double2* __restrict__ wts, int* __restrict__ cnt,
+                                int* __restrict__ slice_of_job, int n1, int ifreq, double w1, double w2) {
+  if (threadIdx.x < 2) {
+    const int j = threadIdx.x;
+    rows[j] = n1;
+    wts[j] = make_double2(j == 0 ? w1 : w2, 0.0);
+    cnt[j] = 1;
+    slice_of_job[j] = ifreq + j;
+  }
+}
+
 }  // namespace
 
 int ff_device(bgw_ctx* ctx, int op, int nb, int nvband, int nfreqeval, int ncouls, int ngpown, int nFreq,
@@ -161,6 +174,48 @@ int ff_device(bgw_ctx* ctx, int op, int nb, int nvband, int nfreqeval, int ncoul
   if (rc) return rc;
   launch_replicate(a.out, nfreqeval, s);
   return check_launch(ctx, "replicate");
+}
+
+// schDttt_corKernel1 (scale 1) / schDttt_corKernel2 (scale -1/2) on device arrays:
+// sum_{p,g} aqsn(n1,g) conj(aqsm(n1,igp)) vcoul(igp) scale (D(ifreq) fact1 + D(ifreq+1) fact2),
+// D = I_epsR - I_epsA, on the MFMA contraction.
+int cor_band_device(bgw_ctx* ctx, double scale, int ncouls, int ngpown, int ifreq, int n1, double fact1,
+                    double fact2, const int* inv, const int* indinv, const double* I_epsR, const double* I_epsA,
+                    const double* aqsm, const double* aqsn, const double* vcoul, double* out) {
+  hipStream_t s = ctx->stream;
+  launch_check_indices(inv, (size_t)ngpown, indinv, (size_t)ncouls, ncouls - 1, ncouls, ctx->d_flag, s);
+  void *prow, *pw, *pcnt, *pslice;
+  int rc = pool_get(ctx, "ws_cb_rows", 2 * sizeof(int), &prow);
+  if (!rc) rc = pool_get(ctx, "ws_cb_wts", 2 * 16, &pw);
+  if (!rc) rc = pool_get(ctx, "ws_cb_cnt", 2 * sizeof(int), &pcnt);
+  if (!rc) rc = pool_get(ctx, "ws_cb_slice", 2 * sizeof(int), &pslice);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_cor_band_plan, dim3(1), dim3(64), 0, s, static_cast<int*>(prow), static_cast<double2*>(pw),
+                     static_cast<int*>(pcnt), static_cast<int*>(pslice), n1, ifreq, scale * fact1, scale * fact2);
+  if ((rc = check_launch(ctx, "k_cor_band_plan"))) return rc;
+  ContractArgs a{};
+  a.A = reinterpret_cast<const double2*>(aqsn);
+  a.K = ncouls;
+  a.N = ngpown;
+  a.slice_stride = (long)ngpown * ncouls;
+  a.Xa = reinterpret_cast<const double2*>(I_epsR);
+  a.Y = reinterpret_cast<const double2*>(I_epsA);
+  a.jobs_a = 2;
+  a.slice_of_job = static_cast<const int*>(pslice);
+  a.rows = static_cast<const int*>(prow);
+  a.wts = static_cast<const double2*>(pw);
+  a.cnt = static_cast<const int*>(pcnt);
+  a.max_rows = 1;
+  a.inv = inv;
+  a.indinv = indinv;
+  a.gpp_index = 0;
+  a.aqsm = reinterpret_cast<const double2*>(aqsm);
+  a.vcoul = vcoul;
+  a.ncouls = ncouls;
+  a.thr2 = 0.0;
+  a.out = reinterpret_cast<double2*>(out);
+  const int bm = contract_row_tile(ctx->precision == BGW_FP32_BULK, true, 1);
+  return contract_launch(ctx, a, 2, bm, 2, true, 2, "contract_ff_corband");
 }
 
 }  // namespace bgw
@@ -278,6 +333,56 @@ int ff_host(bgw_ctx* ctx, int op, int nb, int nvband, int nfreqeval, int ncouls,
   return bgw_synchronize(ctx);
 }
 
+// schDttt_corKernel1/2 arguments: sizes as FF (ngpown <= ncouls, FF/ff.cpp:492-499),
+// ifreq >= 0 and n1 >= 0 (the arrays must hold slices ifreq, ifreq + 1 and row n1, as
+// the reference reads them), and the element count of the slices read within size_t.
+int cor_band_validate(bgw_ctx* ctx, int ncouls, int ngpown, int ifreq, int n1,
+                      std::initializer_list<const void*> ptrs) {
+  if (!ctx) return BGW_EINVAL;
+  for (const void* p : ptrs)
+    if (!p) return fail(ctx, BGW_EINVAL, "null array argument");
+  if (ncouls <= 0 || ngpown <= 0) return fail(ctx, BGW_EINVAL, "zero or negative size");
+  if (ngpown > ncouls) return fail(ctx, BGW_EINVAL, "ngpown cannot be greater than ncouls");
+  if (ifreq < 0 || n1 < 0) return fail(ctx, BGW_EINVAL, "negative ifreq or n1");
+  size_t t;
+  if (!mul_ok((size_t)ngpown, (size_t)ncouls, &t) || !mul_ok(t, (size_t)ifreq + 2, &t) || t > SIZE_MAX / 16 ||
+      !mul_ok((size_t)n1 + 1, (size_t)ncouls, &t) || t > SIZE_MAX / 16)
+    return fail(ctx, BGW_EINVAL, "size overflow");
+  if (hipSetDevice(ctx->device) != hipSuccess) return fail(ctx, BGW_ENODEV, "hipSetDevice");
+  return BGW_OK;
+}
+
+// Host arrays: transfers slices ifreq and ifreq + 1 of I_epsR and I_epsA, row n1 of
+// aqsmtemp / aqsntemp and the index arrays, then runs cor_band_device on them.
+int cor_band_host(bgw_ctx* ctx, double scale, int ncouls, int ngpown, int ifreq, int n1, double fact1, double fact2,
+                  const int* inv, const int* indinv, const double* R, const double* A, const double* aqsm,
+                  const double* aqsn, const double* vcoul, double* out) {
+  int rc = ff_host_indices(ctx, ngpown, ncouls, inv, indinv);
+  if (rc) return rc;
+  const size_t npg = (size_t)ngpown * ncouls;
+  const int *d_inv, *d_indinv;
+  const double *d_am, *d_an, *d_vc, *d_R, *d_A;
+  if ((rc = upload(ctx, "h_inv", inv, (size_t)ngpown, &d_inv))) return rc;
+  if ((rc = upload(ctx, "h_indinv", indinv, (size_t)ncouls, &d_indinv))) return rc;
+  if ((rc = upload(ctx, "h_aqsm", aqsm + 2 * (size_t)n1 * ncouls, 2 * (size_t)ncouls, &d_am))) return rc;
+  if ((rc = upload(ctx, "h_aqsn", aqsn + 2 * (size_t)n1 * ncouls, 2 * (size_t)ncouls, &d_an))) return rc;
+  if ((rc = upload(ctx, "h_vcoul", vcoul, (size_t)ncouls, &d_vc))) return rc;
+  if ((rc = upload(ctx, "h_epsR", R + 2 * npg * (size_t)ifreq, 4 * npg, &d_R))) return rc;
+  if ((rc = upload(ctx, "h_epsA", A + 2 * npg * (size_t)ifreq, 4 * npg, &d_A))) return rc;
+  void* d_out;
+  if ((rc = pool_get(ctx, "h_ffout", 16, &d_out))) return rc;
+  // the staged arrays hold row n1 as row 0 and slices ifreq, ifreq + 1 as slices 0, 1
+  rc = cor_band_device(ctx, scale, ncouls, ngpown, 0, 0, fact1, fact2, d_inv, d_indinv, d_R, d_A, d_am, d_an, d_vc,
+                       static_cast<double*>(d_out));
+  if (rc) {
+    (void)hipStreamSynchronize(ctx->stream);
+    return rc;
+  }
+  hipError_t e = hipMemcpyAsync(out, d_out, 16, hipMemcpyDeviceToHost, ctx->stream);
+  if (e != hipSuccess) return hip_fail(ctx, e, "download");
+  return bgw_synchronize(ctx);
+}
+
 }  // namespace
 
 extern "C" {
@@ -373,6 +478,54 @@ int bgw_achDtemp_cor_Kernel_device(bgw_ctx* ctx, int number_bands, int nvband, i
   return ff_device(ctx, 2, number_bands, nvband, nfreqeval, ncouls, ngpown, nFreq, freqevalmin, freqevalstep, 0.0,
                    ekq, dFreqGrid, inv_igp_index, indinv, aqsmtemp, aqsntemp, vcoul, I_epsR_array, I_epsA_array,
                    achDtemp_cor);
+}
+
+int bgw_schDttt_corKernel1(bgw_ctx* ctx, double* schDttt_cor, const int* inv_igp_index, const int* indinv,
+                           const double* I_epsR_array, const double* I_epsA_array, const double* aqsmtemp,
+                           const double* aqsntemp, const double* vcoul, int ncouls, int ifreq, int ngpown, int n1,
+                           double fact1, double fact2) {
+  int rc = cor_band_validate(ctx, ncouls, ngpown, ifreq, n1,
+                             {schDttt_cor, inv_igp_index, indinv, I_epsR_array, I_epsA_array, aqsmtemp, aqsntemp,
+                              vcoul});
+  if (rc) return rc;
+  return cor_band_host(ctx, 1.0, ncouls, ngpown, ifreq, n1, fact1, fact2, inv_igp_index, indinv, I_epsR_array,
+                       I_epsA_array, aqsmtemp, aqsntemp, vcoul, schDttt_cor);
+}
+
+int bgw_schDttt_corKernel2(bgw_ctx* ctx, double* schDttt_cor, const int* inv_igp_index, const int* indinv,
+                           const double* I_epsR_array, const double* I_epsA_array, const double* aqsmtemp,
+                           const double* aqsntemp, const double* vcoul, int ncouls, int ifreq, int ngpown, int n1,
+                           double fact1, double fact2) {
+  int rc = cor_band_validate(ctx, ncouls, ngpown, ifreq, n1,
+                             {schDttt_cor, inv_igp_index, indinv, I_epsR_array, I_epsA_array, aqsmtemp, aqsntemp,
+                              vcoul});
+  if (rc) return rc;
+  return cor_band_host(ctx, -0.5, ncouls, ngpown, ifreq, n1, fact1, fact2, inv_igp_index, indinv, I_epsR_array,
+                       I_epsA_array, aqsmtemp, aqsntemp, vcoul, schDttt_cor);
+}
+
+int bgw_schDttt_corKernel1_device(bgw_ctx* ctx, double* schDttt_cor, const int* inv_igp_index, const int* indinv,
+                                  const double* I_epsR_array, const double* I_epsA_array, const double* aqsmtemp,
+                                  const double* aqsntemp, const double* vcoul, int ncouls, int ifreq, int ngpown,
+                                  int n1, double fact1, double fact2) {
+  int rc = cor_band_validate(ctx, ncouls, ngpown, ifreq, n1,
+                             {schDttt_cor, inv_igp_index, indinv, I_epsR_array, I_epsA_array, aqsmtemp, aqsntemp,
+                              vcoul});
+  if (rc) return rc;
+  return cor_band_device(ctx, 1.0, ncouls, ngpown, ifreq, n1, fact1, fact2, inv_igp_index, indinv, I_epsR_array,
+                         I_epsA_array, aqsmtemp, aqsntemp, vcoul, schDttt_cor);
+}
+
+int bgw_schDttt_corKernel2_device(bgw_ctx* ctx, double* schDttt_cor, const int* inv_igp_index, const int* indinv,
+                                  const double* I_epsR_array, const double* I_epsA_array, const double* aqsmtemp,
+                                  const double* aqsntemp, const double* vcoul, int ncouls, int ifreq, int ngpown,
+                                  int n1, double fact1, double fact2) {
+  int rc = cor_band_validate(ctx, ncouls, ngpown, ifreq, n1,
+                             {schDttt_cor, inv_igp_index, indinv, I_epsR_array, I_epsA_array, aqsmtemp, aqsntemp,
+                              vcoul});
+  if (rc) return rc;
+  return cor_band_device(ctx, -0.5, ncouls, ngpown, ifreq, n1, fact1, fact2, inv_igp_index, indinv, I_epsR_array,
+                         I_epsA_array, aqsmtemp, aqsntemp, vcoul, schDttt_cor);
 }
 
 }  // extern "C"

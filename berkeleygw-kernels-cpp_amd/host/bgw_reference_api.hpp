@@ -5,9 +5,11 @@
 //   achsDtemp_Kernel      FF/ff.cpp:171-175
 //   asxDtemp_Kernel       FF/ff.cpp:211-217
 //   achDtemp_cor_Kernel   FF/ff.cpp:295-302
+//   schDttt_corKernel1/2  common/Defines.h:36-48, FF/ff.cpp:381-451
 // Array arguments are any ArrayMD-like container exposing `dptr` (row-major,
 // std::complex<double> or a layout-compatible complex type), so the reference's
-// own ArrayMD<..., Device::cpu> objects can be passed unchanged.  The calls are
+// own ArrayMD<..., Device::cpu> objects can be passed unchanged.  Include it after the
+// reference's common/Defines.h (see the end of this file).  The calls are
 // synchronous like the reference; failures throw bgw::Error (the reference has no
 // error channel).  One process-wide context on device 0 is created on first use.
 #pragma once
@@ -87,3 +89,61 @@ void achDtemp_cor_Kernel(int number_bands, int nvband, int nfreqeval, int ncouls
                                      indinv.dptr, bgw::cptr(aqsmtemp), bgw::cptr(aqsntemp), vcoul.dptr,
                                      bgw::cptr(I_epsR_array), bgw::cptr(I_epsA_array), bgw::mptr(achDtemp_cor)));
 }
+
+// schDttt_corKernel1 / schDttt_corKernel2: one band's term of achDtemp_cor (FF/ff.cpp:381-451).
+template <class C, class AInt, class ACplx3, class ACplx2, class AReal>
+void schDttt_corKernel1(C& schDttt_cor, AInt& inv_igp_index, AInt& indinv, ACplx3& I_epsR_array,
+                        ACplx3& I_epsA_array, ACplx2& aqsmtemp, ACplx2& aqsntemp, AReal& vcoul, int ncouls,
+                        int ifreq, int ngpown, int n1, double fact1, double fact2) {
+  bgw::check(bgw_schDttt_corKernel1(bgw::default_context(), reinterpret_cast<double*>(&schDttt_cor),
+                                    inv_igp_index.dptr, indinv.dptr, bgw::cptr(I_epsR_array), bgw::cptr(I_epsA_array),
+                                    bgw::cptr(aqsmtemp), bgw::cptr(aqsntemp), vcoul.dptr, ncouls, ifreq, ngpown, n1,
+                                    fact1, fact2));
+}
+
+template <class C, class AInt, class ACplx3, class ACplx2, class AReal>
+void schDttt_corKernel2(C& schDttt_cor, AInt& inv_igp_index, AInt& indinv, ACplx3& I_epsR_array,
+                        ACplx3& I_epsA_array, ACplx2& aqsmtemp, ACplx2& aqsntemp, AReal& vcoul, int ncouls,
+                        int ifreq, int ngpown, int n1, double fact1, double fact2) {
+  bgw::check(bgw_schDttt_corKernel2(bgw::default_context(), reinterpret_cast<double*>(&schDttt_cor),
+                                    inv_igp_index.dptr, indinv.dptr, bgw::cptr(I_epsR_array), bgw::cptr(I_epsA_array),
+                                    bgw::cptr(aqsmtemp), bgw::cptr(aqsntemp), vcoul.dptr, ncouls, ifreq, ngpown, n1,
+                                    fact1, fact2));
+}
+
+// The reference's common/Defines.h declares noflagOCC_solver and schDttt_corKernel1/2 as
+// plain functions (common/Defines.h:36-62).  A call prefers such a declaration over the
+// templates above, so, included after that header, this file also defines exactly those
+// declarations, and the reference's call sites link against libbgw once their CPU bodies
+// are removed.
+#if defined(ARRAY1D_int) && defined(ARRAY1D_DataType) && defined(ARRAY1D) && defined(ARRAY2D) && defined(ARRAY3D)
+inline void noflagOCC_solver(size_t number_bands, size_t ngpown, size_t ncouls, ARRAY1D_int& inv_igp_index,
+                             ARRAY1D_int& indinv, ARRAY1D_DataType& wx_array, ARRAY2D& wtilde_array,
+                             ARRAY2D& aqsmtemp, ARRAY2D& aqsntemp, ARRAY2D& I_eps_array, ARRAY1D_DataType& vcoul,
+                             ARRAY1D& achtemp) {
+  bgw::check(bgw_noflagOCC_solver(bgw::default_context(), number_bands, ngpown, ncouls, inv_igp_index.dptr,
+                                  indinv.dptr, wx_array.dptr, bgw::cptr(wtilde_array), bgw::cptr(aqsmtemp),
+                                  bgw::cptr(aqsntemp), bgw::cptr(I_eps_array), vcoul.dptr, bgw::mptr(achtemp)));
+}
+
+inline void schDttt_corKernel1(ComplexType& schDttt_cor, ARRAY1D_int& inv_igp_index, ARRAY1D_int& indinv,
+                               ARRAY3D& I_epsR_array, ARRAY3D& I_epsA_array, ARRAY2D& aqsmtemp, ARRAY2D& aqsntemp,
+                               ARRAY1D_DataType& vcoul, int ncouls, int ifreq, int ngpown, int n1, double fact1,
+                               double fact2) {
+  bgw::check(bgw_schDttt_corKernel1(bgw::default_context(), reinterpret_cast<double*>(&schDttt_cor),
+                                    inv_igp_index.dptr, indinv.dptr, bgw::cptr(I_epsR_array), bgw::cptr(I_epsA_array),
+                                    bgw::cptr(aqsmtemp), bgw::cptr(aqsntemp), vcoul.dptr, ncouls, ifreq, ngpown, n1,
+                                    fact1, fact2));
+}
+
+inline void schDttt_corKernel2(ComplexType& schDttt_cor, ARRAY1D_int& inv_igp_index, ARRAY1D_int& indinv,
+                               ARRAY3D& I_epsR_array, ARRAY3D& I_epsA_array, ARRAY2D& aqsmtemp, ARRAY2D& aqsntemp,
+                               ARRAY1D_DataType& vcoul, int ncouls, int ifreq, int ngpown, int n1, double fact1,
+                               double fact2) {
+  bgw::check(bgw_schDttt_corKernel2(bgw::default_context(), reinterpret_cast<double*>(&schDttt_cor),
+                                    inv_igp_index.dptr, indinv.dptr, bgw::cptr(I_epsR_array), bgw::cptr(I_epsA_array),
+                                    bgw::cptr(aqsmtemp), bgw::cptr(aqsntemp), vcoul.dptr, ncouls, ifreq, ngpown, n1,
+                                    fact1, fact2));
+}
+#endif
+

@@ -100,6 +100,10 @@ def _declare(L: C.CDLL) -> None:
     cor = [_P, _I, _I, _I, _I, _I, _I, _D, _D, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P]
     L.bgw_achDtemp_cor_Kernel.argtypes = cor
     L.bgw_achDtemp_cor_Kernel_device.argtypes = cor
+    band = [_P, _P, _P, _P, _P, _P, _P, _P, _P, _I, _I, _I, _I, _D, _D]
+    for k in ("1", "2"):
+        getattr(L, f"bgw_schDttt_corKernel{k}").argtypes = band
+        getattr(L, f"bgw_schDttt_corKernel{k}_device").argtypes = band
     L.bgw_synth_uniform.argtypes = [C.c_ulonglong, _D, _D, _P, _SZ]
     L.bgw_synth_uniform.restype = None
     L.bgw_synth_complex.argtypes = [C.c_ulonglong, _D, _D, _D, _D, _P, _SZ]
@@ -525,6 +529,22 @@ class Context:
             _ptr(_cplx(f["aqsm"])), _ptr(_cplx(f["aqsn"])), _ptr(f["vcoul"]), _ptr(_cplx(f["I_epsR"])),
             _ptr(_cplx(f["I_epsA"])), _ptr(out)))
         return out
+
+    def schDttt_corKernel(self, kernel: int, f: dict, n1: int, ifreq: int, fact1: float, fact2: float) -> complex:
+        """schDttt_corKernel1 (kernel=1) or schDttt_corKernel2 (kernel=2) on host arrays:
+        band n1's term of achDtemp_cor on slices ifreq, ifreq + 1."""
+        out = np.zeros(1, np.complex128)
+        fn = getattr(lib(), f"bgw_schDttt_corKernel{int(kernel)}")
+        self._check(fn(self._h, _ptr(out), _ptr(f["inv"]), _ptr(f["indinv"]), _ptr(_cplx(f["I_epsR"])),
+                       _ptr(_cplx(f["I_epsA"])), _ptr(_cplx(f["aqsm"])), _ptr(_cplx(f["aqsn"])), _ptr(f["vcoul"]),
+                       f["ncouls"], ifreq, f["ngpown"], n1, fact1, fact2))
+        return out[0]
+
+    def schDttt_corKernel_device(self, kernel: int, f: dict, ptrs: dict, n1: int, ifreq: int, fact1: float,
+                                 fact2: float, out_ptr: int):
+        fn = getattr(lib(), f"bgw_schDttt_corKernel{int(kernel)}_device")
+        self._check(fn(self._h, out_ptr, ptrs["inv"], ptrs["indinv"], ptrs["I_epsR"], ptrs["I_epsA"], ptrs["aqsm"],
+                       ptrs["aqsn"], ptrs["vcoul"], f["ncouls"], ifreq, f["ngpown"], n1, fact1, fact2))
 
     def ff_device(self, op: str, f: dict, ptrs: dict, out_ptr: int):
         L = lib()

@@ -6,6 +6,7 @@
  *   FF   achsDtemp_Kernel            FF/ff.cpp:171-209
  *        asxDtemp_Kernel             FF/ff.cpp:211-293
  *        achDtemp_cor_Kernel         FF/ff.cpp:295-379     (+ schDttt_corKernel1/2 :381-451)
+ *        schDttt_corKernel1/2        FF/ff.cpp:381-451     (decl common/Defines.h:36-48)
  *        compute_fact                FF/ff.cpp:103-152
  *
  * Conventions
@@ -189,6 +190,37 @@ int bgw_achDtemp_cor_Kernel_device(bgw_ctx* ctx, int number_bands, int nvband, i
                                    const double* aqsmtemp, const double* aqsntemp,
                                    const double* vcoul, const double* I_epsR_array,
                                    const double* I_epsA_array, double* achDtemp_cor);
+
+/* Replace schDttt_corKernel1 (FF/ff.cpp:381-416) and schDttt_corKernel2 (:418-451),
+ * declared in common/Defines.h:36-48, with the reference's argument order:
+ *   schDttt_cor = s * sum_{p<ngpown} sum_{g<ncouls} aqsn(n1,g) conj(aqsm(n1,igp)) vcoul(igp)
+ *                 * ((R-A)(ifreq,p,g) fact1 + (R-A)(ifreq+1,p,g) fact2)
+ * with s = 1 (Kernel1) or -1/2 (Kernel2), R/A = I_eps{R,A}_array c128[..][ngpown][ncouls],
+ * igp = indinv[inv_igp_index[p]], indinv[ncouls].  schDttt_cor is c128[1].  As in the
+ * reference, the arrays must hold slices ifreq and ifreq + 1 and band row n1; ifreq < 0,
+ * n1 < 0 or ngpown > ncouls return BGW_EINVAL.  The host forms transfer only those two
+ * slices of each array and that row. */
+int bgw_schDttt_corKernel1(bgw_ctx* ctx, double* schDttt_cor, const int* inv_igp_index,
+                           const int* indinv, const double* I_epsR_array,
+                           const double* I_epsA_array, const double* aqsmtemp,
+                           const double* aqsntemp, const double* vcoul, int ncouls, int ifreq,
+                           int ngpown, int n1, double fact1, double fact2);
+int bgw_schDttt_corKernel2(bgw_ctx* ctx, double* schDttt_cor, const int* inv_igp_index,
+                           const int* indinv, const double* I_epsR_array,
+                           const double* I_epsA_array, const double* aqsmtemp,
+                           const double* aqsntemp, const double* vcoul, int ncouls, int ifreq,
+                           int ngpown, int n1, double fact1, double fact2);
+/* Device-pointer forms (all arrays on the device, result after bgw_synchronize). */
+int bgw_schDttt_corKernel1_device(bgw_ctx* ctx, double* schDttt_cor, const int* inv_igp_index,
+                                  const int* indinv, const double* I_epsR_array,
+                                  const double* I_epsA_array, const double* aqsmtemp,
+                                  const double* aqsntemp, const double* vcoul, int ncouls,
+                                  int ifreq, int ngpown, int n1, double fact1, double fact2);
+int bgw_schDttt_corKernel2_device(bgw_ctx* ctx, double* schDttt_cor, const int* inv_igp_index,
+                                  const int* indinv, const double* I_epsR_array,
+                                  const double* I_epsA_array, const double* aqsmtemp,
+                                  const double* aqsntemp, const double* vcoul, int ncouls,
+                                  int ifreq, int ngpown, int n1, double fact1, double fact2);
 
 /* ---- seeded synthetic inputs (host only) ---------------------------------
  * SplitMix64 counter stream: x_i = mix64(seed + (i+1)*0x9E3779B97F4A7C15),

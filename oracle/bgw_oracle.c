@@ -774,3 +774,39 @@ void oracle_achDtemp_cor(int nb, int nvband, int nfreqeval, int ncouls, int ngpo
   free(w1);
   free(w2);
 }
+
+void oracle_schDttt_cor(int kernel, int ncouls, int ngpown, int ifreq, int n1, double fact1,
+                        double fact2, const int* inv_igp_index, const int* indinv,
+                        const double* I_epsR, const double* I_epsA, const double* aqsm,
+                        const double* aqsn, const double* vcoul, double* schDttt_cor,
+                        int nthreads) {
+  /* FF/ff.cpp:381-451: sum over my_igp and ig of aqsn(n1,ig) conj(aqsm(n1,igp)) vcoul(igp)
+   * times (R-A)(ifreq)*fact1 + (R-A)(ifreq+1)*fact2, times -1/2 in Kernel2 (:443).  The
+   * G-vector sum per column is ff_row_dot (compensated), the column sum double-double. */
+  const int nt = nthreads_of(nthreads);
+  cdd_t* part = (cdd_t*)calloc((size_t)nt, sizeof(cdd_t));
+#pragma omp parallel num_threads(nt)
+  {
+#ifdef _OPENMP
+    const int tid = omp_get_thread_num();
+#else
+    const int tid = 0;
+#endif
+#pragma omp for schedule(static)
+    for (long lp = 0; lp < (long)ngpown; ++lp) {
+      const int igp = igp_of(inv_igp_index, indinv, (size_t)lp);
+      cdd_t M = ff_row_dot(aqsn + 2 * (size_t)n1 * ncouls, I_epsR, I_epsA, ifreq, fact1, fact2, (size_t)lp,
+                           (size_t)ngpown, (size_t)ncouls);
+      const double complex am = aqsm[2 * ((size_t)n1 * ncouls + igp)] + I * aqsm[2 * ((size_t)n1 * ncouls + igp) + 1];
+      double complex c = conj(am) * vcoul[igp];
+      if (kernel == 2) c = c * -0.5;
+      cdd_add_cprod_dd(&part[tid], creal(c), cimag(c), M);
+    }
+  }
+  cdd_t a;
+  memset(&a, 0, sizeof(a));
+  for (int t = 0; t < nt; ++t) cdd_add(&a, part[t]);
+  schDttt_cor[0] = dd_val(a.re);
+  schDttt_cor[1] = dd_val(a.im);
+  free(part);
+}

@@ -68,6 +68,13 @@ void oracle_achDtemp_cor(int nb, int nvband, int nfreqeval, int ncouls, int ngpo
                          const double* aqsm, const double* aqsn, const double* vcoul,
                          const double* I_epsR, const double* I_epsA, double* achDtemp_cor,
                          int nthreads);
+/* schDttt_corKernel1 (kernel 1, FF/ff.cpp:381-416) / schDttt_corKernel2 (kernel 2,
+ * :418-451, factor -1/2): band n1 on slices ifreq, ifreq+1 of I_epsR - I_epsA. */
+void oracle_schDttt_cor(int kernel, int ncouls, int ngpown, int ifreq, int n1, double fact1,
+                        double fact2, const int* inv_igp_index, const int* indinv,
+                        const double* I_epsR, const double* I_epsA, const double* aqsm,
+                        const double* aqsn, const double* vcoul, double* schDttt_cor,
+                        int nthreads);
 
 #ifdef __cplusplus
 }

@@ -52,6 +52,8 @@ def olib() -> C.CDLL:
         L.oracle_asxDtemp.restype = None
         L.oracle_achDtemp_cor.argtypes = [_I] + ff + [_P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _I]
         L.oracle_achDtemp_cor.restype = None
+        L.oracle_schDttt_cor.argtypes = [_I, _I, _I, _I, _I, _D, _D] + [_P] * 8 + [_I]
+        L.oracle_schDttt_cor.restype = None
         _o = L
     return _o
 
@@ -159,6 +161,16 @@ def achDtemp_cor(f: dict, nthreads: int | None = None) -> np.ndarray:
                                _p(f["indinv"]), _p(f["aqsm"]), _p(f["aqsn"]), _p(f["vcoul"]), _p(f["I_epsR"]),
                                _p(f["I_epsA"]), _p(out), nthreads_default() if nthreads is None else nthreads)
     return out
+
+
+def schDttt_cor(f: dict, kernel: int, n1: int, ifreq: int, fact1: float, fact2: float,
+                nthreads: int | None = None) -> complex:
+    """schDttt_corKernel1 (kernel=1) / schDttt_corKernel2 (kernel=2), FF/ff.cpp:381-451."""
+    out = np.zeros(1, np.complex128)
+    olib().oracle_schDttt_cor(int(kernel), f["ncouls"], f["ngpown"], ifreq, n1, fact1, fact2, _p(f["inv"]),
+                              _p(f["indinv"]), _p(f["I_epsR"]), _p(f["I_epsA"]), _p(f["aqsm"]), _p(f["aqsn"]),
+                              _p(f["vcoul"]), _p(out), nthreads_default() if nthreads is None else nthreads)
+    return out[0]
 
 
 # -------------------------------------------------------- the reference itself

@@ -72,3 +72,31 @@ def ff_reference_closed_form(nb, nvband, ncouls, ngpown, nFreq):
     k2 = np.sum(~(wx > 0) & occ)
     cor = ngpown * ncouls * (k1 * 0.5 * (-0.5) * 1j + k2 * 0.5 * (-0.5) * 1j)
     return achs, asx, cor
+
+
+def ff_with_kernel2_band(f):
+    """The same FF inputs with occupied band 0 at wx = -0.4 (ekq 0.9), so that
+    achDtemp_cor has a schDttt_corKernel2 term (FF/ff.cpp:347-350)."""
+    f = dict(f)
+    f["ekq"] = f["ekq"].copy()
+    f["ekq"][0] = 0.9
+    return f
+
+
+def ff_band_terms(f):
+    """(kernel, n1, ifreq, fact1, fact2) of every band achDtemp_cor_Kernel sums
+    (FF/ff.cpp:337-352): Kernel1 for wx > 0 unoccupied, Kernel2 for wx <= 0 occupied,
+    with compute_fact's loop-2 interval and weights from the oracle."""
+    import oracle as O
+
+    out = []
+    for n1 in range(f["nb"]):
+        wx = f["freqevalmin"] - f["ekq"][n1] + f["freqevalstep"]
+        occ = n1 < f["nvband"]
+        f1, f2, ifreq = O.compute_fact(wx, f["dFreqGrid"], 2, occ)
+        if wx > 0 and not occ:
+            out.append((1, n1, ifreq, f1, f2))
+        elif not wx > 0 and occ:
+            out.append((2, n1, ifreq, f1, f2))
+    return out
+

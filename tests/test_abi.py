@@ -96,3 +96,19 @@ def test_product_compute_fact_sweep_matches_oracle():
         for loop in (1, 2):
             for occ in (False, True):
                 assert bgw.compute_fact(wx, grid, loop, occ) == O.compute_fact(wx, grid, loop, occ)
+
+
+def test_reference_shim_links_after_the_reference_headers():
+    """host/bgw_reference_api.hpp included after the reference's common/Defines.h, which
+    declares noflagOCC_solver and schDttt_corKernel1/2 as plain functions (:36-62): every
+    kernel call of the reference API compiles and links against libbgw, in both of the
+    reference's complex modes (oracle/Makefile, target shim-check; nothing is run)."""
+    import subprocess
+    from pathlib import Path
+
+    odir = Path(O.__file__).resolve().parent
+    ref = subprocess.run(["make", "-s", "-C", str(odir), "print-ref"], capture_output=True, text=True).stdout.strip()
+    if not (Path(ref) / "common" / "Defines.h").exists():
+        pytest.skip("the reference tree is not present")
+    r = subprocess.run(["make", "-s", "-B", "-C", str(odir), "shim-check"], capture_output=True, text=True)
+    assert r.returncode == 0, r.stdout + r.stderr

@@ -9,7 +9,7 @@ import pytest
 import bgw
 import oracle as O
 from conftest import rel_err
-from helpers import GOLDEN, ff_reference_closed_form, load_exact_ff
+from helpers import GOLDEN, ff_band_terms, ff_reference_closed_form, ff_with_kernel2_band, load_exact_ff
 
 pytestmark = pytest.mark.gpu
 TOL = 1e-12
@@ -125,3 +125,58 @@ def test_full_config4_vs_reference_build(ctx):
     assert abs(achs - r_achs) <= 1e-11 * abs(r_achs)
     assert np.all(np.abs(asx - r_asx) <= 1e-11 * abs(r_asx))
     assert np.all(np.abs(cor - r_cor) <= 1e-11 * abs(r_cor))
+
+
+@pytest.mark.parametrize("shape", [(40, 3, 96, 24, 15, 2), (33, 5, 70, 9, 7, 4), (6, 2, 1000, 65, 4, 1)])
+def test_band_kernels_vs_oracle(ctx, shape):
+    """bgw_schDttt_corKernel1/2 (FF/ff.cpp:381-451) on host arrays against the oracle, on
+    the first and the last slice pair and band (G' counts 24, 9 and 65: partial tiles)."""
+    f = bgw.ff_inputs(*shape, 2027)
+    nb, nFreq = f["nb"], f["nFreq"]
+    for kernel in (1, 2):
+        for n1, ifreq, f1, f2 in ((0, 0, 0.3, 0.7), (nb - 1, nFreq - 2, -1.25, 0.5)):
+            got = ctx.schDttt_corKernel(kernel, f, n1, ifreq, f1, f2)
+            ref = O.schDttt_cor(f, kernel, n1, ifreq, f1, f2)
+            assert abs(got - ref) <= TOL * abs(ref), (kernel, n1, ifreq)
+
+
+def test_band_kernels_compose_achDtemp_cor(ctx):
+    """Summed over achDtemp_cor's bands (compute_fact's intervals and weights), the
+    per-band entry points give achDtemp_cor."""
+    f = ff_with_kernel2_band(bgw.ff_inputs(40, 3, 96, 24, 15, 2, 11))
+    terms = ff_band_terms(f)
+    assert {t[0] for t in terms} == {1, 2}
+    total = sum(ctx.schDttt_corKernel(k, f, n1, ifreq, f1, f2) for k, n1, ifreq, f1, f2 in terms)
+    cor = ctx.achDtemp_cor_Kernel(f)[0]
+    assert abs(total - cor) <= TOL * abs(cor)
+
+
+def test_band_kernels_device_form_and_validation(ctx):
+    import torch
+
+    f = bgw.ff_inputs(20, 2, 300, 30, 6, 1, 5)
+    keys = ("inv", "indinv", "aqsm", "aqsn", "vcoul", "I_epsR", "I_epsA")
+    T = {k: torch.from_numpy(np.ascontiguousarray(f[k])).to("cuda:0") for k in keys}
+    out = torch.zeros(1, dtype=torch.complex128, device="cuda:0")
+    torch.cuda.synchronize()
+    ptrs = {k: v.data_ptr() for k, v in T.items()}
+    for kernel in (1, 2):
+        h = ctx.schDttt_corKernel(kernel, f, 7, 3, 0.25, -0.75)
+        ctx.schDttt_corKernel_device(kernel, f, ptrs, 7, 3, 0.25, -0.75, out.data_ptr())
+        ctx.synchronize()
+        assert out.cpu().numpy()[0] == h  # same kernels on the same data: bit for bit
+    for bad in ({"ifreq": -1}, {"n1": -1}):
+        args = dict(n1=0, ifreq=0) | bad
+        with pytest.raises(bgw.BgwError) as ei:
+            ctx.schDttt_corKernel(1, f, args["n1"], args["ifreq"], 0.5, 0.5)
+        assert ei.value.code == bgw.BGW_EINVAL
+    g = dict(f, ngpown=f["ncouls"] + 1)
+    with pytest.raises(bgw.BgwError) as ei:
+        ctx.schDttt_corKernel(2, g, 0, 0, 0.5, 0.5)
+    assert ei.value.code == bgw.BGW_EINVAL
+    bad_inv = dict(f, inv=f["inv"].copy())
+    bad_inv["inv"][3] = f["ncouls"]
+    with pytest.raises(bgw.BgwError) as ei:
+        ctx.schDttt_corKernel(1, bad_inv, 0, 0, 0.5, 0.5)
+    assert ei.value.code == bgw.BGW_EINVAL
+

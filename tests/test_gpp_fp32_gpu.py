@@ -84,6 +84,14 @@ def test_ff_split_fp32_long_valence_lists(ctx32):
     assert rel_err(asx, O.asxDtemp(f)) > 1e-13
 
 
+def test_band_kernels_split_fp32(ctx32):
+    f = bgw.ff_inputs(40, 3, 500, 70, 15, 2, 1031)
+    for kernel in (1, 2):
+        got = ctx32.schDttt_corKernel(kernel, f, 5, 4, 0.6, 0.4)
+        ref = O.schDttt_cor(f, kernel, 5, 4, 0.6, 0.4)
+        assert TOL32 * abs(ref) >= abs(got - ref) > 1e-13 * abs(ref)
+
+
 @pytest.mark.slow
 def test_ff_config4_shape_split_fp32(ctx32):
     f = bgw.ff_config4_inputs(ncouls=2048)

@@ -15,8 +15,8 @@ import pytest
 import bgw
 import oracle as O
 from conftest import rel_err
-from helpers import (ff_reference_closed_form, gpp_reference_achstemp, gpp_reference_closed_form,
-                     load_exact_ff, load_exact_gpp)
+from helpers import (ff_band_terms, ff_reference_closed_form, ff_with_kernel2_band, gpp_reference_achstemp,
+                     gpp_reference_closed_form, load_exact_ff, load_exact_gpp)
 
 # ------------------------------------------------------------------------- GPP
 
@@ -274,3 +274,27 @@ def test_compute_fact_oracle_vs_exact(wx, loop, occ):
     # in/out semantics: nothing written for loop 2 unoccupied wx<=0
     if loop == 2 and not occ and wx <= 0:
         assert O.compute_fact(wx, GRID, loop, occ, ifreq=7, fact1=3.0, fact2=4.0) == (3.0, 4.0, 7)
+
+
+def test_schDttt_cor_oracle_composes_achDtemp_cor():
+    """achDtemp_cor is the sum of schDttt_corKernel1/2 over its bands: the per-band
+    oracle must reproduce the (reference-pinned) achDtemp_cor oracle."""
+    for seed, shape in ((11, (40, 3, 96, 24, 15, 2)), (12, (33, 5, 70, 9, 7, 4))):
+        f = ff_with_kernel2_band(bgw.ff_inputs(*shape, seed))
+        terms = ff_band_terms(f)
+        assert {t[0] for t in terms} == {1, 2}  # both kernels take part
+        total = sum(O.schDttt_cor(f, *t) for t in terms)
+        ref = O.achDtemp_cor(f)[0]
+        assert abs(total - ref) <= 1e-13 * abs(ref)
+
+
+@pytest.mark.parametrize("kernel,scale", [(1, 1.0), (2, -0.5)])
+def test_schDttt_cor_oracle_closed_form_reference_constants(kernel, scale):
+    # FF/ff.cpp:584-633 constants: aqsn conj(aqsm) = 1/2, vcoul = 1, I_epsR - I_epsA = i in
+    # every slice, so the sum is ngpown ncouls (1/2) i (fact1 + fact2) times the kernel's scale
+    nb, ngpown, ncouls, nFreq = 9, 7, 50, 6
+    f = bgw.ff_inputs(nb, 2, ncouls, ngpown, nFreq, 2, 0, kind="reference")
+    fact1, fact2 = 0.375, -1.25
+    want = ngpown * ncouls * 0.5j * (fact1 + fact2) * scale
+    got = O.schDttt_cor(f, kernel, nb - 1, nFreq - 2, fact1, fact2)
+    assert abs(got - want) <= 1e-15 * abs(want)
