@@ -55,6 +55,9 @@ struct bgw_ctx {
   // copy stream of the overlapped host-pointer GPP path and its hand-over event
   hipStream_t copy_stream = nullptr;
   hipEvent_t xfer_event = nullptr;
+  // host buffers of the gathered aqsm / aqsn columns: a copy from them may still be
+  // queued when the staging function returns, so they live until the next call
+  std::vector<double2> gather_m, gather_n;
 };
 
 namespace bgw {

@@ -704,8 +704,9 @@ int gpp_forward_staged(bgw_ctx* ctx, int nb, int nvband, int ngpown, int ncouls,
   // aqsm/aqsn: whole, when asxtemp or achstemp read them; otherwise only the columns
   // igp(p) that sch_store1 reads, gathered on the host as [nb][ngpown]
   const int gathered = !d_achs && !d_asx;
-  std::vector<double2> gm, gn;
   if (gathered) {
+    std::vector<double2>& gm = ctx->gather_m;
+    std::vector<double2>& gn = ctx->gather_n;
     gm.resize((size_t)nb * ngpown);
     gn.resize((size_t)nb * ngpown);
     const double2* hm = reinterpret_cast<const double2*>(h.aqsm);
