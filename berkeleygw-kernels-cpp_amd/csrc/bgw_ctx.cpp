@@ -96,6 +96,17 @@ void parallel_copy(void* dst, const void* src, size_t n) {
   }
 }
 
+// dst = a - b elementwise over n doubles, split over the host threads
+void parallel_sub(double* dst, const double* a, const double* b, size_t n) {
+  const int nt = std::max(1, std::min(16, omp_get_max_threads()));
+  const size_t piece = (n + nt - 1) / nt;
+#pragma omp parallel for num_threads(nt) schedule(static)
+  for (int t = 0; t < nt; ++t) {
+    const size_t lo = std::min(n, (size_t)t * piece), hi = std::min(n, lo + piece);
+    for (size_t i = lo; i < hi; ++i) dst[i] = a[i] - b[i];
+  }
+}
+
 // Makes the ring usable; false if pinned memory is unavailable (copies then go direct).
 bool ring_ready(bgw_ctx* ctx) {
   if (!ctx->ring.empty()) return true;
@@ -145,6 +156,30 @@ int upload_host(bgw_ctx* ctx, void* dst, const void* src, size_t bytes, hipStrea
     hipError_t e = hipMemcpyAsync(static_cast<char*>(dst) + off, s->host, n, hipMemcpyHostToDevice, stream);
     if (e == hipSuccess) e = hipEventRecord(s->done, stream);
     if (e != hipSuccess) return hip_fail(ctx, e, "staged upload");
+    s->busy = true;
+  }
+  return BGW_OK;
+}
+
+int upload_host_diff(bgw_ctx* ctx, double* dst, const double* a, const double* b, size_t n) {
+  if (!ring_ready(ctx)) {
+    // no pinned memory: form the difference in pageable memory, copy it, and wait
+    std::vector<double> tmp(n);
+    parallel_sub(tmp.data(), a, b, n);
+    hipError_t e = hipMemcpyAsync(dst, tmp.data(), n * sizeof(double), hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    return e == hipSuccess ? BGW_OK : hip_fail(ctx, e, "difference upload");
+  }
+  const size_t per = kSlotBytes / sizeof(double);
+  for (size_t off = 0; off < n; off += per) {
+    const size_t m = std::min(per, n - off);
+    PinnedSlot* s;
+    int rc = take_slot(ctx, &s);
+    if (rc) return rc;
+    parallel_sub(static_cast<double*>(s->host), a + off, b + off, m);
+    hipError_t e = hipMemcpyAsync(dst + off, s->host, m * sizeof(double), hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess) e = hipEventRecord(s->done, ctx->stream);
+    if (e != hipSuccess) return hip_fail(ctx, e, "staged difference upload");
     s->busy = true;
   }
   return BGW_OK;

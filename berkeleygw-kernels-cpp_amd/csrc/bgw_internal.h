@@ -90,6 +90,10 @@ bool mul_ok(size_t a, size_t b, size_t* out);
 int upload_host(bgw_ctx* ctx, void* dst, const void* src, size_t bytes);
 int upload_host(bgw_ctx* ctx, void* dst, const void* src, size_t bytes, hipStream_t s);
 int download_host(bgw_ctx* ctx, void* dst, const void* src, size_t bytes);
+// Uploads a - b (n doubles, elementwise, IEEE double subtraction) on ctx->stream: the
+// difference is formed by the host copy into the pinned ring, so only n doubles cross
+// the bus.  Returns once a and b have been consumed.
+int upload_host_diff(bgw_ctx* ctx, double* dst, const double* a, const double* b, size_t n);
 // Releases the pinned ring (the stream must be idle).
 void ring_release(bgw_ctx* ctx);
 // The context's copy stream (created on first use, default priority).

@@ -178,7 +178,7 @@ int ff_device(bgw_ctx* ctx, int op, int nb, int nvband, int nfreqeval, int ncoul
 
 // schDttt_corKernel1 (scale 1) / schDttt_corKernel2 (scale -1/2) on device arrays:
 // sum_{p,g} aqsn(n1,g) conj(aqsm(n1,igp)) vcoul(igp) scale (D(ifreq) fact1 + D(ifreq+1) fact2),
-// D = I_epsR - I_epsA, on the MFMA contraction.
+// D = I_epsR - I_epsA (or D = I_epsR when I_epsA is null), on the MFMA contraction.
 int cor_band_device(bgw_ctx* ctx, double scale, int ncouls, int ngpown, int ifreq, int n1, double fact1,
                     double fact2, const int* inv, const int* indinv, const double* I_epsR, const double* I_epsA,
                     const double* aqsm, const double* aqsn, const double* vcoul, double* out) {
@@ -308,16 +308,29 @@ int ff_host(bgw_ctx* ctx, int op, int nb, int nvband, int nfreqeval, int ncouls,
         if ((wx > 0 && !occd) || (!(wx > 0) && occd)) needR[ifreq] = needR[ifreq + 1] = needA[ifreq] = needA[ifreq + 1] = 1;
       }
     }
-    void *pR, *pA;
+    void *pR, *pA = nullptr;
     if ((rc = pool_get(ctx, "h_epsR", 2 * npg * nFreq * sizeof(double), &pR))) return rc;
-    if ((rc = pool_get(ctx, "h_epsA", 2 * npg * nFreq * sizeof(double), &pA))) return rc;
     const size_t slice = npg * 16;
-    for (int f = 0; f < nFreq; ++f) {
-      if (needR[f] && (rc = upload_host(ctx, static_cast<char*>(pR) + f * slice, R + 2 * npg * f, slice))) return rc;
-      if (needA[f] && (rc = upload_host(ctx, static_cast<char*>(pA) + f * slice, A + 2 * npg * f, slice))) return rc;
+    if (op == 2) {
+      // achDtemp_cor reads its slices only as I_epsR - I_epsA (FF/ff.cpp:399-404, :436-441):
+      // the host forms the difference while staging, so half the bytes cross the bus, and
+      // the contraction runs on it in its plain mode (the same IEEE subtraction as on the
+      // device, so the result equals the device entry point's bit for bit)
+      for (int f = 0; f < nFreq; ++f)
+        if (needR[f] && (rc = upload_host_diff(ctx, static_cast<double*>(pR) + 2 * npg * f, R + 2 * npg * f,
+                                               A + 2 * npg * f, 2 * npg)))
+          return rc;
+    } else {
+      if ((rc = pool_get(ctx, "h_epsA", 2 * npg * nFreq * sizeof(double), &pA))) return rc;
+      for (int f = 0; f < nFreq; ++f) {
+        if (needR[f] && (rc = upload_host(ctx, static_cast<char*>(pR) + f * slice, R + 2 * npg * f, slice)))
+          return rc;
+        if (needA[f] && (rc = upload_host(ctx, static_cast<char*>(pA) + f * slice, A + 2 * npg * f, slice)))
+          return rc;
+      }
     }
     d_R = static_cast<const double*>(pR);
-    d_A = static_cast<const double*>(pA);
+    d_A = static_cast<const double*>(pA);  // null for achDtemp_cor: d_R holds R - A
   }
   void* d_out;
   if ((rc = pool_get(ctx, "h_ffout", (size_t)(nfreqeval > 1 ? nfreqeval : 1) * 16, &d_out))) return rc;
@@ -367,11 +380,18 @@ int cor_band_host(bgw_ctx* ctx, double scale, int ncouls, int ngpown, int ifreq,
   if ((rc = upload(ctx, "h_aqsm", aqsm + 2 * (size_t)n1 * ncouls, 2 * (size_t)ncouls, &d_am))) return rc;
   if ((rc = upload(ctx, "h_aqsn", aqsn + 2 * (size_t)n1 * ncouls, 2 * (size_t)ncouls, &d_an))) return rc;
   if ((rc = upload(ctx, "h_vcoul", vcoul, (size_t)ncouls, &d_vc))) return rc;
-  if ((rc = upload(ctx, "h_epsR", R + 2 * npg * (size_t)ifreq, 4 * npg, &d_R))) return rc;
-  if ((rc = upload(ctx, "h_epsA", A + 2 * npg * (size_t)ifreq, 4 * npg, &d_A))) return rc;
+  // slices ifreq, ifreq + 1 of I_epsR - I_epsA, formed by the host while staging (the
+  // kernels read the arrays only through that difference, FF/ff.cpp:399-404, :436-441)
+  void* pD;
+  if ((rc = pool_get(ctx, "h_epsR", 4 * npg * sizeof(double), &pD))) return rc;
+  if ((rc = upload_host_diff(ctx, static_cast<double*>(pD), R + 2 * npg * (size_t)ifreq, A + 2 * npg * (size_t)ifreq,
+                             4 * npg)))
+    return rc;
+  d_R = static_cast<const double*>(pD);
+  d_A = nullptr;
   void* d_out;
   if ((rc = pool_get(ctx, "h_ffout", 16, &d_out))) return rc;
-  // the staged arrays hold row n1 as row 0 and slices ifreq, ifreq + 1 as slices 0, 1
+  // the staged arrays hold row n1 as row 0 and slices ifreq, ifreq + 1 of R - A as slices 0, 1
   rc = cor_band_device(ctx, scale, ncouls, ngpown, 0, 0, fact1, fact2, d_inv, d_indinv, d_R, d_A, d_am, d_an, d_vc,
                        static_cast<double*>(d_out));
   if (rc) {
