@@ -45,10 +45,35 @@ def read_traffic():
     return None
 
 
+def cpu_reference_one_core(O, d_full, bands: int):
+    """The same reference build on one thread (the OpenMP runtime it runs on, set to one
+    thread for the call and restored after), on `bands` bands of the workload."""
+    import ctypes
+
+    try:
+        gomp = ctypes.CDLL("libgomp.so.1")
+    except OSError:
+        return None
+    nc = d_full["ncouls"]
+    sub = dict(d_full, nb=bands, aqsm=d_full["aqsm"][: bands * nc].copy(), aqsn=d_full["aqsn"][: bands * nc].copy(),
+               wx=d_full["wx"][: 3 * bands].copy())
+    before = gomp.omp_get_max_threads()
+    gomp.omp_set_num_threads(1)
+    try:
+        O.ref_gpp_per_band(sub, kind="omp")
+        dt = O.ref_last_seconds("omp")
+    finally:
+        gomp.omp_set_num_threads(before)
+    flops = 15.0 * bands * sub["ngpown"] * nc * 3
+    return {"value": flops / dt / 1e12, "unit": "TFLOP/s", "cores": 1, "seconds": dt,
+            "sample": f"{bands} of the 512 bands, one thread"}
+
+
 def cpu_baseline(d_full, bands: int):
     """Times the reference's CPU path (oracle/_ref, built from the reference
     sources with its GCC flags + OpenMP) on `bands` bands of the workload; falls
-    back to the oracle port when the reference build is absent."""
+    back to the oracle port when the reference build is absent.  Returns the per-band
+    values, a one-thread sample of the same build (or None) and the baseline entry."""
     sys.path.insert(0, str(ROOT / "oracle"))
     import oracle as O  # checker / baseline only
 
@@ -69,7 +94,8 @@ def cpu_baseline(d_full, bands: int):
         per_band = O.gpp_forward(sub, want=("achtemp", "acht_n1_loc"), nthreads=nthr)["acht_n1_loc"]
         dt = time.perf_counter() - t0
     flops = 15.0 * bands * sub["ngpown"] * nc * 3
-    return per_band, {"value": flops / dt / 1e12, "unit": "TFLOP/s", "cores": nthr, "kind": kind,
+    one_core = cpu_reference_one_core(O, d_full, 16) if kind == "reference" else None
+    return per_band, one_core, {"value": flops / dt / 1e12, "unit": "TFLOP/s", "cores": nthr, "kind": kind,
             "sample": f"{bands} of the 512 bands of the benchmark workload (ngpown 1638 x ncouls 32768 x 3 iw), "
                       f"{dt:.2f} s wall, the reference noflagOCC_solver (GPP/gpp.cpp:307-402, OpenMP, its GCC "
                       f"flags) called once per band with that band's wx"}
@@ -405,7 +431,9 @@ def main():
     cpu = None
     if not args.no_cpu_baseline:
         log("cpu baseline")
-        per_band, cpu = cpu_baseline(d, args.cpu_bands)
+        per_band, one_core, cpu = cpu_baseline(d, args.cpu_bands)
+        if one_core:
+            extras["cpu_reference_one_core"] = one_core
         # the same bands from the GPU step above, against the CPU path's values
         nbc = per_band.shape[0]
         extras[f"cpu_{cpu['kind']}_bands_compared"] = nbc
