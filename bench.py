@@ -434,6 +434,12 @@ def main():
         per_band, one_core, cpu = cpu_baseline(d, args.cpu_bands)
         if one_core:
             extras["cpu_reference_one_core"] = one_core
+        extras["host_nproc"] = os.cpu_count()
+        try:
+            extras["host_cpu_model"] = next(line.split(":", 1)[1].strip() for line in open("/proc/cpuinfo")
+                                            if line.startswith("model name"))
+        except (OSError, StopIteration):
+            pass
         # the same bands from the GPU step above, against the CPU path's values
         nbc = per_band.shape[0]
         extras[f"cpu_{cpu['kind']}_bands_compared"] = nbc
