@@ -60,6 +60,24 @@ def test_config2_reduced_bands_fp32_bulk(ctx32):
     assert rel_err(r["acht_n1_loc"], o["acht_n1_loc"]) < TOL32
 
 
+@pytest.mark.slow
+def test_config2_full_fp32_bulk_vs_fp64(ctx):
+    """Config 2 at full size (512 bands, ngpown 1638, ncouls 32768, every output): the
+    FP32-bulk form against the FP64 path on the same inputs (which is itself checked
+    against the reference's own kernel at this size), per output at most 1e-6 relative
+    and visibly above FP64 rounding."""
+    d = bgw.gpp_config_inputs(2)
+    r64 = ctx.gpp_solver(d)
+    ctx.set_precision(bgw.BGW_FP32_BULK)
+    try:
+        r32 = ctx.gpp_solver(d)
+    finally:
+        ctx.set_precision(bgw.BGW_FP64)
+    for k in ("achtemp", "acht_n1_loc", "achstemp", "asxtemp"):
+        assert rel_err(r32[k], r64[k]) < TOL32, k
+    assert rel_err(r32["achtemp"], r64["achtemp"]) > 1e-13
+
+
 def test_ff_split_fp32_vs_oracle(ctx32):
     f = bgw.ff_inputs(130, 5, 1000, 65, 15, 3, 1030)
     achs = ctx32.achsDtemp_Kernel(f)
