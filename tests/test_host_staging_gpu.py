@@ -171,3 +171,40 @@ def test_staged_fp32_form_and_caller_stream(ctx):
     finally:
         ctx.set_precision(bgw.BGW_FP64)
         ctx.set_stream(None)
+
+
+def test_two_contexts_in_two_threads(ctx):
+    """Distinct contexts may be used from distinct threads (include/bgw.h): two threads,
+    each with its own context, run the host GPP and FF entry points at the same time
+    (ctypes releases the GIL during the calls) through their own pinned rings and copy
+    streams, and get what one context gets alone, bit for bit."""
+    import threading
+
+    d = bgw.gpp_inputs(64, 2, 4000, 200, 7)  # wtilde / I_eps 12.8 MB each: staged
+    f = bgw.ff_inputs(40, 3, 96, 24, 15, 2, 8)
+    ref_g = ctx.gpp_solver(d)
+    ref_f = ctx.achDtemp_cor_Kernel(f)
+    out, errs = {}, []
+
+    def work(i):
+        try:
+            with bgw.Context(0) as c:
+                for _ in range(3):
+                    g = c.gpp_solver(d)
+                    ff = c.achDtemp_cor_Kernel(f)
+                out[i] = (g, ff)
+        except Exception as e:  # reported below, in the main thread
+            errs.append(repr(e))
+
+    ts = [threading.Thread(target=work, args=(i,)) for i in range(2)]
+    for t in ts:
+        t.start()
+    for t in ts:
+        t.join(timeout=100)
+    assert not errs, errs
+    assert sorted(out) == [0, 1]
+    for i in range(2):
+        g, ff = out[i]
+        for k in ref_g:
+            assert np.array_equal(g[k], ref_g[k]), (i, k)
+        assert np.array_equal(ff, ref_f), i
