@@ -143,11 +143,45 @@ def _ptr(a):
     return a.ctypes.data
 
 
-def _cplx(a: np.ndarray) -> np.ndarray:
-    a = np.ascontiguousarray(a)
-    if a.dtype != np.complex128:
-        raise TypeError(f"expected complex128, got {a.dtype}")
-    return a
+class _Hold:
+    """The array arguments of one foreign call, in the layout and dtype the C ABI reads
+    (C-contiguous int32, float64 or complex128).  A converted copy must outlive the call,
+    so the holder keeps every array it returns a pointer to; it lives as long as the
+    calling method."""
+
+    def __init__(self):
+        self._keep = []
+
+    def _ptr(self, a):
+        self._keep.append(a)
+        return a.ctypes.data
+
+    def i32(self, a):
+        if a is None:
+            return None
+        a = np.asarray(a)
+        if not np.issubdtype(a.dtype, np.integer):
+            raise TypeError(f"expected an integer index array, got {a.dtype}")
+        b = np.ascontiguousarray(a, np.int32)
+        if b.dtype != a.dtype and not np.array_equal(b, a):
+            raise ValueError("index array has values outside the int32 range")
+        return self._ptr(b)
+
+    def f64(self, a):
+        if a is None:
+            return None
+        a = np.asarray(a)
+        if np.iscomplexobj(a):
+            raise TypeError(f"expected a real array, got {a.dtype}")
+        return self._ptr(np.ascontiguousarray(a, np.float64))
+
+    def c128(self, a):
+        if a is None:
+            return None
+        a = np.asarray(a)
+        if not np.iscomplexobj(a):
+            raise TypeError(f"expected a complex array, got {a.dtype}")
+        return self._ptr(np.ascontiguousarray(a, np.complex128))
 
 
 # ----------------------------------------------------------------- synthetic inputs
@@ -457,11 +491,10 @@ class Context:
     # ------------------------------------------------------------ GPP (host)
     def noflagOCC_solver(self, nb, ngpown, ncouls, inv, indinv, wx3, wtilde, aqsm, aqsn, I_eps, vcoul):
         ach = np.zeros(3, np.complex128)
+        h = _Hold()
         self._check(lib().bgw_noflagOCC_solver(
-            self._h, nb, ngpown, ncouls, _ptr(np.ascontiguousarray(inv, np.int32)),
-            _ptr(np.ascontiguousarray(indinv, np.int32)), _ptr(np.ascontiguousarray(wx3, np.float64)),
-            _ptr(_cplx(wtilde)), _ptr(_cplx(aqsm)), _ptr(_cplx(aqsn)), _ptr(_cplx(I_eps)),
-            _ptr(np.ascontiguousarray(vcoul, np.float64)), _ptr(ach)))
+            self._h, nb, ngpown, ncouls, h.i32(inv), h.i32(indinv), h.f64(wx3), h.c128(wtilde), h.c128(aqsm),
+            h.c128(aqsn), h.c128(I_eps), h.f64(vcoul), _ptr(ach)))
         return ach
 
     def gpp_solver(self, d: dict, occ: float = 1.0, outputs=("achtemp", "acht_n1_loc", "achstemp", "asxtemp")):
@@ -473,10 +506,11 @@ class Context:
             res["achstemp"] = np.zeros(1, np.complex128)
         if "asxtemp" in outputs:
             res["asxtemp"] = np.zeros(3, np.complex128)
+        h = _Hold()
         self._check(lib().bgw_gpp_solver(
-            self._h, nb, d["nvband"], d["ngpown"], d["ncouls"], _ptr(d["inv"]), _ptr(d["indinv"]), _ptr(d["wx"]),
-            _ptr(_cplx(d["wtilde"])), _ptr(_cplx(d["aqsm"])), _ptr(_cplx(d["aqsn"])), _ptr(_cplx(d["I_eps"])),
-            _ptr(d["vcoul"]), occ, _ptr(res["achtemp"]), _ptr(res.get("acht_n1_loc")), _ptr(res.get("achstemp")),
+            self._h, nb, d["nvband"], d["ngpown"], d["ncouls"], h.i32(d["inv"]), h.i32(d["indinv"]),
+            h.f64(d["wx"]), h.c128(d["wtilde"]), h.c128(d["aqsm"]), h.c128(d["aqsn"]), h.c128(d["I_eps"]),
+            h.f64(d["vcoul"]), occ, _ptr(res["achtemp"]), _ptr(res.get("acht_n1_loc")), _ptr(res.get("achstemp")),
             _ptr(res.get("asxtemp"))))
         if "acht_n1_loc" in res:
             res["acht_n1_loc"] = res["acht_n1_loc"].reshape(nb, 3)
@@ -487,10 +521,11 @@ class Context:
         ds = np.zeros(3, np.complex128)
         dw = np.zeros(npg, np.complex128)
         dI = np.zeros(npg, np.complex128)
+        h = _Hold()
         self._check(lib().bgw_gpp_backward(
-            self._h, d["nb"], d["nvband"], d["ngpown"], d["ncouls"], _ptr(d["inv"]), _ptr(d["indinv"]),
-            _ptr(d["wx"]), _ptr(_cplx(d["wtilde"])), _ptr(_cplx(d["aqsm"])), _ptr(_cplx(d["aqsn"])),
-            _ptr(_cplx(d["I_eps"])), _ptr(d["vcoul"]), occ, _ptr(ds), _ptr(dw), _ptr(dI)))
+            self._h, d["nb"], d["nvband"], d["ngpown"], d["ncouls"], h.i32(d["inv"]), h.i32(d["indinv"]),
+            h.f64(d["wx"]), h.c128(d["wtilde"]), h.c128(d["aqsm"]), h.c128(d["aqsn"]), h.c128(d["I_eps"]),
+            h.f64(d["vcoul"]), occ, _ptr(ds), _ptr(dw), _ptr(dI)))
         return {"dsigma_domega": ds, "dL_dwtilde": dw, "dL_dI_eps": dI}
 
     # ------------------------------------------------------------- GPP (device)
@@ -507,27 +542,30 @@ class Context:
     # -------------------------------------------------------------------- FF
     def achsDtemp_Kernel(self, f: dict):
         out = np.zeros(1, np.complex128)
+        h = _Hold()
         self._check(lib().bgw_achsDtemp_Kernel(
-            self._h, f["nb"], f["ngpown"], f["ncouls"], _ptr(f["inv"]), _ptr(f["indinv"]), _ptr(_cplx(f["aqsn"])),
-            _ptr(_cplx(f["aqsm"])), _ptr(_cplx(f["I_epsR"])), _ptr(f["vcoul"]), _ptr(out)))
+            self._h, f["nb"], f["ngpown"], f["ncouls"], h.i32(f["inv"]), h.i32(f["indinv"]), h.c128(f["aqsn"]),
+            h.c128(f["aqsm"]), h.c128(f["I_epsR"]), h.f64(f["vcoul"]), _ptr(out)))
         return out[0]
 
     def asxDtemp_Kernel(self, f: dict):
         out = np.zeros(f["nfreqeval"], np.complex128)
+        h = _Hold()
         self._check(lib().bgw_asxDtemp_Kernel(
             self._h, f["nvband"], f["nfreqeval"], f["ncouls"], f["ngpown"], f["nFreq"], f["freqevalmin"],
-            f["freqevalstep"], f["occ"], _ptr(f["ekq"]), _ptr(f["dFreqGrid"]), _ptr(f["inv"]), _ptr(f["indinv"]),
-            _ptr(_cplx(f["aqsm"])), _ptr(_cplx(f["aqsn"])), _ptr(f["vcoul"]), _ptr(_cplx(f["I_epsR"])),
-            _ptr(_cplx(f["I_epsA"])), _ptr(out)))
+            f["freqevalstep"], f["occ"], h.f64(f["ekq"]), h.f64(f["dFreqGrid"]), h.i32(f["inv"]), h.i32(f["indinv"]),
+            h.c128(f["aqsm"]), h.c128(f["aqsn"]), h.f64(f["vcoul"]), h.c128(f["I_epsR"]), h.c128(f["I_epsA"]),
+            _ptr(out)))
         return out
 
     def achDtemp_cor_Kernel(self, f: dict):
         out = np.zeros(f["nfreqeval"], np.complex128)
+        h = _Hold()
         self._check(lib().bgw_achDtemp_cor_Kernel(
             self._h, f["nb"], f["nvband"], f["nfreqeval"], f["ncouls"], f["ngpown"], f["nFreq"], f["freqevalmin"],
-            f["freqevalstep"], _ptr(f["ekq"]), _ptr(f["dFreqGrid"]), _ptr(f["inv"]), _ptr(f["indinv"]),
-            _ptr(_cplx(f["aqsm"])), _ptr(_cplx(f["aqsn"])), _ptr(f["vcoul"]), _ptr(_cplx(f["I_epsR"])),
-            _ptr(_cplx(f["I_epsA"])), _ptr(out)))
+            f["freqevalstep"], h.f64(f["ekq"]), h.f64(f["dFreqGrid"]), h.i32(f["inv"]), h.i32(f["indinv"]),
+            h.c128(f["aqsm"]), h.c128(f["aqsn"]), h.f64(f["vcoul"]), h.c128(f["I_epsR"]), h.c128(f["I_epsA"]),
+            _ptr(out)))
         return out
 
     def schDttt_corKernel(self, kernel: int, f: dict, n1: int, ifreq: int, fact1: float, fact2: float) -> complex:
@@ -535,9 +573,10 @@ class Context:
         band n1's term of achDtemp_cor on slices ifreq, ifreq + 1."""
         out = np.zeros(1, np.complex128)
         fn = getattr(lib(), f"bgw_schDttt_corKernel{int(kernel)}")
-        self._check(fn(self._h, _ptr(out), _ptr(f["inv"]), _ptr(f["indinv"]), _ptr(_cplx(f["I_epsR"])),
-                       _ptr(_cplx(f["I_epsA"])), _ptr(_cplx(f["aqsm"])), _ptr(_cplx(f["aqsn"])), _ptr(f["vcoul"]),
-                       f["ncouls"], ifreq, f["ngpown"], n1, fact1, fact2))
+        h = _Hold()
+        self._check(fn(self._h, _ptr(out), h.i32(f["inv"]), h.i32(f["indinv"]), h.c128(f["I_epsR"]),
+                       h.c128(f["I_epsA"]), h.c128(f["aqsm"]), h.c128(f["aqsn"]), h.f64(f["vcoul"]), f["ncouls"],
+                       ifreq, f["ngpown"], n1, fact1, fact2))
         return out[0]
 
     def schDttt_corKernel_device(self, kernel: int, f: dict, ptrs: dict, n1: int, ifreq: int, fact1: float,

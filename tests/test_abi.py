@@ -112,3 +112,30 @@ def test_reference_shim_links_after_the_reference_headers():
         pytest.skip("the reference tree is not present")
     r = subprocess.run(["make", "-s", "-B", "-C", str(odir), "shim-check"], capture_output=True, text=True)
     assert r.returncode == 0, r.stdout + r.stderr
+
+
+def test_binding_converts_and_keeps_call_arguments():
+    """bgw._Hold, which every host-array method of bgw.Context passes its arrays through:
+    it rejects dtypes the C ABI cannot take as they are (an interleaved float64 buffer as a
+    complex array, a complex array as a real one, indices beyond int32) and keeps the
+    contiguous copies it makes alive, so the pointer it returns stays valid for the call."""
+    import gc
+
+    h = bgw._Hold()
+    with pytest.raises(TypeError):
+        h.c128(np.zeros(4))
+    with pytest.raises(TypeError):
+        h.f64(np.zeros(4, np.complex128))
+    with pytest.raises(TypeError):
+        h.i32(np.zeros(4))
+    with pytest.raises(ValueError):
+        h.i32(np.array([1, 2**40]))
+    z = (np.arange(8) + 1j * np.arange(8))[::2]  # strided: a copy is made
+    p = h.c128(z)
+    ip = h.i32(np.arange(5, dtype=np.int64))
+    del z
+    gc.collect()
+    got = np.ctypeslib.as_array((C.c_double * 8).from_address(p))
+    assert np.array_equal(got, [0, 0, 2, 2, 4, 4, 6, 6])
+    assert list((C.c_int * 5).from_address(ip)) == [0, 1, 2, 3, 4]
+    assert h.c128(None) is None and h.i32(None) is None

@@ -202,3 +202,22 @@ def test_config1_full_vs_reference_build(ctx):
     r = O.ref_gpp_per_band(d, kind="omp")
     assert np.max(np.abs(a["acht_n1_loc"] - r) / np.abs(r)) < 1e-11
     assert rel_err(a["achtemp"], r.sum(0)) < 1e-11
+
+
+def test_binding_takes_int64_indices_and_strided_arrays(ctx):
+    """The Python binding converts int64 index arrays and non-contiguous views before the
+    call (and keeps the copies alive through it): results equal those of canonical inputs."""
+    d = bgw.gpp_inputs(16, 2, 600, 40, 3)
+    ref = ctx.gpp_solver(d)
+    e = dict(d, inv=d["inv"].astype(np.int64), indinv=d["indinv"].astype(np.int64))
+    for k in ("wtilde", "aqsm", "aqsn", "I_eps"):
+        big = np.zeros(2 * d[k].size, np.complex128)
+        big[::2] = d[k]
+        e[k] = big[::2]
+        assert not e[k].flags["C_CONTIGUOUS"]
+    got = ctx.gpp_solver(e)
+    for k in ref:
+        assert np.array_equal(got[k], ref[k]), k
+    f = bgw.ff_inputs(20, 2, 96, 24, 6, 2, 3)
+    g = dict(f, inv=f["inv"].astype(np.int64), I_epsA=np.asfortranarray(f["I_epsA"].reshape(6, 24, 96)))
+    assert np.array_equal(ctx.achDtemp_cor_Kernel(g), ctx.achDtemp_cor_Kernel(f))
