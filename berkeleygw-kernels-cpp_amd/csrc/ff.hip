@@ -233,8 +233,6 @@ int ff_validate(bgw_ctx* ctx, int op, int nb, int nvband, int nfreqeval, int nco
   if (nb <= 0 || ncouls <= 0 || ngpown <= 0) return fail(ctx, BGW_EINVAL, "zero or negative size");
   if (ngpown > ncouls) return fail(ctx, BGW_EINVAL, "ngpown cannot be greater than ncouls");  // FF/ff.cpp:492-499
   if (op != 0 && (nFreq < 2 || nfreqeval <= 0)) return fail(ctx, BGW_EINVAL, "nFreq < 2 or nfreqeval <= 0");
-  // one grid z-slice per frequency slice (two per slice for asxDtemp)
-  if (op != 0 && (op == 1 ? 2L * nFreq : (long)nFreq) > 65535) return fail(ctx, BGW_EINVAL, "nFreq too large");
   if (op != 0 && (nvband < 0 || nvband > nb)) return fail(ctx, BGW_EINVAL, "nvband out of [0, number_bands]");
   size_t t;
   if (!mul_ok((size_t)ngpown, (size_t)ncouls, &t) || !mul_ok(t, (size_t)(nFreq > 2 ? nFreq : 2), &t) ||

@@ -578,8 +578,12 @@ int prep_alloc(bgw_ctx* ctx, int nb, int ngpown, PrepTables* t) {
 void prep_rows(bgw_ctx* ctx, const PrepTables& t, int nb, int p0, int p1, int ncouls, const int* inv,
                const int* indinv, const double* wx, int wx_per_band, const double2* wt, const double2* am,
                const double2* an, const double* vcoul, int gathered = 0, int ngpown = 0) {
-  hipLaunchKernelGGL(k_gpp_prep, dim3((nb + 255) / 256, p1 - p0), dim3(256), 0, ctx->stream, inv, indinv, wt, am,
-                     an, vcoul, wx, wx_per_band, nb, ncouls, p0, gathered, ngpown, t.S, t.C, t.wxb);
+  // one grid row per G' row, and a grid has at most 65535 rows: launch in chunks of that many
+  for (int q0 = p0; q0 < p1; q0 += 65535) {
+    const int q1 = std::min(p1, q0 + 65535);
+    hipLaunchKernelGGL(k_gpp_prep, dim3((nb + 255) / 256, q1 - q0), dim3(256), 0, ctx->stream, inv, indinv, wt, am,
+                       an, vcoul, wx, wx_per_band, nb, ncouls, q0, gathered, ngpown, t.S, t.C, t.wxb);
+  }
 }
 
 // Work-item bookkeeping of one forward call: launch shape and the partial buffers.

@@ -63,8 +63,8 @@ def test_committed_reference_vectors(ctx):
 
 
 def test_large_frequency_grid_vs_oracle(ctx):
-    # the reference's "compute bound" regime has many frequencies (FF/ff.cpp:56-57);
-    # 2*nFreq slices exceed the plan kernel's LDS counters and use its global path
+    # the reference's "compute bound" regime has many frequencies (FF/ff.cpp:56-57):
+    # 2 * 5000 jobs in the plan and the tile prefix
     f = bgw.ff_inputs(8, 2, 100, 10, 5000, 7, 5000)
     achs, asx, cor = _all(ctx, f)
     assert abs(achs - O.achsDtemp(f)) <= TOL * abs(O.achsDtemp(f))

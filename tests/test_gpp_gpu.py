@@ -74,6 +74,12 @@ def test_noflag_reference_signature_vs_reference_semantics(ctx):
     assert rel_err(ach, o["achtemp"]) < TOL
 
 
+def test_more_gprime_rows_than_one_grid_dimension(ctx):
+    # ngpown above 65535 (a grid's row limit): the S-table prep launches in chunks
+    d = bgw.gpp_inputs(3, 2, 4, 66000, 66)
+    _check_all(ctx.gpp_solver(d), O.gpp_forward(d))
+
+
 def test_reference_index_init_reaches_ncouls(ctx):
     # inv_igp_index[p] = (p+1)*ncouls/ngpown reaches ncouls for the last p
     # (GPP/gpp.cpp:263-269): indinv[ncouls] = ncouls-1 must be honoured
