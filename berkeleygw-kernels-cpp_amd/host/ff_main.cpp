@@ -93,8 +93,14 @@ int main(int argc, char** argv) {
       dump_path = argv[++i];
     else if (a == "--random" && i + 1 < argc)
       seed = std::atoll(argv[++i]);
-    else if (a == "--precision" && i + 1 < argc)
-      precision = std::string(argv[++i]) == "fp32" ? BGW_FP32_BULK : BGW_FP64;
+    else if (a == "--precision" && i + 1 < argc) {
+      const std::string p = argv[++i];
+      if (p != "fp32" && p != "fp64") {
+        usage();
+        return 1;
+      }
+      precision = p == "fp32" ? BGW_FP32_BULK : BGW_FP64;
+    }
     else if (a == "-h" || a == "--help") {
       usage();
       return 0;

@@ -97,7 +97,11 @@ int main(int argc, char** argv) {
     } else if (a == "--random" && i + 1 < argc) {
       seed = std::atoll(argv[++i]);
     } else if (a == "--precision" && i + 1 < argc) {
-      std::string p = argv[++i];
+      const std::string p = argv[++i];
+      if (p != "fp32" && p != "fp64") {
+        usage();
+        return 1;
+      }
       precision = p == "fp32" ? BGW_FP32_BULK : BGW_FP64;
     } else if (a == "--all") {
       all = true;

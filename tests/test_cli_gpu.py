@@ -60,6 +60,8 @@ def test_gpp_driver_usage_errors():
     assert _run("gpp.ex", 1, 2, 3)[0] == 1  # three positional arguments
     assert _run("gpp.ex", 8, 9, 100, 4)[0] == 1  # nvband > number_bands
     assert _run("gpp.ex", 8, 2, 10, 20)[0] == 1  # ngpown = 0
+    assert _run("gpp.ex", 8, 2, 64, 4, "--precision", "fp16")[0] == 1  # unknown precision form
+    assert _run("ff.ex", 10, 2, 20, 5, 5, 2, "--precision", "fp16")[0] == 1
 
 
 @pytest.mark.parametrize("precision", ["fp64", "fp32"])
