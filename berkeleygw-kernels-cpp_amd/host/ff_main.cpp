@@ -13,6 +13,7 @@
 #include <complex>
 #include <cstdio>
 #include <cstdlib>
+#include <new>
 #include <string>
 #include <vector>
 
@@ -80,7 +81,7 @@ static void verdict(const char* name, cplx got, cplx exact, double tol) {
     std::printf("\n!!!! %s : FAILURE - Correctness test failed :-( :-(  (rel err %.3e vs exact)\n", name, rel);
 }
 
-int main(int argc, char** argv) {
+static int run(int argc, char** argv) {
   std::vector<std::string> pos;
   long long seed = -1;
   int precision = BGW_FP64;
@@ -129,6 +130,12 @@ int main(int argc, char** argv) {
   }
   if (ngpown > ncouls) {
     std::printf("Incorrect Parameters!!! ngpown cannot be greater than ncouls. The correct form is\n");
+    usage();
+    return 1;
+  }
+  if (number_bands <= 0 || nvband < 0 || nvband > number_bands || ncouls <= 0 || ngpown <= 0 || nFreq < 2 ||
+      nfreqeval <= 0) {
+    std::printf("Incorrect Parameters!!! The correct form is\n");
     usage();
     return 1;
   }
@@ -236,4 +243,17 @@ int main(int argc, char** argv) {
               achDtemp_cor(0).imag());
   std::printf("********** Kernel  Time Taken (Includes data allocation) **********= %g secs\n", kernel_s);
   return 0;
+}
+
+int main(int argc, char** argv) {
+  // the reference-signature shim reports library errors as bgw::Error (its functions
+  // return void, like the reference's): print them and exit with status 1
+  try {
+    return run(argc, argv);
+  } catch (const bgw::Error& e) {
+    std::fprintf(stderr, "libbgw error %d: %s\n", e.code, e.what());
+  } catch (const std::bad_alloc&) {
+    std::fprintf(stderr, "out of host memory\n");
+  }
+  return 1;
 }

@@ -14,6 +14,7 @@
 #include <complex>
 #include <cstdio>
 #include <cstdlib>
+#include <new>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -82,7 +83,7 @@ static int run_input_set(const char* path, int precision, bool all) {
   return 0;
 }
 
-int main(int argc, char** argv) {
+static int run(int argc, char** argv) {
   std::vector<std::string> pos;
   long long seed = -1;
   int precision = BGW_FP64;
@@ -238,4 +239,17 @@ int main(int argc, char** argv) {
   std::printf("********** Kernel Time Taken **********= %g secs\n", kernel_s);
   std::printf("********** Total Time Taken **********= %g secs\n", total_s);
   return 0;
+}
+
+int main(int argc, char** argv) {
+  // the reference-signature shim reports library errors as bgw::Error (its functions
+  // return void, like the reference's): print them and exit with status 1
+  try {
+    return run(argc, argv);
+  } catch (const bgw::Error& e) {
+    std::fprintf(stderr, "libbgw error %d: %s\n", e.code, e.what());
+  } catch (const std::bad_alloc&) {
+    std::fprintf(stderr, "out of host memory\n");
+  }
+  return 1;
 }

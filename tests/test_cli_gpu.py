@@ -84,6 +84,8 @@ def test_ff_driver_random_inputs_vs_oracle():
 def test_ff_driver_rejects_ngpown_above_ncouls():
     rc, out = _run("ff.ex", 10, 2, 20, 30, 5, 2)
     assert rc == 1 and "ngpown cannot be greater than ncouls" in out
+    for bad in ((0, 0, 0, 0, 0, 0), (10, 2, 20, 5, 1, 2), (10, 12, 20, 5, 5, 2)):  # sizes, nFreq < 2, nvband
+        assert _run("ff.ex", *bad)[0] == 1, bad
 
 
 def test_gpp_driver_dump_then_input_reproduces_random_run(tmp_path):
