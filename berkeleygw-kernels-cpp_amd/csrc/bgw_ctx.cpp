@@ -325,8 +325,17 @@ const char* bgw_last_error(const bgw_ctx* ctx) { return ctx ? ctx->err.c_str() :
 
 int bgw_set_stream(bgw_ctx* ctx, void* s) {
   if (!ctx) return BGW_EINVAL;
-  ctx->stream = s ? static_cast<hipStream_t>(s) : ctx->own_stream;
-  return BGW_OK;
+  hipStream_t next = s ? static_cast<hipStream_t>(s) : ctx->own_stream;
+  hipError_t e = hipSuccess;
+  if (next != ctx->stream) {
+    // work queued on the previous stream may still use the context's buffers, which
+    // the next call on the new stream reuses: let it finish first.  The switch happens
+    // even when that fails (a caller's stream already destroyed), and the error is reported
+    e = hipSetDevice(ctx->device);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  }
+  ctx->stream = next;
+  return e == hipSuccess ? BGW_OK : hip_fail(ctx, e, "hipStreamSynchronize (stream switch)");
 }
 
 void* bgw_get_stream(bgw_ctx* ctx) { return ctx ? static_cast<void*>(ctx->stream) : nullptr; }

@@ -58,7 +58,9 @@ typedef struct bgw_ctx bgw_ctx;
 int         bgw_create(bgw_ctx** ctx, int device);
 void        bgw_destroy(bgw_ctx* ctx);
 const char* bgw_last_error(const bgw_ctx* ctx);
-/* Launch on a caller-owned hipStream_t (NULL = the context's own stream). */
+/* Launch on a caller-owned hipStream_t (NULL = the context's own stream).  Switching
+ * to a different stream first waits for the work queued on the current one (it may
+ * still use the context's buffers). */
 int         bgw_set_stream(bgw_ctx* ctx, void* hip_stream);
 void*       bgw_get_stream(bgw_ctx* ctx);
 int         bgw_set_precision(bgw_ctx* ctx, int precision);
