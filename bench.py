@@ -378,10 +378,21 @@ def main():
 
     full_ms = None
     host_ms = None
+    full_kernels = {}
     if not args.no_extras:
         step(True)
+        ctx.reset_timing()
+        ctx.set_timing(True)
         full_ms = timed(max(2, args.steps // 2), full=True)
-        log(f"[full outputs] {full_ms:.3f} ms/step")
+        ctx.set_timing(False)
+        # achstemp: the masked MFMA contraction, 8 FLOP per complex MAC over nb x ngpown x ncouls;
+        # asxtemp: the SX term of the nvband occupied bands (VALU)
+        for key, kname in (("achstemp", "contract_achs"), ("asxtemp", "gpp_asx")):
+            kms, kn = ctx.kernel_time(kname)
+            full_kernels[f"{key}_kernel_ms"] = kms / max(kn, 1)
+        full_kernels["achstemp_tflops"] = 8.0 * nb * ngpown * ncouls / (full_kernels["achstemp_kernel_ms"] * 1e-3) / 1e12
+        log(f"[full outputs] {full_ms:.3f} ms/step, achstemp {full_kernels['achstemp_kernel_ms']:.3f} ms, "
+            f"asxtemp {full_kernels['asxtemp_kernel_ms']:.3f} ms")
         # host-pointer entry point (bgw_gpp_solver): uploads the inputs from pageable
         # host memory, computes achtemp + acht_n1_loc, copies back, synchronises
         ctx.gpp_solver(d, outputs=("achtemp", "acht_n1_loc"))  # grows the upload pools
@@ -394,6 +405,7 @@ def main():
     value = flops / (f64["ms"] * 1e-3) / 1e12
     kern_tflops = flops / (f64["kernel_ms"] * 1e-3) / 1e12
     extras = {"fp64_gpp_main_ms": f64["kernel_ms"], "fp64_step_ms": f64["ms"], "full_outputs_step_ms": full_ms,
+              **{f"full_outputs_{k}": v for k, v in full_kernels.items()},
               "host_api_pcie_inclusive_ms": host_ms,
               "host_api_pcie_inclusive_tflops": flops / (host_ms * 1e-3) / 1e12 if host_ms else None}
     if "fp32_bulk" in res:

@@ -455,29 +455,26 @@ __global__ __launch_bounds__(NT) void k_gpp_asx(const int* __restrict__ inv, con
     const int gb = c * chunk, ge = min(ncouls, gb + chunk);
     const int igp = gpp_igp(inv, indinv, p, ncouls);
     const double vc = vcoul[igp];
-    for (int n1 = 0; n1 < nvband; ++n1) {
-      const double2 am = aqsm[(size_t)n1 * ncouls + igp];
-      // m = conj(aqsm) * occ * vcoul
-      const double2 m = make_double2(am.x * occ * vc, -am.y * occ * vc);
-      double s[3][2] = {{0, 0}, {0, 0}, {0, 0}};
-      for (int g = gb + threadIdx.x; g < ge; g += NT) {
-        const double2 w = wt[(size_t)p * ncouls + g];
-        const double2 I = Ieps[(size_t)p * ncouls + g];
-        const double2 an = aqsn[(size_t)n1 * ncouls + g];
+    // each (wtilde, I_eps) is read once and its wx-independent parts formed once for
+    // every occupied band and frequency; the band weight m(n1) = conj(aqsm) occ vcoul
+    // multiplies each term
+    double s[3][2] = {{0, 0}, {0, 0}, {0, 0}};
+    for (int g = gb + threadIdx.x; g < ge; g += NT) {
+      const SsxG G = gpp_ssx_g(wt[(size_t)p * ncouls + g], Ieps[(size_t)p * ncouls + g]);
+      for (int n1 = 0; n1 < nvband; ++n1) {
+        const double2 am = aqsm[(size_t)n1 * ncouls + igp];
+        const double2 m = make_double2(am.x * occ * vc, -am.y * occ * vc);
+        const double2 mn = cmul(m, aqsn[(size_t)n1 * ncouls + g]);
 #pragma unroll
         for (int iw = 0; iw < 3; ++iw) {
-          const double2 ss = gpp_ssx(wxb[n1 * 3 + iw], w, I);
-          const double2 t = cmul(an, ss);
+          const double2 t = cmul(mn, gpp_ssx(wxb[n1 * 3 + iw], G));
           s[iw][0] += t.x;
           s[iw][1] += t.y;
         }
       }
-#pragma unroll
-      for (int iw = 0; iw < 3; ++iw) {
-        const double2 t = cmul(m, make_double2(s[iw][0], s[iw][1]));
-        tot[iw] = cdd_add(tot[iw], t.x, t.y);
-      }
     }
+#pragma unroll
+    for (int iw = 0; iw < 3; ++iw) tot[iw] = cdd_add(tot[iw], s[iw][0], s[iw][1]);
   }
 #pragma unroll
   for (int iw = 0; iw < 3; ++iw) {

@@ -44,9 +44,25 @@ __device__ inline int ff_igp(const int* __restrict__ inv, const int* __restrict_
 // near-pole region ssx = -Omega2 conj(cden)/|cden|^2 delw with cden = 4 wtilde^2 (delw+1/2),
 // zero otherwise (gpp_ssx_raw); gpp_ssx also applies the cut-off to zero when
 // |ssx| > sexcut |I_eps| for wx < 0.
-__device__ inline double2 gpp_ssx_raw(double wx, double2 wt, double2 I) {
-  const double2 wt2 = cmul(wt, wt);
-  const double2 Omega2 = cmul(wt2, I);
+// The parts of the ssx term that do not depend on wx, for one (wtilde, I_eps): computed
+// once and shared by every band and frequency of a G-vector (the same operations as
+// inline, so the values are bitwise those of the per-term form).
+struct SsxG {
+  double2 wt, wt2, Omega2;
+  double icut;  // sexcut |I_eps|
+};
+
+__device__ inline SsxG gpp_ssx_g(double2 wt, double2 I) {
+  SsxG s;
+  s.wt = wt;
+  s.wt2 = cmul(wt, wt);
+  s.Omega2 = cmul(s.wt2, I);
+  s.icut = kSexcut * sqrt(cnorm2(I));
+  return s;
+}
+
+__device__ inline double2 gpp_ssx_raw(double wx, const SsxG& g) {
+  const double2 wt = g.wt, wt2 = g.wt2, Omega2 = g.Omega2;
   const double2 wdiff = make_double2(wx - wt.x, -wt.y);
   const double wdiffr = cnorm2(wdiff);
   const double rden = 1.0 / wdiffr;
@@ -68,15 +84,22 @@ __device__ inline double2 gpp_ssx_raw(double wx, double2 wt, double2 I) {
   return ssx;
 }
 
-// the |ssx| > sexcut |I| cut-off for wx < 0, decided on the value before the cut
+__device__ inline double2 gpp_ssx_raw(double wx, double2 wt, double2 I) { return gpp_ssx_raw(wx, gpp_ssx_g(wt, I)); }
+
+// the |ssx| > sexcut |I| cut-off for wx < 0, decided on the value before the cut (wx is
+// tested first: it is the same for every lane of a band and frequency)
+__device__ inline bool gpp_ssx_cut(double wx, double2 ssx, double icut) { return wx < 0.0 && sqrt(cnorm2(ssx)) > icut; }
+
 __device__ inline bool gpp_ssx_cut(double wx, double2 ssx, double2 I) {
-  return sqrt(cnorm2(ssx)) > kSexcut * sqrt(cnorm2(I)) && wx < 0.0;
+  return gpp_ssx_cut(wx, ssx, kSexcut * sqrt(cnorm2(I)));
 }
 
-__device__ inline double2 gpp_ssx(double wx, double2 wt, double2 I) {
-  const double2 ssx = gpp_ssx_raw(wx, wt, I);
-  return gpp_ssx_cut(wx, ssx, I) ? make_double2(0.0, 0.0) : ssx;
+__device__ inline double2 gpp_ssx(double wx, const SsxG& g) {
+  const double2 ssx = gpp_ssx_raw(wx, g);
+  return gpp_ssx_cut(wx, ssx, g.icut) ? make_double2(0.0, 0.0) : ssx;
 }
+
+__device__ inline double2 gpp_ssx(double wx, double2 wt, double2 I) { return gpp_ssx(wx, gpp_ssx_g(wt, I)); }
 
 // MFMA trace contraction (contract.hip):
 //   out = sum_{r} sum_p W(r,p) sum_g A(rows[r], g) * B(p, g),
