@@ -53,8 +53,9 @@ extern "C" {
 typedef struct bgw_ctx bgw_ctx;
 
 /* ---- context ------------------------------------------------------------ */
-/* Creates a context on HIP device `device`.  No reference counterpart (the
- * reference maps data per call inside the solver, GPP/gpp.cpp:320-334). */
+/* None of the context, stream, precision and timing calls below has a reference
+ * counterpart: the reference has no handle and maps data per call inside the
+ * solver (GPP/gpp.cpp:320-334).  Creates a context on HIP device `device`. */
 int         bgw_create(bgw_ctx** ctx, int device);
 void        bgw_destroy(bgw_ctx* ctx);
 const char* bgw_last_error(const bgw_ctx* ctx);
@@ -109,6 +110,8 @@ int bgw_gpp_solver(bgw_ctx* ctx, size_t number_bands, size_t nvband, size_t ngpo
                    const double* aqsmtemp, const double* aqsntemp,
                    const double* I_eps_array, const double* vcoul, double occ,
                    double* achtemp, double* acht_n1_loc, double* achstemp, double* asxtemp);
+/* Device-pointer form of bgw_gpp_solver (all arrays on the device, results after
+ * bgw_synchronize); the same loop nest, GPP/gpp.cpp:339-391. */
 int bgw_gpp_solver_device(bgw_ctx* ctx, size_t number_bands, size_t nvband, size_t ngpown, size_t ncouls,
                           const int* inv_igp_index, const int* indinv,
                           const double* wx_array, const double* wtilde_array,
@@ -172,7 +175,8 @@ int bgw_achDtemp_cor_Kernel(bgw_ctx* ctx, int number_bands, int nvband, int nfre
                             const double* vcoul, const double* I_epsR_array,
                             const double* I_epsA_array, double* achDtemp_cor);
 
-/* Device-pointer forms of the three FF kernels (all arrays on the device). */
+/* Device-pointer forms of the three FF kernels above (all arrays on the device, results
+ * after bgw_synchronize); they replace the same reference functions, FF/ff.cpp:171-302. */
 int bgw_achsDtemp_Kernel_device(bgw_ctx* ctx, int number_bands, int ngpown, int ncouls,
                                 const int* inv_igp_index, const int* indinv,
                                 const double* aqsntemp, const double* aqsmtemp,
@@ -212,7 +216,8 @@ int bgw_schDttt_corKernel2(bgw_ctx* ctx, double* schDttt_cor, const int* inv_igp
                            const double* I_epsA_array, const double* aqsmtemp,
                            const double* aqsntemp, const double* vcoul, int ncouls, int ifreq,
                            int ngpown, int n1, double fact1, double fact2);
-/* Device-pointer forms (all arrays on the device, result after bgw_synchronize). */
+/* Device-pointer forms of the two above (all arrays on the device, result after
+ * bgw_synchronize); same reference functions, FF/ff.cpp:381-451. */
 int bgw_schDttt_corKernel1_device(bgw_ctx* ctx, double* schDttt_cor, const int* inv_igp_index,
                                   const int* indinv, const double* I_epsR_array,
                                   const double* I_epsA_array, const double* aqsmtemp,
@@ -237,8 +242,8 @@ void bgw_synth_index(unsigned long long seed, int n_max, int* out, size_t n);
 void bgw_synth_permutation(unsigned long long seed, int* out, size_t n);
 
 /* ---- input sets on disk (host only) --------------------------------------
- * The reference drivers build their inputs in memory (GPP/gpp.cpp:220-276,
- * FF/ff.cpp:552-633) and read no files; these calls store and load the same
+ * No reference counterpart: the reference drivers build their inputs in memory
+ * (GPP/gpp.cpp:220-276, FF/ff.cpp:552-633) and read no files; these calls store and load the same
  * arrays as a self-describing container, memory-mapped on read so the pointers
  * of a loaded set go to the solver entry points above without a copy.
  * Layout (little-endian):
