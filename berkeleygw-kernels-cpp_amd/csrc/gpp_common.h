@@ -20,7 +20,6 @@ __host__ __device__ inline double2 cmul(double2 a, double2 b) {
 __host__ __device__ inline double2 cconj(double2 a) { return make_double2(a.x, -a.y); }
 __host__ __device__ inline double2 cscale(double2 a, double s) { return make_double2(a.x * s, a.y * s); }
 __host__ __device__ inline double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
-__host__ __device__ inline double2 csub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
 __host__ __device__ inline double cnorm2(double2 a) { return a.x * a.x + a.y * a.y; }
 
 // igp = indinv[inv_igp_index[p]] (GPP/gpp.cpp:362-363), clamped into range so a bad
