@@ -621,5 +621,3 @@ def gpp_flops(nb: int, ngpown: int, ncouls: int) -> float:
     return 15.0 * nb * ngpown * ncouls * 3
 
 
-def ff_flops(n_cmacs: float) -> float:
-    return 8.0 * n_cmacs
