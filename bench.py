@@ -165,7 +165,15 @@ def ff_leg(bgw, torch, dev, stream, ctx, reps=3, cpu_reference=False):
     work = {"achsDtemp": 8.0 * f["nb"] * npg, "asxDtemp": 8.0 * 2 * f["nvband"] * npg,
             "achDtemp_cor": 8.0 * 2 * n_active * npg}
     names = {"achsDtemp": "contract_ff_achs", "asxDtemp": "contract_ff_asx", "achDtemp_cor": "contract_ff_cor"}
-    res = {"n_active_bands": n_active}
+    # asxDtemp reads each slice its valence bands fall in (ifreq and ifreq + 1 of I_epsR
+    # for wx > 0, of I_epsA otherwise) once: its bound is that reading
+    asx_slices = set()
+    for n1 in range(f["nvband"]):
+        wx = f["freqevalmin"] - f["ekq"][n1] + f["freqevalstep"]
+        ifreq = bgw.compute_fact(wx, f["dFreqGrid"], 1, False)[2]
+        asx_slices |= {(wx > 0, ifreq), (wx > 0, ifreq + 1)}
+    asx_bytes = 16.0 * len(asx_slices) * npg
+    res = {"n_active_bands": n_active, "asxDtemp_slice_bytes": asx_bytes}
     vals = {}
     for prec, pname, peak in ((bgw.BGW_FP64, "fp64", 78.6), (bgw.BGW_FP32_BULK, "split_fp32", 157.3)):
         ctx.set_precision(prec)
@@ -183,6 +191,8 @@ def ff_leg(bgw, torch, dev, stream, ctx, reps=3, cpu_reference=False):
             res[f"{op}_{pname}_ms"] = ms
             res[f"{op}_{pname}_kernel_ms"] = kms
             res[f"{op}_{pname}_mfma_frac"] = work[op] / (kms * 1e-3) / 1e12 / peak
+            if op == "asxDtemp":
+                res[f"asxDtemp_{pname}_hbm_tbs"] = asx_bytes / (kms * 1e-3) / 1e12
     ctx.set_precision(bgw.BGW_FP64)
     for op in ("achsDtemp", "asxDtemp", "achDtemp_cor"):
         n = 1 if op == "achsDtemp" else f["nfreqeval"]  # achsDtemp is a single value

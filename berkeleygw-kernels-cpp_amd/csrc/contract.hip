@@ -396,6 +396,102 @@ int pick_ksplit(int K, int BK, size_t tiles_x_ntn, int slots) {
   return best;
 }
 
+
+// Streaming form for sparse plans whose jobs hold at most kStreamRows rows
+// (asxDtemp's valence bands, the one-band schDttt terms).  There each slice element
+// meets at most a few rows, about 1 FLOP per byte, so the work is reading the slices
+// once: workgroup (job, pair of G' rows) streams the pair's rows of X (minus Y) with
+// unrolled coalesced loads, forms sum_g A[n1][g] x[g] per row in FP64 (the few A rows
+// stay in L2) and weights it with the row's w conj(aqsm[n1][igp]) vcoul[igp], as the
+// MFMA epilogue does.  One double-double partial per workgroup, zero for empty jobs.
+constexpr int kStreamNT = 256, kStreamUnroll = 4, kStreamPPer = 2;
+
+__global__ __launch_bounds__(kStreamNT) void k_contract_stream(ContractArgs a, int pblocks) {
+  const int job = blockIdx.x / pblocks, pb = blockIdx.x - job * pblocks;
+  const int tid = threadIdx.x;
+  __shared__ cdd red[kStreamNT / 64];
+  const int nrows = a.cnt[job];
+  cdd acc = cdd_zero();
+  if (nrows > 0) {
+    const int slice = a.slice_of_job[job];
+    const int* rows = a.rows + (size_t)job * a.max_rows;
+    int n1[kStreamRows];
+#pragma unroll
+    for (int r = 0; r < kStreamRows; ++r) n1[r] = rows[r < nrows ? r : 0];
+    // the workgroup's G' rows p0 .. p0 + kStreamPPer - 1 in one pass, so each A value
+    // loaded serves all of them; rows beyond N read row p0 and are left out below
+    const int p0 = pb * kStreamPPer;
+    const double2* __restrict__ X[kStreamPPer];
+    const double2* __restrict__ Y[kStreamPPer];
+    const double2* Xs = a.Xb && job >= a.jobs_a ? a.Xb : a.Xa;
+#pragma unroll
+    for (int q = 0; q < kStreamPPer; ++q) {
+      const int p = p0 + q < a.N ? p0 + q : p0;
+      const size_t off = (size_t)slice * a.slice_stride + (size_t)p * a.K;
+      X[q] = Xs + off;
+      Y[q] = a.Y ? a.Y + off : nullptr;
+    }
+    double sr[kStreamPPer][kStreamRows], si[kStreamPPer][kStreamRows];
+#pragma unroll
+    for (int q = 0; q < kStreamPPer; ++q)
+#pragma unroll
+      for (int r = 0; r < kStreamRows; ++r) sr[q][r] = si[q][r] = 0.0;
+    for (int g0 = tid; g0 < a.K; g0 += kStreamNT * kStreamUnroll) {
+      double2 x[kStreamPPer][kStreamUnroll];
+#pragma unroll
+      for (int q = 0; q < kStreamPPer; ++q)
+#pragma unroll
+        for (int u = 0; u < kStreamUnroll; ++u) {
+          const int g = g0 + u * kStreamNT;
+          x[q][u] = g < a.K ? X[q][g] : make_double2(0.0, 0.0);
+        }
+      if (a.Y) {
+#pragma unroll
+        for (int q = 0; q < kStreamPPer; ++q)
+#pragma unroll
+          for (int u = 0; u < kStreamUnroll; ++u) {
+            const int g = g0 + u * kStreamNT;
+            const double2 y = g < a.K ? Y[q][g] : make_double2(0.0, 0.0);
+            x[q][u].x -= y.x;
+            x[q][u].y -= y.y;
+          }
+      }
+#pragma unroll
+      for (int r = 0; r < kStreamRows; ++r) {
+        if (r >= nrows) break;
+        const double2* __restrict__ Ar = a.A + (size_t)n1[r] * a.K;
+#pragma unroll
+        for (int u = 0; u < kStreamUnroll; ++u) {
+          const int g = g0 + u * kStreamNT;
+          const double2 av = g < a.K ? Ar[g] : make_double2(0.0, 0.0);
+#pragma unroll
+          for (int q = 0; q < kStreamPPer; ++q) {
+            sr[q][r] = fma(av.x, x[q][u].x, fma(-av.y, x[q][u].y, sr[q][r]));
+            si[q][r] = fma(av.x, x[q][u].y, fma(av.y, x[q][u].x, si[q][r]));
+          }
+        }
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < kStreamPPer; ++q) {
+      const int p = p0 + q;
+      if (p >= a.N) break;
+      const int igp = ff_igp(a.inv, a.indinv, p, a.ncouls);
+      const double vc = a.vcoul[igp];
+#pragma unroll
+      for (int r = 0; r < kStreamRows; ++r) {
+        if (r >= nrows) break;
+        const double2 am = a.aqsm[(size_t)n1[r] * a.ncouls + igp];
+        const double2 w = cmul(a.wts[(size_t)job * a.max_rows + r], make_double2(vc * am.x, -vc * am.y));
+        const double2 t = cmul(w, make_double2(sr[q][r], si[q][r]));
+        acc = cdd_add(acc, t.x, t.y);
+      }
+    }
+  }
+  const cdd tot = cdd_block_reduce(acc, red);
+  if (tid == 0) cdd_store(&a.partial[blockIdx.x], tot);
+}
+
 }  // namespace
 
 int contract_launch(bgw_ctx* ctx, ContractArgs a, int njobs, int bm, int max_tiles, bool exact_tiles,
@@ -462,6 +558,29 @@ int contract_launch(bgw_ctx* ctx, ContractArgs a, int njobs, int bm, int max_til
     hipLaunchKernelGGL(kern, grid, dim3(NT), 0, ctx->stream, a);
   }
   if ((rc = check_launch(ctx, "k_contract"))) return rc;
+  launch_reduce_cdd(static_cast<const double4*>(pp), (int)W, 1, a.out, ctx->stream);
+  return check_launch(ctx, "reduce");
+}
+
+int contract_stream(bgw_ctx* ctx, ContractArgs a, int njobs, int max_job_rows, const char* tname) {
+  if (max_job_rows > kStreamRows || a.thr2 > 0.0)
+    return fail(ctx, BGW_EINVAL, "streaming contraction: a job has more rows than it handles");
+  const int pblocks = (a.N + kStreamPPer - 1) / kStreamPPer;
+  const long long W = (long long)njobs * pblocks;
+  if (W < 1) {
+    fill_zero(a.out, sizeof(double2), ctx->stream);
+    return check_launch(ctx, "fill");
+  }
+  if (W > INT32_MAX) return fail(ctx, BGW_EINVAL, "contraction grid too large");
+  void* pp;
+  int rc = pool_get(ctx, "ws_contract", (size_t)W * sizeof(double4), &pp);
+  if (rc) return rc;
+  a.partial = static_cast<double4*>(pp);
+  {
+    KernelTimer kt(ctx, tname);
+    hipLaunchKernelGGL(k_contract_stream, dim3((unsigned)W), dim3(kStreamNT), 0, ctx->stream, a, pblocks);
+  }
+  if ((rc = check_launch(ctx, "k_contract_stream"))) return rc;
   launch_reduce_cdd(static_cast<const double4*>(pp), (int)W, 1, a.out, ctx->stream);
   return check_launch(ctx, "reduce");
 }

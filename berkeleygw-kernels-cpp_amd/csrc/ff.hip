@@ -164,7 +164,11 @@ int ff_device(bgw_ctx* ctx, int op, int nb, int nvband, int nfreqeval, int ncoul
     a.Xa = reinterpret_cast<const double2*>(I_epsR);
     a.Xb = reinterpret_cast<const double2*>(I_epsA);
     a.jobs_a = nFreq;
-    rc = contract_launch(ctx, a, njobs, BMr, max_tiles, false, est_tiles, "contract_ff_asx");
+    // a job's rows are valence bands: few of them make the slices' reading the bound
+    if (nvband <= kStreamRows)
+      rc = contract_stream(ctx, a, njobs, nvband, "contract_ff_asx");
+    else
+      rc = contract_launch(ctx, a, njobs, BMr, max_tiles, false, est_tiles, "contract_ff_asx");
   } else {
     a.Xa = reinterpret_cast<const double2*>(I_epsR);
     a.Y = reinterpret_cast<const double2*>(I_epsA);
@@ -214,8 +218,7 @@ int cor_band_device(bgw_ctx* ctx, double scale, int ncouls, int ngpown, int ifre
   a.ncouls = ncouls;
   a.thr2 = 0.0;
   a.out = reinterpret_cast<double2*>(out);
-  const int bm = contract_row_tile(ctx->precision == BGW_FP32_BULK, true, 1);
-  return contract_launch(ctx, a, 2, bm, 2, true, 2, "contract_ff_corband");
+  return contract_stream(ctx, a, 2, 1, "contract_ff_corband");
 }
 
 }  // namespace bgw

@@ -38,7 +38,10 @@ def test_closed_form_reference_constants(ctx, nb, nvband, ncouls, ngpown, nFreq)
     assert np.all(np.abs(g[2] - cor) <= TOL * abs(cor))
 
 
-@pytest.mark.parametrize("shape", [(40, 3, 96, 24, 15, 2), (130, 5, 1000, 65, 15, 3), (77, 77, 530, 33, 4, 1)])
+# nvband up to 4 takes asxDtemp's streaming form, above it the MFMA tiles; (60, 4, ...)
+# is the largest streaming case, with an odd G' count and a ragged last G chunk
+@pytest.mark.parametrize("shape", [(40, 3, 96, 24, 15, 2), (60, 4, 700, 51, 9, 2), (130, 5, 1000, 65, 15, 3),
+                                   (77, 77, 530, 33, 4, 1)])
 def test_random_vs_oracle(ctx, shape):
     f = bgw.ff_inputs(*shape, 900 + shape[0])
     achs, asx, cor = _all(ctx, f)
