@@ -286,6 +286,9 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-bands", type=int, default=512)
     ap.add_argument("--no-extras", action="store_true", help="skip the FP32-bulk, full-output, FF and adjoint legs")
+    ap.add_argument("--form", choices=("fp64", "fp32_bulk"), default=None,
+                    help="only time the GPP step in this arithmetic form and print its time as one JSON line "
+                         "(for profiling one kernel)")
     ap.add_argument("--ff-only", action="store_true",
                     help="only run the FF leg (config 4) and print its results as one JSON line")
     ap.add_argument("--host-api-only", action="store_true",
@@ -332,6 +335,9 @@ def main():
 
     log("generating inputs")
     d = bgw.gpp_config_inputs(1, nb=args.nb)
+    if args.form:
+        args.no_extras = True
+        args.no_cpu_baseline = True
     nb, nvband, ngpown, ncouls = d["nb"], d["nvband"], d["ngpown"], d["ncouls"]
     T = {k: torch.from_numpy(np.ascontiguousarray(d[k])).to(dev) for k in
          ("inv", "indinv", "wx", "wtilde", "aqsm", "aqsn", "I_eps", "vcoul")}
@@ -366,7 +372,9 @@ def main():
     flops = bgw.gpp_flops(nb, ngpown, ncouls)
     res = {}
     for prec, name in ((bgw.BGW_FP64, "fp64"), (bgw.BGW_FP32_BULK, "fp32_bulk")):
-        if name == "fp32_bulk" and args.no_extras:
+        if args.form and name != args.form:
+            continue
+        if name == "fp32_bulk" and args.no_extras and not args.form:
             continue
         ctx.set_precision(prec)
         log(f"[{name}] warmup {args.warmup}")
@@ -411,6 +419,12 @@ def main():
         host_ms = (time.perf_counter() - t0) * 1e3
         log(f"[host api] {host_ms:.3f} ms/call")
 
+    if args.form:
+        r = res[args.form]
+        print(json.dumps({"form": args.form, "ms_per_step": r["ms"], "gpp_main_ms": r["kernel_ms"],
+                          "tflops": flops / (r["ms"] * 1e-3) / 1e12}), flush=True)
+        ctx.close()
+        return
     f64 = res["fp64"]
     value = flops / (f64["ms"] * 1e-3) / 1e12
     kern_tflops = flops / (f64["kernel_ms"] * 1e-3) / 1e12

@@ -51,7 +51,7 @@ __global__ void k_gpp_prep(const int* __restrict__ inv, const int* __restrict__ 
   t.y *= 0.5;
   t.x *= vc;
   t.y *= vc;
-  C[(size_t)p * nb + n1] = t;
+  if (C) C[(size_t)p * nb + n1] = t;
   S[(size_t)p * nb + n1] = cmul(t, w);
   if (p == 0) {
 #pragma unroll
@@ -547,8 +547,9 @@ int run_main(bgw_ctx* ctx, MainCfg cfg, const double2* wt, const double2* I, con
 
 }  // namespace
 
-// S[p][n1], C[p][n1] (sch_store1 without its wtilde(p,igp) factor), the band
-// frequencies wxb[n1][iw] and their [min,max]; buffers owned by the context.
+// S[p][n1], C[p][n1] (sch_store1 without its wtilde(p,igp) factor; the backward
+// reads it, the forward does not ask for it), the band frequencies wxb[n1][iw] and
+// their [min,max]; buffers owned by the context.
 namespace {
 
 struct PrepTables {
@@ -556,10 +557,10 @@ struct PrepTables {
   double *wxb, *range;
 };
 
-int prep_alloc(bgw_ctx* ctx, int nb, int ngpown, PrepTables* t) {
-  void *pS, *pC, *pwx, *prange;
+int prep_alloc(bgw_ctx* ctx, int nb, int ngpown, bool want_C, PrepTables* t) {
+  void *pS, *pC = nullptr, *pwx, *prange;
   int rc = pool_get(ctx, "ws_S", (size_t)nb * ngpown * 16, &pS);
-  if (!rc) rc = pool_get(ctx, "ws_C", (size_t)nb * ngpown * 16, &pC);
+  if (!rc && want_C) rc = pool_get(ctx, "ws_C", (size_t)nb * ngpown * 16, &pC);
   if (!rc) rc = pool_get(ctx, "ws_wxb", (size_t)nb * 3 * 8, &pwx);
   if (!rc) rc = pool_get(ctx, "ws_range", 64, &prange);
   if (rc) return rc;
@@ -648,10 +649,10 @@ int gpp_prep_tables(bgw_ctx* ctx, int nb, int ngpown, int ncouls, const int* inv
                     const double* wx, int wx_per_band, const double2* wt, const double2* am, const double2* an,
                     const double* vcoul, double2** S, double2** C, double** wxb, double** range) {
   PrepTables t;
-  int rc = prep_alloc(ctx, nb, ngpown, &t);
+  int rc = prep_alloc(ctx, nb, ngpown, C != nullptr, &t);
   if (rc) return rc;
   *S = t.S;
-  *C = t.C;
+  if (C) *C = t.C;
   *wxb = t.wxb;
   *range = t.range;
   prep_rows(ctx, t, nb, 0, ngpown, ncouls, inv, indinv, wx, wx_per_band, wt, am, an, vcoul);
@@ -674,9 +675,10 @@ int gpp_forward_device(bgw_ctx* ctx, size_t nb_, size_t nvband_, size_t ngpown_,
   int rc = fwd_begin(ctx, nb, ngpown, ncouls, &f);
   if (rc) return rc;
   launch_check_indices(inv, ngpown_, indinv, ncouls_ + 1, ncouls, ncouls, ctx->d_flag, ctx->stream);
-  double2 *S, *C;
+  double2* S;
   double *wxb, *range;
-  rc = gpp_prep_tables(ctx, nb, ngpown, ncouls, inv, indinv, wx, wx_per_band, wt, am, an, vcoul, &S, &C, &wxb, &range);
+  rc = gpp_prep_tables(ctx, nb, ngpown, ncouls, inv, indinv, wx, wx_per_band, wt, am, an, vcoul, &S, nullptr, &wxb,
+                       &range);
   if (rc) return rc;
   rc = run_main(ctx, f.cfg, wt, I, S, wxb, range, nb, ngpown, ncouls, f.part, &f.W);
   if (rc) return rc;
@@ -724,7 +726,7 @@ int gpp_forward_staged(bgw_ctx* ctx, int nb, int nvband, int ngpown, int ncouls,
   if ((rc = fwd_begin(ctx, nb, ngpown, ncouls, &f))) return rc;
   launch_check_indices(d.inv, ngpown, d.indinv, (size_t)ncouls + 1, ncouls, ncouls, ctx->d_flag, ctx->stream);
   PrepTables t;
-  if ((rc = prep_alloc(ctx, nb, ngpown, &t))) return rc;
+  if ((rc = prep_alloc(ctx, nb, ngpown, false, &t))) return rc;
   const double2* wt = reinterpret_cast<const double2*>(d.wtilde);
   const double2* I = reinterpret_cast<const double2*>(d.I_eps);
   const double2* am = reinterpret_cast<const double2*>(d.aqsm);

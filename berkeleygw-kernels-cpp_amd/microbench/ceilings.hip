@@ -1,19 +1,28 @@
-// ceilings.hip -- issue-rate ceilings of the instruction forms the GPP and FF
-// kernels are built from, measured on the device: FP64/FP32 FMA and MUL,
-// v_rcp_f64/f32, and v_mfma_f64_16x16x4f64 at one and two waves per SIMD.
-// Each kernel runs 8 independent dependency chains per lane (register-only work),
-// 256-thread blocks; the rate is printed in wave instructions per CU per ns and,
-// for the arithmetic forms, in TFLOP/s.
-// A second group pins the operands to fixed registers with inline asm (VALU and
-// MFMA instructions only, no memory access) to isolate operand forms: FP32 FMAs
-// whose source VGPRs share a register bank (index mod 4) or include an SGPR, and
-// VALU FMAs interleaved with MFMAs of the same precision (do the two overlap?).
+// ceilings.hip -- measured ceilings of the instruction forms and memory streams the
+// GPP and FF kernels are built from.  Not part of the library: a measurement tool.
 //
-//   build/ceilings            (make -C berkeleygw-kernels-cpp_amd build/ceilings)
+//   valu   FP64/FP32 FMA and MUL, v_rcp_f64/f32: 8 independent dependency chains per
+//          lane (register-only work), 256-thread blocks, 8 blocks per CU; printed in
+//          wave instructions per CU per ns and, for the arithmetic forms, TFLOP/s.
+//   fixed  operands pinned to fixed registers with inline asm (VALU and MFMA only, no
+//          memory access): FP32 FMAs whose source VGPRs share a register bank or include
+//          an SGPR, and VALU FMAs interleaved with MFMAs of the same precision.
+//   mfma   v_mfma_f64_4x4x4f64 and v_mfma_f64_16x16x4f64 streams: 1..8 waves per SIMD
+//          x 4, 8 or 16 independent accumulators per wave, up to saturation.
+//   hbm    read stream of 4 GiB (16-byte loads, plain and non-temporal, several grid
+//          sizes): the attainable HBM read bandwidth.
+//   dual   FP32 VALU dual issue: independent vs dependent streams, transcendental and
+//          FP64 instructions mixed in, at 1..8 waves per SIMD.
+//   copy   one 1 GiB device-to-device copy by a plain kernel (16-byte loads and
+//          stores): the reference dispatch for scaling rocprofv3 FETCH_SIZE/WRITE_SIZE.
+//
+//   make -C berkeleygw-kernels-cpp_amd ceilings
+//   build/ceilings [valu|fixed|mfma|dual|hbm|copy ...]      (no argument: all but copy)
 #include <hip/hip_runtime.h>
 
 #include <cstdio>
 #include <cstdlib>
+#include <cstring>
 
 namespace {
 
@@ -74,19 +83,74 @@ __global__ __launch_bounds__(256) void k_valu(const double* __restrict__ in, dou
   }
 }
 
+template <int NACC>
 __global__ __launch_bounds__(256) void k_mfma_f64(const double* __restrict__ in, double* __restrict__ out, int iters) {
   const double a = 1.0 + threadIdx.x * 1e-9 + in[0], b = 0.999999 + in[1];
-  f64x4 acc[8];
+  f64x4 acc[NACC];
 #pragma unroll
-  for (int j = 0; j < 8; ++j) acc[j] = (f64x4){0.0, 0.0, 0.0, 0.0};
+  for (int j = 0; j < NACC; ++j) acc[j] = (f64x4){0.0, 0.0, 0.0, 0.0};
   for (int i = 0; i < iters; ++i) {
 #pragma unroll
-    for (int j = 0; j < 8; ++j) acc[j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc[j], 0, 0, 0);
+    for (int j = 0; j < NACC; ++j) acc[j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc[j], 0, 0, 0);
   }
   double s = 0.0;
 #pragma unroll
-  for (int j = 0; j < 8; ++j) s += acc[j][0] + acc[j][1] + acc[j][2] + acc[j][3];
+  for (int j = 0; j < NACC; ++j) s += acc[j][0] + acc[j][1] + acc[j][2] + acc[j][3];
   out[blockIdx.x * 256 + threadIdx.x] = s;
+}
+
+// v_mfma_f64_4x4x4f64 (four 4x4x4 blocks per wave, 512 FLOP) stream
+template <int NACC>
+__global__ __launch_bounds__(256) void k_mfma4_f64(const double* __restrict__ in, double* __restrict__ out, int iters) {
+  const double a = 1.0 + threadIdx.x * 1e-9 + in[0], b = 0.999999 + in[1];
+  double acc[NACC];
+#pragma unroll
+  for (int j = 0; j < NACC; ++j) acc[j] = 0.0;
+  for (int i = 0; i < iters; ++i) {
+#pragma unroll
+    for (int j = 0; j < NACC; ++j) acc[j] = __builtin_amdgcn_mfma_f64_4x4x4f64(a, b, acc[j], 0, 0, 0);
+  }
+  double s = 0.0;
+#pragma unroll
+  for (int j = 0; j < NACC; ++j) s += acc[j];
+  out[blockIdx.x * 256 + threadIdx.x] = s;
+}
+
+// Read stream: every element of p[0, n) is read once (grid-stride, 4 loads of 16 B
+// in flight per lane); one sum per thread is written so the loads stay live.
+template <bool NONTEMPORAL>
+__global__ __launch_bounds__(256) void k_read(const double2* __restrict__ p, size_t n, double* __restrict__ out) {
+  double s0 = 0.0, s1 = 0.0;
+  const size_t stride = (size_t)gridDim.x * 1024;
+  for (size_t i = (size_t)blockIdx.x * 1024 + threadIdx.x; i < n; i += stride) {
+    double2 v[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const size_t k = i + (size_t)u * 256;
+      if (k < n) {
+        if (NONTEMPORAL) {
+          v[u].x = __builtin_nontemporal_load(&p[k].x);
+          v[u].y = __builtin_nontemporal_load(&p[k].y);
+        } else {
+          v[u] = p[k];
+        }
+      } else {
+        v[u] = make_double2(0.0, 0.0);
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      s0 += v[u].x;
+      s1 += v[u].y;
+    }
+  }
+  out[(size_t)blockIdx.x * 256 + threadIdx.x] = s0 + s1;
+}
+
+// Plain copy dst[0, n) = src[0, n), grid-stride, 16-byte loads and stores.
+__global__ __launch_bounds__(256) void k_copy(const double2* __restrict__ src, double2* __restrict__ dst, size_t n) {
+  const size_t stride = (size_t)gridDim.x * 256;
+  for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += stride) dst[i] = src[i];
 }
 
 // ---- fixed-register forms.  Chains live in v16..v31, operands in v40..v47 and s20,
@@ -107,6 +171,10 @@ enum Fixed {
   kF32Valu,       // 8 v_fma_f32
   kF32Mfma,       // 2 v_mfma_f32_4x4x1_16b_f32
   kF32Mix,        // interleaved
+  kF32Chain1,     // 32 v_fma_f32 in one dependency chain
+  kF32Chain2,     // two chains, alternating
+  kF32Rcp,        // per group of 8: 7 independent v_fma_f32 + 1 v_rcp_f32
+  kF32F64,        // per group of 8: 7 independent v_fma_f32 + 1 v_fma_f64
 };
 
 #define F32_DISTINCT                                                                   \
@@ -141,6 +209,19 @@ enum Fixed {
 #define F32_D "v_fma_f32 v22, v40, v22, v41\n v_fma_f32 v23, v42, v23, v43\n"
 #define M32_0 "v_mfma_f32_4x4x1_16b_f32 a[0:3], v44, v45, a[0:3]\n"
 #define M32_1 "v_mfma_f32_4x4x1_16b_f32 a[4:7], v44, v45, a[4:7]\n"
+#define F32_CHAIN1_1 "v_fma_f32 v16, v40, v16, v41\n"
+#define F32_CHAIN1 F32_CHAIN1_1 F32_CHAIN1_1 F32_CHAIN1_1 F32_CHAIN1_1 F32_CHAIN1_1 F32_CHAIN1_1 F32_CHAIN1_1 F32_CHAIN1_1
+#define F32_CHAIN2_2 "v_fma_f32 v16, v40, v16, v41\n v_fma_f32 v17, v42, v17, v43\n"
+#define F32_CHAIN2 F32_CHAIN2_2 F32_CHAIN2_2 F32_CHAIN2_2 F32_CHAIN2_2
+#define F32_SEVEN                                                                      \
+  "v_fma_f32 v16, v41, v46, v16\n v_fma_f32 v17, v42, v47, v17\n"                     \
+  "v_fma_f32 v18, v43, v44, v18\n v_fma_f32 v19, v40, v45, v19\n"                     \
+  "v_fma_f32 v20, v41, v46, v20\n v_fma_f32 v21, v42, v47, v21\n"                     \
+  "v_fma_f32 v22, v43, v44, v22\n"
+// v24 stays 1.0 under v_rcp_f32; its next use is 8 instructions later (past the
+// transcendental-use hazard)
+#define F32_RCP F32_SEVEN "v_rcp_f32 v24, v24\n"
+#define F32_F64 F32_SEVEN "v_fma_f64 v[28:29], v[40:41], v[28:29], v[42:43]\n"
 #define X4(G) G G G G
 
 // Operand values come from `seed` (1.0f - 2^-24 as bits of each 32-bit half), so
@@ -172,6 +253,10 @@ __global__ __launch_bounds__(256) void k_fixed(float* __restrict__ out, int iter
     if (F == kF32Valu) asm volatile(X4(F32_A F32_B F32_C F32_D) ::: FIXED_CLOBBERS);
     if (F == kF32Mfma) asm volatile(X4(M32_0 M32_1) ::: FIXED_CLOBBERS);
     if (F == kF32Mix) asm volatile(X4(F32_A M32_0 F32_B F32_C M32_1 F32_D) ::: FIXED_CLOBBERS);
+    if (F == kF32Chain1) asm volatile(X4(F32_CHAIN1) ::: FIXED_CLOBBERS);
+    if (F == kF32Chain2) asm volatile(X4(F32_CHAIN2) ::: FIXED_CLOBBERS);
+    if (F == kF32Rcp) asm volatile(X4(F32_RCP) ::: FIXED_CLOBBERS);
+    if (F == kF32F64) asm volatile(X4(F32_F64) ::: FIXED_CLOBBERS);
   }
   float r;
   asm volatile("s_nop 7\n s_nop 7\n v_mov_b32 %0, v16" : "=v"(r) : : FIXED_CLOBBERS);
@@ -197,80 +282,192 @@ float time_ms(F launch) {
   return ms;
 }
 
+bool want(int argc, char** argv, const char* mode) {
+  if (argc < 2) return std::strcmp(mode, "copy") != 0;
+  for (int i = 1; i < argc; ++i)
+    if (std::strcmp(argv[i], mode) == 0) return true;
+  return false;
+}
+
+template <int NACC>
+void mfma_sweep(int cus, const double* in, double* out) {
+  const int miters = 4000;
+  int max_per_cu = 0;
+  HIP_OK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&max_per_cu, k_mfma_f64<NACC>, 256, 0));
+  for (int per_cu = 1; per_cu <= 8; ++per_cu) {
+    if (per_cu > max_per_cu) {
+      std::printf("v_mfma_f64_16x16x4f64, %2d accumulators, %d wave(s)/SIMD: not resident (max %d)\n", NACC,
+                  per_cu, max_per_cu);
+      continue;
+    }
+    const int blocks = cus * per_cu;
+    const float ms =
+        time_ms([&] { hipLaunchKernelGGL(k_mfma_f64<NACC>, dim3(blocks), dim3(256), 0, 0, in, out, miters); });
+    const double flop = (double)blocks * 4 * miters * NACC * 2048.0;  // 4 waves per block, 16x16x4x2 per MFMA
+    std::printf("v_mfma_f64_16x16x4f64, %2d accumulators, %d wave(s)/SIMD %8.3f ms  %6.2f TFLOP/s\n", NACC, per_cu,
+                ms, flop / ms / 1e9);
+  }
+}
+
+template <int NACC>
+void mfma4_sweep(int cus, const double* in, double* out) {
+  const int miters = 16000;
+  int max_per_cu = 0;
+  HIP_OK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&max_per_cu, k_mfma4_f64<NACC>, 256, 0));
+  for (int per_cu = 1; per_cu <= 8; per_cu *= 2) {
+    if (per_cu > max_per_cu) continue;
+    const int blocks = cus * per_cu;
+    const float ms =
+        time_ms([&] { hipLaunchKernelGGL(k_mfma4_f64<NACC>, dim3(blocks), dim3(256), 0, 0, in, out, miters); });
+    const double flop = (double)blocks * 4 * miters * NACC * 512.0;
+    std::printf("v_mfma_f64_4x4x4f64,   %2d accumulators, %d wave(s)/SIMD %8.3f ms  %6.2f TFLOP/s\n", NACC, per_cu,
+                ms, flop / ms / 1e9);
+  }
+}
+
 }  // namespace
 
-int main() {
+int main(int argc, char** argv) {
   hipDeviceProp_t pr;
   HIP_OK(hipGetDeviceProperties(&pr, 0));
   const int cus = pr.multiProcessorCount;
-  const int max_blocks = cus * 8;
+  const int max_blocks = cus * 16;
   double *in, *out;
   HIP_OK(hipMalloc(&in, 32 * sizeof(double)));
   HIP_OK(hipMemset(in, 0, 32 * sizeof(double)));
   HIP_OK(hipMalloc(&out, (size_t)max_blocks * 256 * sizeof(double)));
   std::printf("device %s, %d CUs\n", pr.name, cus);
 
-  // VALU forms: 8 blocks per CU (8 waves per SIMD), 8 chains per lane
-  struct V {
-    const char* name;
-    void (*k)(const double*, double*, int);
-    double flop_per_lane_instr;
-  };
-  const V valu[] = {
-      {"v_fma_f64  (3 VGPR operands)", k_valu<kFmaF64>, 2.0},
-      {"v_mul_f64", k_valu<kMulF64>, 1.0},
-      {"v_rcp_f64", k_valu<kRcpF64>, 0.0},
-      {"v_fma_f32  (3 VGPR operands)", k_valu<kFmaF32Vgpr3>, 2.0},
-      {"v_fma_f32  (2 VGPR + literal)", k_valu<kFmaF32Vgpr2>, 2.0},
-      {"v_mul_f32", k_valu<kMulF32>, 1.0},
-      {"v_rcp_f32", k_valu<kRcpF32>, 0.0},
-  };
-  const int iters = 20000;
-  for (const V& v : valu) {
-    const float ms = time_ms([&] { hipLaunchKernelGGL(v.k, dim3(max_blocks), dim3(256), 0, 0, in, out, iters); });
-    const double lane_instr = (double)max_blocks * 256 * iters * 8;
-    const double wave_per_cu_ns = lane_instr / 64 / ((double)ms * 1e6) / cus;
-    std::printf("%-30s %8.3f ms  %6.3f wave-instr/CU/ns", v.name, ms, wave_per_cu_ns);
-    if (v.flop_per_lane_instr > 0) std::printf("  %6.2f TFLOP/s", lane_instr * v.flop_per_lane_instr / ms / 1e9);
-    std::printf("\n");
-  }
-  // FP64 MFMA: 1 and 2 blocks per CU (1 and 2 waves per SIMD), 8 accumulators per wave
-  const int miters = 4000;
-  for (int per_cu = 1; per_cu <= 2; ++per_cu) {
-    const int blocks = cus * per_cu;
-    const float ms = time_ms([&] { hipLaunchKernelGGL(k_mfma_f64, dim3(blocks), dim3(256), 0, 0, in, out, miters); });
-    const double flop = (double)blocks * 4 * miters * 8 * 2048.0;  // 4 waves per block, 16x16x4x2 per MFMA
-    std::printf("v_mfma_f64_16x16x4f64, %d wave(s)/SIMD %8.3f ms  %6.2f TFLOP/s\n", per_cu, ms, flop / ms / 1e9);
-  }
-  // fixed-register forms, 8 blocks per CU (8 waves per SIMD); per iteration 32 VALU
-  // instructions, or 8 MFMAs, or both interleaved
-  struct FX {
-    const char* name;
-    void (*k)(float*, int, float);
-  };
-  const FX fixed[] = {
-      {"v_fma_f32  fixed regs, 3 banks", k_fixed<kF32Distinct>},
-      {"v_fma_f32  acc + src one bank", k_fixed<kF32SameBank>},
-      {"v_fma_f32  two srcs one bank", k_fixed<kF32SrcPair>},
-      {"v_fma_f32  VGPR * SGPR + acc", k_fixed<kF32Sgpr>},
-      {"8 v_fma_f64", k_fixed<kF64Valu>},
-      {"2 v_mfma_f64_4x4x4_4b", k_fixed<kF64Mfma>},
-      {"8 v_fma_f64 + 2 mfma_f64 mixed", k_fixed<kF64Mix>},
-      {"8 v_fma_f32", k_fixed<kF32Valu>},
-      {"2 v_mfma_f32_4x4x1_16b", k_fixed<kF32Mfma>},
-      {"8 v_fma_f32 + 2 mfma_f32 mixed", k_fixed<kF32Mix>},
-  };
-  const int fiters = 5000;
-  float* fout = reinterpret_cast<float*>(out);
-  for (int f = 0; f < 10; ++f) {
-    const FX& v = fixed[f];
-    const float ms = time_ms([&] { hipLaunchKernelGGL(v.k, dim3(max_blocks), dim3(256), 0, 0, fout, fiters, 0.99999994f); });
-    if (f < 4) {
-      const double winstr = (double)max_blocks * 4 * fiters * 32;
-      std::printf("%-32s %8.3f ms  %6.3f wave-instr/CU/ns\n", v.name, ms, winstr / ((double)ms * 1e6) / cus);
-    } else {
-      std::printf("%-32s %8.3f ms per %d iterations\n", v.name, ms, fiters);
+  if (want(argc, argv, "valu")) {
+    // VALU forms: 8 blocks per CU (8 waves per SIMD), 8 chains per lane
+    struct V {
+      const char* name;
+      void (*k)(const double*, double*, int);
+      double flop_per_lane_instr;
+    };
+    const V valu[] = {
+        {"v_fma_f64  (3 VGPR operands)", k_valu<kFmaF64>, 2.0},
+        {"v_mul_f64", k_valu<kMulF64>, 1.0},
+        {"v_rcp_f64", k_valu<kRcpF64>, 0.0},
+        {"v_fma_f32  (3 VGPR operands)", k_valu<kFmaF32Vgpr3>, 2.0},
+        {"v_fma_f32  (2 VGPR + literal)", k_valu<kFmaF32Vgpr2>, 2.0},
+        {"v_mul_f32", k_valu<kMulF32>, 1.0},
+        {"v_rcp_f32", k_valu<kRcpF32>, 0.0},
+    };
+    const int iters = 20000;
+    const int blocks = cus * 8;
+    for (const V& v : valu) {
+      const float ms = time_ms([&] { hipLaunchKernelGGL(v.k, dim3(blocks), dim3(256), 0, 0, in, out, iters); });
+      const double lane_instr = (double)blocks * 256 * iters * 8;
+      const double wave_per_cu_ns = lane_instr / 64 / ((double)ms * 1e6) / cus;
+      std::printf("%-30s %8.3f ms  %6.3f wave-instr/CU/ns", v.name, ms, wave_per_cu_ns);
+      if (v.flop_per_lane_instr > 0) std::printf("  %6.2f TFLOP/s", lane_instr * v.flop_per_lane_instr / ms / 1e9);
+      std::printf("\n");
     }
+  }
+  if (want(argc, argv, "mfma")) {
+    mfma4_sweep<4>(cus, in, out);
+    mfma4_sweep<8>(cus, in, out);
+    mfma4_sweep<16>(cus, in, out);
+    mfma_sweep<4>(cus, in, out);
+    mfma_sweep<8>(cus, in, out);
+    mfma_sweep<16>(cus, in, out);
+  }
+  if (want(argc, argv, "fixed")) {
+    // fixed-register forms, 8 blocks per CU (8 waves per SIMD); per iteration 32 VALU
+    // instructions, or 8 MFMAs, or both interleaved
+    struct FX {
+      const char* name;
+      void (*k)(float*, int, float);
+    };
+    const FX fixed[] = {
+        {"v_fma_f32  fixed regs, 3 banks", k_fixed<kF32Distinct>},
+        {"v_fma_f32  acc + src one bank", k_fixed<kF32SameBank>},
+        {"v_fma_f32  two srcs one bank", k_fixed<kF32SrcPair>},
+        {"v_fma_f32  VGPR * SGPR + acc", k_fixed<kF32Sgpr>},
+        {"8 v_fma_f64", k_fixed<kF64Valu>},
+        {"2 v_mfma_f64_4x4x4_4b", k_fixed<kF64Mfma>},
+        {"8 v_fma_f64 + 2 mfma_f64 mixed", k_fixed<kF64Mix>},
+        {"8 v_fma_f32", k_fixed<kF32Valu>},
+        {"2 v_mfma_f32_4x4x1_16b", k_fixed<kF32Mfma>},
+        {"8 v_fma_f32 + 2 mfma_f32 mixed", k_fixed<kF32Mix>},
+    };
+    const int fiters = 5000;
+    const int blocks = cus * 8;
+    float* fout = reinterpret_cast<float*>(out);
+    for (int f = 0; f < 10; ++f) {
+      const FX& v = fixed[f];
+      const float ms =
+          time_ms([&] { hipLaunchKernelGGL(v.k, dim3(blocks), dim3(256), 0, 0, fout, fiters, 0.99999994f); });
+      if (f < 4) {
+        const double winstr = (double)blocks * 4 * fiters * 32;
+        std::printf("%-32s %8.3f ms  %6.3f wave-instr/CU/ns\n", v.name, ms, winstr / ((double)ms * 1e6) / cus);
+      } else {
+        std::printf("%-32s %8.3f ms per %d iterations\n", v.name, ms, fiters);
+      }
+    }
+  }
+  if (want(argc, argv, "dual")) {
+    // VALU dual issue: the same fixed-register groups at 1, 2, 4 and 8 waves per SIMD.
+    // Two VALU instructions issue in one quad-cycle only if they are independent; the
+    // rates at 1 wave/SIMD tell whether the pair may come from one wave.
+    struct FX {
+      const char* name;
+      void (*k)(float*, int, float);
+    };
+    const FX forms[] = {
+        {"8 independent v_fma_f32", k_fixed<kF32Distinct>},
+        {"1 chain of v_fma_f32", k_fixed<kF32Chain1>},
+        {"2 alternating chains", k_fixed<kF32Chain2>},
+        {"7 v_fma_f32 + 1 v_rcp_f32", k_fixed<kF32Rcp>},
+        {"7 v_fma_f32 + 1 v_fma_f64", k_fixed<kF32F64>},
+    };
+    const int fiters = 5000;
+    float* fout = reinterpret_cast<float*>(out);
+    for (const FX& v : forms)
+      for (int per_cu = 1; per_cu <= 8; per_cu *= 2) {
+        const int blocks = cus * per_cu;
+        const float ms =
+            time_ms([&] { hipLaunchKernelGGL(v.k, dim3(blocks), dim3(256), 0, 0, fout, fiters, 0.99999994f); });
+        const double winstr = (double)blocks * 4 * fiters * 32;
+        std::printf("dual: %-28s %d wave(s)/SIMD %8.3f ms  %6.3f wave-instr/CU/ns\n", v.name, per_cu, ms,
+                    winstr / ((double)ms * 1e6) / cus);
+      }
+  }
+  if (want(argc, argv, "hbm")) {
+    const size_t bytes = size_t(4) << 30;
+    const size_t n = bytes / sizeof(double2);
+    double2* buf;
+    HIP_OK(hipMalloc(&buf, bytes));
+    HIP_OK(hipMemset(buf, 0, bytes));
+    for (int nt = 0; nt < 2; ++nt)
+      for (int per_cu : {4, 8, 16}) {
+        const int blocks = cus * per_cu;
+        const float ms = time_ms([&] {
+          if (nt)
+            hipLaunchKernelGGL(k_read<true>, dim3(blocks), dim3(256), 0, 0, buf, n, out);
+          else
+            hipLaunchKernelGGL(k_read<false>, dim3(blocks), dim3(256), 0, 0, buf, n, out);
+        });
+        std::printf("read stream %s, %2d blocks/CU: 4 GiB in %8.3f ms  %6.3f TB/s\n", nt ? "non-temporal" : "plain",
+                    per_cu, ms, bytes / (ms * 1e-3) / 1e12);
+      }
+    HIP_OK(hipFree(buf));
+  }
+  if (want(argc, argv, "copy")) {
+    const size_t bytes = size_t(1) << 30;
+    const size_t n = bytes / sizeof(double2);
+    double2 *src, *dst;
+    HIP_OK(hipMalloc(&src, bytes));
+    HIP_OK(hipMalloc(&dst, bytes));
+    HIP_OK(hipMemset(src, 0, bytes));
+    HIP_OK(hipDeviceSynchronize());
+    hipLaunchKernelGGL(k_copy, dim3(cus * 8), dim3(256), 0, 0, src, dst, n);
+    HIP_OK(hipGetLastError());
+    HIP_OK(hipDeviceSynchronize());
+    std::printf("copy: %zu bytes read and %zu written by k_copy\n", bytes, bytes);
+    HIP_OK(hipFree(src));
+    HIP_OK(hipFree(dst));
   }
   HIP_OK(hipDeviceSynchronize());
   HIP_OK(hipFree(in));

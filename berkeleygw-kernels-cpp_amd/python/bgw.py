@@ -147,41 +147,47 @@ class _Hold:
     """The array arguments of one foreign call, in the layout and dtype the C ABI reads
     (C-contiguous int32, float64 or complex128).  A converted copy must outlive the call,
     so the holder keeps every array it returns a pointer to; it lives as long as the
-    calling method."""
+    calling method.
+
+    The C ABI mirrors the reference's pointer-only signatures and cannot see array
+    lengths, so each argument is given the number of elements the call will read (`n`)
+    and a shorter array raises ValueError here, before any foreign call."""
 
     def __init__(self):
         self._keep = []
 
-    def _ptr(self, a):
+    def _ptr(self, a, n, name):
+        if n is not None and a.size < n:
+            raise ValueError(f"{name}: {a.size} elements, the call reads {n}")
         self._keep.append(a)
         return a.ctypes.data
 
-    def i32(self, a):
+    def i32(self, a, n=None, name="index array"):
         if a is None:
             return None
         a = np.asarray(a)
         if not np.issubdtype(a.dtype, np.integer):
-            raise TypeError(f"expected an integer index array, got {a.dtype}")
+            raise TypeError(f"{name}: expected an integer index array, got {a.dtype}")
         b = np.ascontiguousarray(a, np.int32)
         if b.dtype != a.dtype and not np.array_equal(b, a):
-            raise ValueError("index array has values outside the int32 range")
-        return self._ptr(b)
+            raise ValueError(f"{name}: values outside the int32 range")
+        return self._ptr(b, n, name)
 
-    def f64(self, a):
+    def f64(self, a, n=None, name="real array"):
         if a is None:
             return None
         a = np.asarray(a)
         if np.iscomplexobj(a):
-            raise TypeError(f"expected a real array, got {a.dtype}")
-        return self._ptr(np.ascontiguousarray(a, np.float64))
+            raise TypeError(f"{name}: expected a real array, got {a.dtype}")
+        return self._ptr(np.ascontiguousarray(a, np.float64), n, name)
 
-    def c128(self, a):
+    def c128(self, a, n=None, name="complex array"):
         if a is None:
             return None
         a = np.asarray(a)
         if not np.iscomplexobj(a):
-            raise TypeError(f"expected a complex array, got {a.dtype}")
-        return self._ptr(np.ascontiguousarray(a, np.complex128))
+            raise TypeError(f"{name}: expected a complex array, got {a.dtype}")
+        return self._ptr(np.ascontiguousarray(a, np.complex128), n, name)
 
 
 # ----------------------------------------------------------------- synthetic inputs
@@ -429,6 +435,25 @@ def load_inputs(path, verify: bool = True) -> dict:
         return s.to_dict()
 
 
+def _gpp_arrays(h: _Hold, d: dict):
+    """The eight array arguments of the extended GPP entry points, length-checked."""
+    nb, ng, nc = d["nb"], d["ngpown"], d["ncouls"]
+    return (h.i32(d["inv"], ng, "inv_igp_index"), h.i32(d["indinv"], nc + 1, "indinv"),
+            h.f64(d["wx"], 3 * nb, "wx_array"), h.c128(d["wtilde"], ng * nc, "wtilde_array"),
+            h.c128(d["aqsm"], nb * nc, "aqsmtemp"), h.c128(d["aqsn"], nb * nc, "aqsntemp"),
+            h.c128(d["I_eps"], ng * nc, "I_eps_array"), h.f64(d["vcoul"], nc, "vcoul"))
+
+
+def _ff_arrays(h: _Hold, f: dict, band_rows: int, slices: int):
+    """Length-checked FF arrays: band arrays of `band_rows` rows, `slices` frequency slices."""
+    ng, nc = f["ngpown"], f["ncouls"]
+    return dict(inv=h.i32(f["inv"], ng, "inv_igp_index"), indinv=h.i32(f["indinv"], nc, "indinv"),
+                aqsm=h.c128(f["aqsm"], band_rows * nc, "aqsmtemp"),
+                aqsn=h.c128(f["aqsn"], band_rows * nc, "aqsntemp"),
+                vcoul=h.f64(f["vcoul"], nc, "vcoul"), R=h.c128(f["I_epsR"], slices * ng * nc, "I_epsR_array"),
+                A=h.c128(f.get("I_epsA"), slices * ng * nc, "I_epsA_array"))
+
+
 # ----------------------------------------------------------------------- context
 class Context:
     """One GPU + one stream (bgw_ctx)."""
@@ -492,9 +517,12 @@ class Context:
     def noflagOCC_solver(self, nb, ngpown, ncouls, inv, indinv, wx3, wtilde, aqsm, aqsn, I_eps, vcoul):
         ach = np.zeros(3, np.complex128)
         h = _Hold()
+        npg, nbg = ngpown * ncouls, nb * ncouls
         self._check(lib().bgw_noflagOCC_solver(
-            self._h, nb, ngpown, ncouls, h.i32(inv), h.i32(indinv), h.f64(wx3), h.c128(wtilde), h.c128(aqsm),
-            h.c128(aqsn), h.c128(I_eps), h.f64(vcoul), _ptr(ach)))
+            self._h, nb, ngpown, ncouls, h.i32(inv, ngpown, "inv_igp_index"), h.i32(indinv, ncouls + 1, "indinv"),
+            h.f64(wx3, 3, "wx_array"), h.c128(wtilde, npg, "wtilde_array"), h.c128(aqsm, nbg, "aqsmtemp"),
+            h.c128(aqsn, nbg, "aqsntemp"), h.c128(I_eps, npg, "I_eps_array"), h.f64(vcoul, ncouls, "vcoul"),
+            _ptr(ach)))
         return ach
 
     def gpp_solver(self, d: dict, occ: float = 1.0, outputs=("achtemp", "acht_n1_loc", "achstemp", "asxtemp")):
@@ -508,10 +536,8 @@ class Context:
             res["asxtemp"] = np.zeros(3, np.complex128)
         h = _Hold()
         self._check(lib().bgw_gpp_solver(
-            self._h, nb, d["nvband"], d["ngpown"], d["ncouls"], h.i32(d["inv"]), h.i32(d["indinv"]),
-            h.f64(d["wx"]), h.c128(d["wtilde"]), h.c128(d["aqsm"]), h.c128(d["aqsn"]), h.c128(d["I_eps"]),
-            h.f64(d["vcoul"]), occ, _ptr(res["achtemp"]), _ptr(res.get("acht_n1_loc")), _ptr(res.get("achstemp")),
-            _ptr(res.get("asxtemp"))))
+            self._h, nb, d["nvband"], d["ngpown"], d["ncouls"], *_gpp_arrays(h, d), occ, _ptr(res["achtemp"]),
+            _ptr(res.get("acht_n1_loc")), _ptr(res.get("achstemp")), _ptr(res.get("asxtemp"))))
         if "acht_n1_loc" in res:
             res["acht_n1_loc"] = res["acht_n1_loc"].reshape(nb, 3)
         return res
@@ -523,9 +549,8 @@ class Context:
         dI = np.zeros(npg, np.complex128)
         h = _Hold()
         self._check(lib().bgw_gpp_backward(
-            self._h, d["nb"], d["nvband"], d["ngpown"], d["ncouls"], h.i32(d["inv"]), h.i32(d["indinv"]),
-            h.f64(d["wx"]), h.c128(d["wtilde"]), h.c128(d["aqsm"]), h.c128(d["aqsn"]), h.c128(d["I_eps"]),
-            h.f64(d["vcoul"]), occ, _ptr(ds), _ptr(dw), _ptr(dI)))
+            self._h, d["nb"], d["nvband"], d["ngpown"], d["ncouls"], *_gpp_arrays(h, d), occ, _ptr(ds), _ptr(dw),
+            _ptr(dI)))
         return {"dsigma_domega": ds, "dL_dwtilde": dw, "dL_dI_eps": dI}
 
     # ------------------------------------------------------------- GPP (device)
@@ -543,29 +568,32 @@ class Context:
     def achsDtemp_Kernel(self, f: dict):
         out = np.zeros(1, np.complex128)
         h = _Hold()
+        a = _ff_arrays(h, dict(f, I_epsA=None), f["nb"], 2)  # slice 1 of I_epsR is read
         self._check(lib().bgw_achsDtemp_Kernel(
-            self._h, f["nb"], f["ngpown"], f["ncouls"], h.i32(f["inv"]), h.i32(f["indinv"]), h.c128(f["aqsn"]),
-            h.c128(f["aqsm"]), h.c128(f["I_epsR"]), h.f64(f["vcoul"]), _ptr(out)))
+            self._h, f["nb"], f["ngpown"], f["ncouls"], a["inv"], a["indinv"], a["aqsn"], a["aqsm"], a["R"],
+            a["vcoul"], _ptr(out)))
         return out[0]
 
     def asxDtemp_Kernel(self, f: dict):
         out = np.zeros(f["nfreqeval"], np.complex128)
         h = _Hold()
+        nv, nF = f["nvband"], f["nFreq"]
+        a = _ff_arrays(h, f, nv, nF)  # asxDtemp reads the valence rows n1 < nvband
         self._check(lib().bgw_asxDtemp_Kernel(
-            self._h, f["nvband"], f["nfreqeval"], f["ncouls"], f["ngpown"], f["nFreq"], f["freqevalmin"],
-            f["freqevalstep"], f["occ"], h.f64(f["ekq"]), h.f64(f["dFreqGrid"]), h.i32(f["inv"]), h.i32(f["indinv"]),
-            h.c128(f["aqsm"]), h.c128(f["aqsn"]), h.f64(f["vcoul"]), h.c128(f["I_epsR"]), h.c128(f["I_epsA"]),
-            _ptr(out)))
+            self._h, nv, f["nfreqeval"], f["ncouls"], f["ngpown"], nF, f["freqevalmin"], f["freqevalstep"], f["occ"],
+            h.f64(f["ekq"], nv, "ekq"), h.f64(f["dFreqGrid"], nF, "dFreqGrid"), a["inv"], a["indinv"], a["aqsm"],
+            a["aqsn"], a["vcoul"], a["R"], a["A"], _ptr(out)))
         return out
 
     def achDtemp_cor_Kernel(self, f: dict):
         out = np.zeros(f["nfreqeval"], np.complex128)
         h = _Hold()
+        nb, nF = f["nb"], f["nFreq"]
+        a = _ff_arrays(h, f, nb, nF)
         self._check(lib().bgw_achDtemp_cor_Kernel(
-            self._h, f["nb"], f["nvband"], f["nfreqeval"], f["ncouls"], f["ngpown"], f["nFreq"], f["freqevalmin"],
-            f["freqevalstep"], h.f64(f["ekq"]), h.f64(f["dFreqGrid"]), h.i32(f["inv"]), h.i32(f["indinv"]),
-            h.c128(f["aqsm"]), h.c128(f["aqsn"]), h.f64(f["vcoul"]), h.c128(f["I_epsR"]), h.c128(f["I_epsA"]),
-            _ptr(out)))
+            self._h, nb, f["nvband"], f["nfreqeval"], f["ncouls"], f["ngpown"], nF, f["freqevalmin"],
+            f["freqevalstep"], h.f64(f["ekq"], nb, "ekq"), h.f64(f["dFreqGrid"], nF, "dFreqGrid"), a["inv"],
+            a["indinv"], a["aqsm"], a["aqsn"], a["vcoul"], a["R"], a["A"], _ptr(out)))
         return out
 
     def schDttt_corKernel(self, kernel: int, f: dict, n1: int, ifreq: int, fact1: float, fact2: float) -> complex:
@@ -574,9 +602,9 @@ class Context:
         out = np.zeros(1, np.complex128)
         fn = getattr(lib(), f"bgw_schDttt_corKernel{int(kernel)}")
         h = _Hold()
-        self._check(fn(self._h, _ptr(out), h.i32(f["inv"]), h.i32(f["indinv"]), h.c128(f["I_epsR"]),
-                       h.c128(f["I_epsA"]), h.c128(f["aqsm"]), h.c128(f["aqsn"]), h.f64(f["vcoul"]), f["ncouls"],
-                       ifreq, f["ngpown"], n1, fact1, fact2))
+        a = _ff_arrays(h, f, n1 + 1, ifreq + 2)  # the call reads row n1 and slices ifreq, ifreq + 1
+        self._check(fn(self._h, _ptr(out), a["inv"], a["indinv"], a["R"], a["A"], a["aqsm"], a["aqsn"], a["vcoul"],
+                       f["ncouls"], ifreq, f["ngpown"], n1, fact1, fact2))
         return out[0]
 
     def schDttt_corKernel_device(self, kernel: int, f: dict, ptrs: dict, n1: int, ifreq: int, fact1: float,

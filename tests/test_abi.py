@@ -37,6 +37,52 @@ def test_null_context_is_einval():
     assert L.bgw_gpp_solver(None, 1, 0, 1, 1, *([None] * 8), 1.0, None, None, None, None) == bgw.BGW_EINVAL
 
 
+def _unopened_context():
+    """A Context whose handle is null: every method that reaches the C ABI would get
+    BGW_EINVAL, so a ValueError from it can only come from the binding's own checks."""
+    ctx = bgw.Context.__new__(bgw.Context)
+    ctx._h = C.c_void_p()
+    return ctx
+
+
+@pytest.mark.parametrize("key,cut", [("I_eps", 1), ("wtilde", 17), ("aqsm", 1), ("aqsn", 5), ("vcoul", 1),
+                                     ("inv", 1), ("indinv", 1), ("wx", 3)])
+def test_short_gpp_arrays_are_rejected_before_the_call(key, cut):
+    d = bgw.gpp_inputs(4, 2, 64, 8, 11)
+    d[key] = d[key][:-cut]
+    ctx = _unopened_context()
+    with pytest.raises(ValueError, match="elements, the call reads"):
+        ctx.gpp_solver(d)
+    with pytest.raises(ValueError, match="elements, the call reads"):
+        ctx.gpp_backward(d)
+
+
+def test_short_ff_arrays_are_rejected_before_the_call():
+    f = bgw.ff_inputs(6, 2, 64, 8, 4, 2, 21)
+    ctx = _unopened_context()
+    for key, op in (("I_epsR", "achsDtemp_Kernel"), ("I_epsA", "asxDtemp_Kernel"), ("aqsm", "achDtemp_cor_Kernel"),
+                    ("ekq", "achDtemp_cor_Kernel"), ("dFreqGrid", "asxDtemp_Kernel")):
+        g = dict(f)
+        if key in ("I_epsR", "I_epsA"):
+            g[key] = g[key][: 8 * 64]  # one slice only
+        elif key == "aqsm":
+            g[key] = g[key][: 64 * 5]
+        else:
+            g[key] = g[key][:1]
+        with pytest.raises(ValueError, match="elements, the call reads"):
+            getattr(ctx, op)(g)
+    # schDttt_corKernel reads slices ifreq, ifreq + 1 and band row n1
+    with pytest.raises(ValueError, match="I_epsR_array"):
+        ctx.schDttt_corKernel(1, f, 2, 3, 0.5, 0.5)
+    with pytest.raises(ValueError, match="aqsmtemp"):
+        ctx.schDttt_corKernel(1, f, 6, 0, 0.5, 0.5)
+    # a short wx_array in the reference-signature solver
+    d = bgw.gpp_inputs(4, 2, 64, 8, 11)
+    with pytest.raises(ValueError, match="wx_array"):
+        ctx.noflagOCC_solver(4, 8, 64, d["inv"], d["indinv"], d["wx"][:2], d["wtilde"], d["aqsm"], d["aqsn"],
+                             d["I_eps"], d["vcoul"])
+
+
 def test_synth_is_deterministic_and_in_range():
     a = bgw.synth_complex(17, (0.5, 3.0), (-0.1, 0.0), 10000)
     b = bgw.synth_complex(17, (0.5, 3.0), (-0.1, 0.0), 10000)

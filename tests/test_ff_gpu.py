@@ -173,10 +173,14 @@ def test_band_kernels_device_form_and_validation(ctx):
         with pytest.raises(bgw.BgwError) as ei:
             ctx.schDttt_corKernel(1, f, args["n1"], args["ifreq"], 0.5, 0.5)
         assert ei.value.code == bgw.BGW_EINVAL
+    # ngpown > ncouls (FF/ff.cpp:492-499): the C ABI refuses it before it reads an array
     g = dict(f, ngpown=f["ncouls"] + 1)
     with pytest.raises(bgw.BgwError) as ei:
-        ctx.schDttt_corKernel(2, g, 0, 0, 0.5, 0.5)
+        ctx.schDttt_corKernel_device(2, g, ptrs, 0, 0, 0.5, 0.5, out.data_ptr())
     assert ei.value.code == bgw.BGW_EINVAL
+    # and the host form's binding refuses the arrays, shorter than such a call would read
+    with pytest.raises(ValueError, match="elements, the call reads"):
+        ctx.schDttt_corKernel(2, g, 0, 0, 0.5, 0.5)
     bad_inv = dict(f, inv=f["inv"].copy())
     bad_inv["inv"][3] = f["ncouls"]
     with pytest.raises(bgw.BgwError) as ei:
