@@ -18,14 +18,27 @@ int fail(bgw_ctx* ctx, int code, const std::string& msg) {
   return code;
 }
 
+int fail(bgw_ctx* ctx, int code, const char* msg) {
+  if (ctx) ctx->err.assign(msg);
+  return code;
+}
+
 int hip_fail(bgw_ctx* ctx, hipError_t e, const char* what) {
   std::string m = std::string(what) + ": " + hipGetErrorString(e);
   return fail(ctx, e == hipErrorOutOfMemory ? BGW_ENOMEM : BGW_EHIP, m);
 }
 
 int pool_get(bgw_ctx* ctx, const char* name, size_t bytes, void** out) {
-  DevBuf& b = ctx->pool[name];
   if (bytes == 0) bytes = 16;
+  if (ctx->arena) {
+    const size_t off = (ctx->arena_used + 255) & ~size_t(255);
+    if (off > ctx->arena_bytes || bytes > ctx->arena_bytes - off)
+      return fail(ctx, BGW_EINVAL, "workspace exceeds the scratch array of the allocation-free call");
+    *out = ctx->arena + off;
+    ctx->arena_used = off + bytes;
+    return BGW_OK;
+  }
+  DevBuf& b = ctx->pool[name];
   if (b.bytes < bytes) {
     if (b.ptr) {
       hipError_t e = hipStreamSynchronize(ctx->stream);
@@ -46,7 +59,8 @@ int pool_get(bgw_ctx* ctx, const char* name, size_t bytes, void** out) {
   return BGW_OK;
 }
 
-KernelTimer::KernelTimer(bgw_ctx* c, const char* name) : ctx(c), on(c->timing) {
+// no timing inside allocation-free calls (it creates events)
+KernelTimer::KernelTimer(bgw_ctx* c, const char* name) : ctx(c), on(c->timing && !c->arena) {
   if (!on) return;
   tl.name = name;
   for (hipEvent_t* ev : {&tl.start, &tl.stop}) {
@@ -282,6 +296,7 @@ int bgw_create(bgw_ctx** out, int device) {
   if (device < 0 || device >= n) return BGW_EINVAL;
   bgw_ctx* ctx = new bgw_ctx();
   ctx->device = device;
+  ctx->err.reserve(512);
   hipError_t e = hipSetDevice(device);
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&ctx->own_stream, hipStreamNonBlocking);
   if (e == hipSuccess) e = hipDeviceGetAttribute(&ctx->cus, hipDeviceAttributeMultiprocessorCount, device);
@@ -296,6 +311,10 @@ int bgw_create(bgw_ctx** out, int device) {
     bgw_destroy(ctx);  // releases whatever was created
     return code;
   }
+  // the allocation-free entry point's scratch array; a failure here only disables that
+  // entry point (it then returns the error), not the context
+  (void)gpp_scratch_attach(ctx);
+  ctx->err.clear();
   *out = ctx;
   return BGW_OK;
 }

@@ -58,15 +58,29 @@ struct bgw_ctx {
   // host buffers of the gathered aqsm / aqsn columns: a copy from them may still be
   // queued when the staging function returns, so they live until the next call
   std::vector<double2> gather_m, gather_n;
+  // device address of this device's static scratch array (allocation-free entry point),
+  // looked up once per device when the first context on it is created; null if unavailable
+  void* scratch = nullptr;
+  // while an allocation-free call runs, pool_get carves workspace from this arena (the
+  // scratch array) instead of the grow-only pools
+  char* arena = nullptr;
+  size_t arena_bytes = 0, arena_used = 0;
 };
 
 namespace bgw {
 
 int fail(bgw_ctx* ctx, int code, const std::string& msg);
+// Same, copying a C string into the context's message buffer (reserved at creation, so
+// short messages cost no host allocation).
+int fail(bgw_ctx* ctx, int code, const char* msg);
 int hip_fail(bgw_ctx* ctx, hipError_t e, const char* what);
 
-// Returns a device buffer of at least `bytes` from pool slot `name`.
+// Returns a device buffer of at least `bytes` from pool slot `name`, or, during an
+// allocation-free call, the next 256-byte-aligned piece of the context's arena.
 int pool_get(bgw_ctx* ctx, const char* name, size_t bytes, void** out);
+// Looks up (once per device, with ctx->device current) the device address of the
+// scratch array of the allocation-free entry point and records it in ctx->scratch.
+int gpp_scratch_attach(bgw_ctx* ctx);
 
 // Brackets a launch with HIP events when timing is on.
 struct KernelTimer {

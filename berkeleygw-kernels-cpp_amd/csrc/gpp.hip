@@ -16,6 +16,7 @@
 // acht_n1_loc and achtemp (run-to-run bitwise reproducible).
 #include <algorithm>
 #include <cmath>
+#include <mutex>
 #include <vector>
 
 #include "bgw_dd.h"
@@ -765,4 +766,70 @@ int gpp_forward_staged(bgw_ctx* ctx, int nb, int nvband, int ngpown, int ncouls,
                  d_loc, d_achs, d_asx);
 }
 
+// ------------------------------------------------------- allocation-free entry point
+// Workspace of bgw_gpp_solver_device_noalloc: a file-scope device array, one copy per
+// GPU for the life of the process.  Host code uses only the device address that
+// hipGetSymbolAddress returns for the current device (ctx->scratch), never the name.
+constexpr size_t kScratchBytes = size_t(64) << 20;
+static __device__ __align__(256) char scratch[kScratchBytes];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+size_t al256(size_t n) { return (n + 255) & ~size_t(255); }
+}  // namespace
+
+int gpp_scratch_attach(bgw_ctx* ctx) {
+  if (ctx->device < 0 || ctx->device >= kMaxDevices)
+    return fail(ctx, BGW_ENODEV, "device index beyond the scratch table");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[ctx->device]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));  // ctx->device is current
+    if (e != hipSuccess || !p)
+      return hip_fail(ctx, e != hipSuccess ? e : hipErrorInvalidSymbol, "hipGetSymbolAddress");
+    g_scratch_ptr[ctx->device] = p;
+  }
+  ctx->scratch = g_scratch_ptr[ctx->device];
+  return BGW_OK;
+}
+
 }  // namespace bgw
+
+// The forward's device workspace in the order gpp_forward_device requests it when only
+// achtemp and acht_n1_loc are computed: per-workgroup partials, band sums, S table, band
+// frequencies and their range, each 256-byte aligned.
+extern "C" int bgw_gpp_solver_device_noalloc(bgw_ctx* ctx, size_t number_bands, size_t ngpown, size_t ncouls,
+                                             const int* inv_igp_index, const int* indinv, const double* wx_array,
+                                             const double* wtilde_array, const double* aqsmtemp,
+                                             const double* aqsntemp, const double* I_eps_array, const double* vcoul,
+                                             double* achtemp, double* acht_n1_loc) {
+  using namespace bgw;
+  if (!ctx) return BGW_EINVAL;
+  if (!inv_igp_index || !indinv || !wx_array || !wtilde_array || !aqsmtemp || !aqsntemp || !I_eps_array || !vcoul ||
+      !achtemp)
+    return fail(ctx, BGW_EINVAL, "null array argument");
+  if (number_bands == 0 || ngpown == 0 || ncouls == 0) return fail(ctx, BGW_EINVAL, "zero size");
+  if (number_bands >= (size_t)INT32_MAX || ngpown >= (size_t)INT32_MAX || ncouls >= (size_t)INT32_MAX ||
+      ngpown > SIZE_MAX / 16 / ncouls || number_bands > SIZE_MAX / 16 / ncouls)
+    return fail(ctx, BGW_EINVAL, "size exceeds int range");
+  if (!ctx->scratch) return fail(ctx, BGW_EHIP, "the scratch array is not available on this device");
+  if (hipSetDevice(ctx->device) != hipSuccess) return fail(ctx, BGW_ENODEV, "hipSetDevice");
+  const int nb = (int)number_bands;
+  int W = 0;
+  run_main(ctx, pick_main(nb), nullptr, nullptr, nullptr, nullptr, nullptr, nb, (int)ngpown, (int)ncouls, nullptr, &W);
+  const double need = (double)al256((size_t)W * nb * 48) + (double)al256((size_t)nb * 96) +
+                      (double)al256(16 * number_bands * ngpown) + (double)al256((size_t)nb * 24) + 256.0;
+  if (need > (double)kScratchBytes)
+    return fail(ctx, BGW_EINVAL, "number_bands x ngpown too large for the 64 MiB scratch of the allocation-free call");
+  ctx->arena = static_cast<char*>(ctx->scratch);
+  ctx->arena_bytes = kScratchBytes;
+  ctx->arena_used = 0;
+  const int rc = gpp_forward_device(ctx, number_bands, 0, ngpown, ncouls, inv_igp_index, indinv, wx_array, 1,
+                                    wtilde_array, aqsmtemp, aqsntemp, I_eps_array, vcoul, 0.0, achtemp, acht_n1_loc,
+                                    nullptr, nullptr);
+  ctx->arena = nullptr;
+  return rc;
+}

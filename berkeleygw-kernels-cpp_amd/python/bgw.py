@@ -88,6 +88,7 @@ def _declare(L: C.CDLL) -> None:
     ext = [_P, _SZ, _SZ, _SZ, _SZ, _P, _P, _P, _P, _P, _P, _P, _P, _D, _P, _P, _P, _P]
     L.bgw_gpp_solver.argtypes = ext
     L.bgw_gpp_solver_device.argtypes = ext
+    L.bgw_gpp_solver_device_noalloc.argtypes = [_P, _SZ, _SZ, _SZ, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P]
     L.bgw_gpp_backward.argtypes = ext[:-1]
     L.bgw_gpp_backward_device.argtypes = ext[:-1]
     L.bgw_compute_fact.argtypes = [_D, _I, _P, C.POINTER(_D), C.POINTER(_D), C.POINTER(_I), _I, _I]
@@ -558,6 +559,13 @@ class Context:
                           occ, achtemp, acht_n1_loc=None, achstemp=None, asxtemp=None):
         self._check(lib().bgw_gpp_solver_device(self._h, nb, nvband, ngpown, ncouls, inv, indinv, wx, wtilde, aqsm,
                                                 aqsn, I_eps, vcoul, occ, achtemp, acht_n1_loc, achstemp, asxtemp))
+
+    def gpp_solver_device_noalloc(self, nb, ngpown, ncouls, inv, indinv, wx, wtilde, aqsm, aqsn, I_eps, vcoul, achtemp,
+                                  acht_n1_loc=None):
+        """bgw_gpp_solver_device_noalloc: achtemp (+ acht_n1_loc) on device arrays with no
+        device allocation (workspace from the library's per-GPU scratch array)."""
+        self._check(lib().bgw_gpp_solver_device_noalloc(self._h, nb, ngpown, ncouls, inv, indinv, wx, wtilde, aqsm,
+                                                        aqsn, I_eps, vcoul, achtemp, acht_n1_loc))
 
     def gpp_backward_device(self, nb, nvband, ngpown, ncouls, inv, indinv, wx, wtilde, aqsm, aqsn, I_eps, vcoul,
                             occ, dsigma=None, dwt=None, dI=None):

@@ -119,6 +119,25 @@ int bgw_gpp_solver_device(bgw_ctx* ctx, size_t number_bands, size_t nvband, size
                           const double* I_eps_array, const double* vcoul, double occ,
                           double* achtemp, double* acht_n1_loc, double* achstemp, double* asxtemp);
 
+/* Allocation-free form of bgw_gpp_solver_device for the noflagOCC_solver outputs
+ * (achtemp required, acht_n1_loc optional; wx_array[number_bands*3]), in the context's
+ * precision and on its stream.  No reference counterpart: it is for callers that cannot
+ * allocate device memory -- code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc.  It calls no allocating routine: all workspace comes from a
+ * 64 MiB device array of the library (one per GPU, for the life of the process), and a
+ * problem whose workspace does not fit returns BGW_EINVAL without launching anything (the
+ * benchmark workload, 512 bands x 1638 G', needs 37-49 MiB, depending on the precision).  Every call on a GPU
+ * shares that GPU's array, so calls that overlap in time on one GPU (two streams, or
+ * two contexts on one device) race; calls queued on one stream do not.  Results equal
+ * bgw_gpp_solver_device's bit for bit.  Bad indices are clamped and reported by the
+ * next bgw_synchronize, as for the other _device calls. */
+int bgw_gpp_solver_device_noalloc(bgw_ctx* ctx, size_t number_bands, size_t ngpown, size_t ncouls,
+                                  const int* inv_igp_index, const int* indinv,
+                                  const double* wx_array, const double* wtilde_array,
+                                  const double* aqsmtemp, const double* aqsntemp,
+                                  const double* I_eps_array, const double* vcoul,
+                                  double* achtemp, double* acht_n1_loc);
+
 /* Adjoint of bgw_gpp_solver for L = Re sum_iw (asxtemp[iw] + sum_n1 acht_n1_loc[n1][iw]).
  * Outputs (NULL = not computed):
  *   dsigma_domega c128[3]  d(achtemp[iw]+asxtemp[iw])/d omega, omega shifting every wx(n1,iw)
