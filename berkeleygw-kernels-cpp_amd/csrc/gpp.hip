@@ -196,14 +196,15 @@ __device__ inline void tile_exact(const TileEntry* __restrict__ tile, int n, con
   }
 }
 
-// FP32-bulk form: the per-(p,g) constants and the complex products in FP32,
-// sums over runs of 32 G-vectors in FP32 folded into the FP64 totals.
-// Used for a tile only when every denominator of it is far from the pole:
-// |wx - wr| >= max|wx|/4 for all bands, so u = wx - wr loses at most 2 bits in
-// FP32, and every d in [2^-20, 2^20], so products of six stay inside FP32 range;
-// other tiles take the FP64 paths.
+// FP32-bulk form: the per-(p,g) constants and the complex products in FP32, sums
+// over runs of G-vectors in FP32 folded into the FP64 totals.  Used for a tile only
+// when every denominator of it is far from the pole: |wx - wr| >= max|wx|/4 for all
+// bands, so u = wx - wr loses at most 2 bits in FP32, and every d in [2^-20, 2^20], so
+// products of six stay inside FP32 range; other tiles take the FP64 paths.
+// The numerators are formed from wx itself: P = Ir u + a = Ir wx + c1 with
+// c1 = a - Ir wr (c2 likewise), computed in FP64 at staging, so u is only needed for d.
 struct TileEntry32 {
-  float wr, wi2, Ir, Ii, a, b;
+  float wr, wi2, Ir, Ii, c1, c2;
 };
 
 __device__ inline TileEntry32 entry32(const TileEntry& e) {
@@ -212,8 +213,8 @@ __device__ inline TileEntry32 entry32(const TileEntry& e) {
   f.wi2 = (float)e.wi2;
   f.Ir = (float)e.Ir;
   f.Ii = (float)e.Ii;
-  f.a = (float)e.a;
-  f.b = (float)e.b;
+  f.c1 = (float)(e.a - e.Ir * e.wr);
+  f.c2 = (float)(e.b - e.Ii * e.wr);
   return f;
 }
 
@@ -225,49 +226,60 @@ __device__ inline bool far_from_pole(double2 w, double wxmin, double wxmax) {
   return lo >= 0.25 * scale && fma(lo, lo, wi2) >= 0x1p-20 && fma(hi, hi, wi2) <= 0x1p+20;
 }
 
-constexpr int kRun32 = 32;  // G-vectors per FP32 run
+// One band per lane.  A step takes two G-vectors j, j+1: the six denominators of the
+// lane's (G-vector, frequency) terms share one batched inversion.  The terms of j go to
+// sr and those of j+1 to sr2, so each FP32 sum holds at most 32 terms of a run of 64
+// (the same rounding as single sums over runs of 32, at half the FP64 folds); the run's
+// two sums are added and folded into the per-lane FP64 totals through sch_store1.
+// A tile of odd length reads one padding entry past its end: every entry of the tile
+// array is written, and a padding entry contributes exactly zero.
+constexpr int kRun32 = 64;  // G-vectors per FP32 run
 
-// Runs of kRun32 G-vectors are summed in FP32 and folded straight into the
-// per-lane FP64 totals through sch_store1: tot += s(n1,p) * run sum.
-template <int BPL>
-__device__ inline void tile_f32(const TileEntry32* __restrict__ tile, int n, const float (&x)[BPL][3],
-                                const double2 (&s)[BPL], double (&tot)[BPL][3][2]) {
+__device__ inline void tile_f32(const TileEntry32* __restrict__ tile, int n, const float (&x)[3], double2 s,
+                                double (&tot)[3][2]) {
   for (int j0 = 0; j0 < n; j0 += kRun32) {
     const int j1 = min(n, j0 + kRun32);
-    float sr[BPL][3][2];
+    float sr[3][2], sr2[3][2];
 #pragma unroll
-    for (int bb = 0; bb < BPL; ++bb)
-#pragma unroll
-      for (int iw = 0; iw < 3; ++iw) sr[bb][iw][0] = sr[bb][iw][1] = 0.f;
+    for (int iw = 0; iw < 3; ++iw) sr[iw][0] = sr[iw][1] = sr2[iw][0] = sr2[iw][1] = 0.f;
 #pragma unroll 2
-    for (int j = j0; j < j1; ++j) {
-      const TileEntry32 e = tile[j];
-      float u[BPL * 3], d[BPL * 3], r[BPL * 3];
+    for (int j = j0; j < j1; j += 2) {
+      const TileEntry32 e[2] = {tile[j], tile[j + 1]};
+      float d[6], p[6];
 #pragma unroll
-      for (int k = 0; k < BPL * 3; ++k) {
-        u[k] = x[k / 3][k % 3] - e.wr;
-        d[k] = fmaf(u[k], u[k], e.wi2);
+      for (int k = 0; k < 6; ++k) {
+        const float u = x[k % 3] - e[k / 3].wr;
+        d[k] = fmaf(u, u, e[k / 3].wi2);
       }
-      batch_inverse(d, r);
+      // batched inversion: prefix products, one reciprocal, back-substitution
+      p[0] = d[0];
 #pragma unroll
-      for (int k = 0; k < BPL * 3; ++k) {
-        sr[k / 3][k % 3][0] = fmaf(fmaf(e.Ir, u[k], e.a), r[k], sr[k / 3][k % 3][0]);
-        sr[k / 3][k % 3][1] = fmaf(fmaf(e.Ii, u[k], e.b), r[k], sr[k / 3][k % 3][1]);
+      for (int i = 1; i < 6; ++i) p[i] = p[i - 1] * d[i];
+      float inv = hw_rcp(p[5]);
+#pragma unroll
+      for (int i = 5; i >= 0; --i) {
+        const float r = i > 0 ? inv * p[i > 0 ? i - 1 : 0] : inv;
+        if (i > 0) inv = inv * d[i];
+        const TileEntry32& ei = e[i / 3];
+        float (&acc)[3][2] = i >= 3 ? sr2 : sr;
+        acc[i % 3][0] = fmaf(fmaf(ei.Ir, x[i % 3], ei.c1), r, acc[i % 3][0]);
+        acc[i % 3][1] = fmaf(fmaf(ei.Ii, x[i % 3], ei.c2), r, acc[i % 3][1]);
       }
     }
 #pragma unroll
-    for (int bb = 0; bb < BPL; ++bb)
-#pragma unroll
-      for (int iw = 0; iw < 3; ++iw) {
-        const double re = sr[bb][iw][0], im = sr[bb][iw][1];
-        tot[bb][iw][0] = fma(s[bb].x, re, fma(-s[bb].y, im, tot[bb][iw][0]));
-        tot[bb][iw][1] = fma(s[bb].x, im, fma(s[bb].y, re, tot[bb][iw][1]));
-      }
+    for (int iw = 0; iw < 3; ++iw) {
+      const double re = sr[iw][0] + sr2[iw][0], im = sr[iw][1] + sr2[iw][1];
+      tot[iw][0] = fma(s.x, re, fma(-s.y, im, tot[iw][0]));
+      tot[iw][1] = fma(s.x, im, fma(s.y, re, tot[iw][1]));
+    }
   }
 }
 
+// Waves per SIMD: FP64 four (128 VGPRs); FP32 eight (64 VGPRs), since FP32 VALU
+// instructions of two waves of a SIMD issue together and the pair rate grows with the
+// number of waves.
 template <int NT, int BPL, bool F32>
-__global__ __launch_bounds__(NT, F32 ? 6 : 4) void k_gpp_main(const double2* __restrict__ wt,
+__global__ __launch_bounds__(NT, F32 ? 8 : 4) void k_gpp_main(const double2* __restrict__ wt,
                                                  const double2* __restrict__ Ieps,
                                                  const double2* __restrict__ S,
                                                  const double* __restrict__ wxb,
@@ -275,6 +287,7 @@ __global__ __launch_bounds__(NT, F32 ? 6 : 4) void k_gpp_main(const double2* __r
                                                  int ngpown, int ncouls, int chunk, int nchunks,
                                                  long item0, long item1, int carry,
                                                  double2* __restrict__ partial) {
+  static_assert(!F32 || BPL == 1, "the FP32-bulk form runs one band per lane");
   __shared__ TileEntry tile[NT];
   __shared__ TileEntry32 tile32[F32 ? NT : 1];
   const int tid = threadIdx.x;
@@ -346,7 +359,7 @@ __global__ __launch_bounds__(NT, F32 ? 6 : 4) void k_gpp_main(const double2* __r
       if constexpr (F32) {
         // FP32-bulk form: FP32 tiles far from every pole, FP64 (IEEE division) otherwise
         if (!unsafe && !__syncthreads_or(!ok32)) {
-          tile_f32<BPL>(tile32, n, x32, sv, tot);
+          if constexpr (BPL == 1) tile_f32(tile32, n, x32[0], sv[0], tot[0]);
         } else {
           double xe[BPL][3], at[BPL][3][2];
 #pragma unroll
@@ -526,13 +539,24 @@ template <bool F32>
 int run_main_t(bgw_ctx* ctx, MainCfg cfg, const double2* wt, const double2* I, const double2* S, const double* wxb,
                const double* range, int nb, int ngpown, int ncouls, double2* ws, int* W, long item0, long item1) {
   const int chunk = kGChunk;
-  if (cfg.nt == 256 && cfg.bpl == 2)
-    return launch_main<256, 2, F32>(ctx, wt, I, S, wxb, range, nb, ngpown, ncouls, ws, W, chunk, item0, item1);
-  if (cfg.nt == 128 && cfg.bpl == 2)
-    return launch_main<128, 2, F32>(ctx, wt, I, S, wxb, range, nb, ngpown, ncouls, ws, W, chunk, item0, item1);
-  if (cfg.nt == 128)
-    return launch_main<128, 1, F32>(ctx, wt, I, S, wxb, range, nb, ngpown, ncouls, ws, W, chunk, item0, item1);
-  return launch_main<64, 1, F32>(ctx, wt, I, S, wxb, range, nb, ngpown, ncouls, ws, W, chunk, item0, item1);
+  if constexpr (F32) {
+    // one band per lane, a workgroup covering up to 512 bands
+    if (cfg.nt == 256 && cfg.bpl == 2)
+      return launch_main<512, 1, true>(ctx, wt, I, S, wxb, range, nb, ngpown, ncouls, ws, W, chunk, item0, item1);
+    if (cfg.nt == 128 && cfg.bpl == 2)
+      return launch_main<256, 1, true>(ctx, wt, I, S, wxb, range, nb, ngpown, ncouls, ws, W, chunk, item0, item1);
+    if (cfg.nt == 128)
+      return launch_main<128, 1, true>(ctx, wt, I, S, wxb, range, nb, ngpown, ncouls, ws, W, chunk, item0, item1);
+    return launch_main<64, 1, true>(ctx, wt, I, S, wxb, range, nb, ngpown, ncouls, ws, W, chunk, item0, item1);
+  } else {
+    if (cfg.nt == 256 && cfg.bpl == 2)
+      return launch_main<256, 2, false>(ctx, wt, I, S, wxb, range, nb, ngpown, ncouls, ws, W, chunk, item0, item1);
+    if (cfg.nt == 128 && cfg.bpl == 2)
+      return launch_main<128, 2, false>(ctx, wt, I, S, wxb, range, nb, ngpown, ncouls, ws, W, chunk, item0, item1);
+    if (cfg.nt == 128)
+      return launch_main<128, 1, false>(ctx, wt, I, S, wxb, range, nb, ngpown, ncouls, ws, W, chunk, item0, item1);
+    return launch_main<64, 1, false>(ctx, wt, I, S, wxb, range, nb, ngpown, ncouls, ws, W, chunk, item0, item1);
+  }
 }
 
 // Launches the main kernel over work items [item0, item1) (item1 < 0: all).  A
