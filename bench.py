@@ -145,13 +145,13 @@ def ff_cpu_reference(f, vals_fp64):
 
 def ff_leg(bgw, torch, dev, stream, ctx, reps=3, cpu_reference=False):
     """FF config 4 (nb 512, nvband 2, ncouls 16384, ngpown 819, nFreq 15, nfreqeval 10):
-    the three FF kernels in FP64 MFMA and split-FP32 MFMA; FLOP = 8 per distinct
-    complex MAC (achsDtemp nb*ngpown*ncouls, achDtemp_cor 2*n_active*ngpown*ncouls,
-    asxDtemp 2*nvband*ngpown*ncouls).  With cpu_reference, the reference's own CPU
-    kernels are timed beside them on the same inputs."""
+    the three FF kernels in FP64 MFMA and in FP32-operand MFMA with FP64 stage folds.
+    FLOP = 8 per distinct complex MAC (achsDtemp nb*ngpown*ncouls, achDtemp_cor
+    2*n_active*ngpown*ncouls, asxDtemp 2*nvband*ngpown*ncouls).  With cpu_reference,
+    the reference's own CPU kernels are timed beside them on the same inputs."""
     log("[ff] generating config 4 inputs")
     f = bgw.ff_config4_inputs()
-    keys = ("inv", "indinv", "ekq", "aqsm", "aqsn", "vcoul", "I_epsR", "I_epsA", "dFreqGrid")
+    keys = ("inv", "indinv", "ekq", "aqsm", "aqsn", "vcoul", "I_epsR", "I_epsA", "dFreqGrid", "pref")
     T = {k: torch.from_numpy(np.ascontiguousarray(f[k])).to(dev) for k in keys}
     ptrs = {k: v.data_ptr() for k, v in T.items()}
     out = torch.zeros(f["nfreqeval"], dtype=torch.complex128, device=dev)
@@ -163,8 +163,9 @@ def ff_leg(bgw, torch, dev, stream, ctx, reps=3, cpu_reference=False):
         n_active += int((wx > 0 and not occ) or (not wx > 0 and occ))
     npg = f["ngpown"] * f["ncouls"]
     work = {"achsDtemp": 8.0 * f["nb"] * npg, "asxDtemp": 8.0 * 2 * f["nvband"] * npg,
-            "achDtemp_cor": 8.0 * 2 * n_active * npg}
-    names = {"achsDtemp": "contract_ff_achs", "asxDtemp": "contract_ff_asx", "achDtemp_cor": "contract_ff_cor"}
+            "achDtemp_cor": 8.0 * 2 * n_active * npg, "achDtemp": 8.0 * f["nb"] * npg * f["nFreq"]}
+    names = {"achsDtemp": "contract_ff_achs", "asxDtemp": "contract_ff_asx", "achDtemp_cor": "contract_ff_cor",
+             "achDtemp": "contract_ff_achD"}
     # asxDtemp reads each slice its valence bands fall in (ifreq and ifreq + 1 of I_epsR
     # for wx > 0, of I_epsA otherwise) once: its bound is that reading
     asx_slices = set()
@@ -175,9 +176,9 @@ def ff_leg(bgw, torch, dev, stream, ctx, reps=3, cpu_reference=False):
     asx_bytes = 16.0 * len(asx_slices) * npg
     res = {"n_active_bands": n_active, "asxDtemp_slice_bytes": asx_bytes}
     vals = {}
-    for prec, pname, peak in ((bgw.BGW_FP64, "fp64", 78.6), (bgw.BGW_FP32_BULK, "split_fp32", 157.3)):
+    for prec, pname, peak in ((bgw.BGW_FP64, "fp64", 78.6), (bgw.BGW_FP32_BULK, "fp32_operand", 157.3)):
         ctx.set_precision(prec)
-        for op in ("achsDtemp", "asxDtemp", "achDtemp_cor"):
+        for op in ("achsDtemp", "asxDtemp", "achDtemp_cor", "achDtemp"):
             ctx.ff_device(op, f, ptrs, out.data_ptr())  # warm-up
             ctx.synchronize()
             ctx.reset_timing()
@@ -194,10 +195,10 @@ def ff_leg(bgw, torch, dev, stream, ctx, reps=3, cpu_reference=False):
             if op == "asxDtemp":
                 res[f"asxDtemp_{pname}_hbm_tbs"] = asx_bytes / (kms * 1e-3) / 1e12
     ctx.set_precision(bgw.BGW_FP64)
-    for op in ("achsDtemp", "asxDtemp", "achDtemp_cor"):
+    for op in ("achsDtemp", "asxDtemp", "achDtemp_cor", "achDtemp"):
         n = 1 if op == "achsDtemp" else f["nfreqeval"]  # achsDtemp is a single value
-        a, b = vals[("split_fp32", op)][:n], vals[("fp64", op)][:n]
-        res[f"{op}_split_fp32_rel_err"] = float(np.max(np.abs(a - b) / np.abs(b)))
+        a, b = vals[("fp32_operand", op)][:n], vals[("fp64", op)][:n]
+        res[f"{op}_fp32_operand_rel_err"] = float(np.max(np.abs(a - b) / np.abs(b)))
     res["value_fp64"] = {op: complex(vals[("fp64", op)][0]).__repr__() for op in names}
     del T
     torch.cuda.empty_cache()
@@ -237,8 +238,16 @@ def adjoint_leg(bgw, torch, dev, stream, ctx, reps=3):
     ctx.set_timing(False)
     bk, bn = ctx.kernel_time("gpp_bwd")
     ctx.synchronize()
-    res = {"forward_all_outputs_ms": f_ms, "backward_ms": b_ms, "gpp_bwd_kernel_ms": bk / max(bn, 1),
-           "backward_over_forward": b_ms / f_ms,
+    bk /= max(bn, 1)
+    # algorithmic FP64 work of the backward's pole-term sums, per (n1, p, g, iw) term:
+    # 1/(wx - wtilde) 7 (u, |wd|^2 3, reciprocal, two scalings), its square 5, c*inv^2
+    # multiply-add 8, and c*inv multiply-add shared by the three frequencies of a band
+    # (2 adds + 8/3): 74/3 FLOP
+    bwd_flops = 74.0 / 3.0 * d["nb"] * npg * 3
+    res = {"forward_all_outputs_ms": f_ms, "backward_ms": b_ms, "gpp_bwd_kernel_ms": bk,
+           "backward_over_forward": b_ms / f_ms, "gpp_bwd_flop": bwd_flops,
+           "gpp_bwd_tflops": bwd_flops / (bk * 1e-3) / 1e12,
+           "gpp_bwd_roofline_frac": bwd_flops / (bk * 1e-3) / 1e12 / FP64_PEAK_TFLOPS,
            "dsigma_domega": [repr(complex(z)) for z in o["ds"].cpu().numpy()]}
     del T, o
     torch.cuda.empty_cache()
@@ -294,8 +303,6 @@ def main():
     ap.add_argument("--host-api-only", action="store_true",
                     help="only time the host-pointer entry points (GPP benchmark workload, FF config 4), "
                          "PCIe transfers included, and print one JSON line")
-    ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
-                    help="only run a device-to-device copy of BYTES (reference for rocprofv3 FETCH/WRITE_SIZE)")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -310,15 +317,6 @@ def main():
     if not torch.cuda.is_available():
         raise SystemExit("no GPU visible")
     dev = torch.device("cuda:0")
-    if args.copy_calibration:
-        n = args.copy_calibration // 8
-        a = torch.ones(n, dtype=torch.float64, device=dev)
-        b = torch.empty_like(a)
-        torch.cuda.synchronize()
-        b.copy_(a)
-        torch.cuda.synchronize()
-        print(json.dumps({"copy_bytes": n * 8}), flush=True)
-        return
 
     if args.host_api_only:
         print(json.dumps({"host_api": host_api_leg(bgw)}), flush=True)
@@ -495,12 +493,18 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic (seeded SplitMix64, benchmark distributions)",
-        "config": {"workload": "GPP noflagOCC_solver sum, achtemp + acht_n1_loc",
+        "config": {"workload": "GPP noflagOCC_solver sum; timed outputs achtemp + acht_n1_loc (the reference's "
+                               "noflagOCC_solver outputs); achstemp + asxtemp in full_outputs",
                    "number_bands": nb, "nvband": nvband, "ncouls": ncouls, "nodes_per_group": 20,
                    "ngpown": ngpown, "frequencies": 3, "flop_per_term": 15, "parallelism": "single GPU"},
         "roofline": {"bound": "valu", "achieved": kern_tflops, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": kern_tflops / FP64_PEAK_TFLOPS, "traffic": read_traffic(),
                      "kernel": "k_gpp_main", "flop_per_launch": flops},
+        "full_outputs": None if full_ms is None else {
+            "outputs": ["achtemp", "acht_n1_loc", "achstemp", "asxtemp"], "ms_per_step": full_ms,
+            "tflops_same_count": flops / (full_ms * 1e-3) / 1e12,
+            "note": "the same step with all four outputs; TFLOP/s by the headline's count (15 FLOP per achtemp "
+                    "term), so achstemp's and asxtemp's own work is not credited"},
         "cpu_baseline": cpu,
         "extras": extras,
     }

@@ -157,10 +157,11 @@ int gpp_backward_device(bgw_ctx* ctx, size_t nb, size_t nvband, size_t ngpown, s
                         const double* I_eps, const double* vcoul, double occ,
                         double* dsigma, double* dwt, double* dI);
 
-// FF: op 0 = achsDtemp, 1 = asxDtemp, 2 = achDtemp_cor
+// FF: op 0 = achsDtemp, 1 = asxDtemp, 2 = achDtemp_cor, 3 = achDtemp (frequency sum,
+// reads pref[nFreq]; I_epsA null means I_epsR holds R - A)
 int ff_device(bgw_ctx* ctx, int op, int nb, int nvband, int nfreqeval, int ncouls, int ngpown,
               int nFreq, double freqevalmin, double freqevalstep, double occ, const double* ekq,
               const double* dFreqGrid, const int* inv, const int* indinv, const double* aqsm,
               const double* aqsn, const double* vcoul, const double* I_epsR,
-              const double* I_epsA, double* out);
+              const double* I_epsA, double* out, const double* pref = nullptr);
 }  // namespace bgw

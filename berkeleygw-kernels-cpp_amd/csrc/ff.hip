@@ -87,6 +87,30 @@ __global__ __launch_bounds__(256) void k_ff_plan(int op, int nb, int nvband, int
   }
 }
 
+// op 3 (achDtemp, the real-axis frequency sum): job f = slice f of R - A, rows = every
+// band in ascending order, weight w(n1,f) = -i/2 pref(f) / (wx(n1) - s(n1) dFreqGrid(f)),
+// s = -1 for occupied bands (n1 < nvband), +1 otherwise; zero where the denominator is
+// exactly zero.  One block per slice.
+__global__ __launch_bounds__(256) void k_ff_freq_plan(int nb, int nvband, double freqevalmin, double freqevalstep,
+                                                      const double* __restrict__ ekq,
+                                                      const double* __restrict__ grid,
+                                                      const double* __restrict__ pref, int* __restrict__ rows,
+                                                      double2* __restrict__ wts, int* __restrict__ cnt,
+                                                      int* __restrict__ slice_of_job) {
+  const int f = blockIdx.x;
+  const double wf = grid[f], pf = pref[f];
+  for (int n1 = threadIdx.x; n1 < nb; n1 += blockDim.x) {
+    const double wx = freqevalmin - ekq[n1] + freqevalstep;
+    const double den = n1 < nvband ? wx + wf : wx - wf;
+    rows[(size_t)f * nb + n1] = n1;
+    wts[(size_t)f * nb + n1] = make_double2(0.0, den == 0.0 ? 0.0 : -0.5 * (pf / den));
+  }
+  if (threadIdx.x == 0) {
+    cnt[f] = nb;
+    slice_of_job[f] = f;
+  }
+}
+
 // One band's term of achDtemp_cor (schDttt_corKernel1/2, FF/ff.cpp:381-451) as two
 // one-row jobs: band n1 on slice ifreq with weight w1 and on slice ifreq + 1 with w2.
 __global__ void k_cor_band_plan(int* __restrict__ rows, double2* __restrict__ wts, int* __restrict__ cnt,
@@ -105,7 +129,7 @@ __global__ void k_cor_band_plan(int* __restrict__ rows, double2* __restrict__ wt
 int ff_device(bgw_ctx* ctx, int op, int nb, int nvband, int nfreqeval, int ncouls, int ngpown, int nFreq,
               double freqevalmin, double freqevalstep, double occ, const double* ekq, const double* dFreqGrid,
               const int* inv, const int* indinv, const double* aqsm, const double* aqsn, const double* vcoul,
-              const double* I_epsR, const double* I_epsA, double* out) {
+              const double* I_epsR, const double* I_epsA, double* out, const double* pref) {
   hipStream_t s = ctx->stream;
   launch_check_indices(inv, (size_t)ngpown, indinv, (size_t)ncouls, ncouls - 1, ncouls, ctx->d_flag, s);
   ContractArgs a{};
@@ -123,7 +147,7 @@ int ff_device(bgw_ctx* ctx, int op, int nb, int nvband, int nfreqeval, int ncoul
   a.out = reinterpret_cast<double2*>(out);
   const bool f32 = ctx->precision == BGW_FP32_BULK;
   void *prow, *pw, *pcnt, *pslice;
-  const int njobs = op == 0 ? 1 : (op == 1 ? 2 * nFreq : nFreq);
+  const int njobs = op == 0 ? 1 : (op == 1 ? 2 * nFreq : nFreq);  // op 2 and 3: one job per slice
   const int maxrows = op == 0 ? nb : (op == 1 ? nvband : nb);
   int rc = pool_get(ctx, "ws_ff_rows", (size_t)njobs * nb * sizeof(int), &prow);
   if (!rc) rc = pool_get(ctx, "ws_ff_wts", (size_t)njobs * nb * 16, &pw);
@@ -144,6 +168,24 @@ int ff_device(bgw_ctx* ctx, int op, int nb, int nvband, int nfreqeval, int ncoul
     const int BMd = contract_row_tile(f32, false, nb);
     const int tiles = (nb + BMd - 1) / BMd;
     return contract_launch(ctx, a, 1, BMd, tiles, true, tiles, "contract_ff_achs");
+  }
+  if (op == 3) {
+    // the frequency sum: every band on every slice of R - A (dense row lists).  With
+    // I_epsA null, I_epsR already holds R - A (the host path forms it while staging).
+    hipLaunchKernelGGL(k_ff_freq_plan, dim3(nFreq), dim3(256), 0, s, nb, nvband, freqevalmin, freqevalstep, ekq,
+                       dFreqGrid, pref, static_cast<int*>(prow), static_cast<double2*>(pw), static_cast<int*>(pcnt),
+                       static_cast<int*>(pslice));
+    if ((rc = check_launch(ctx, "k_ff_freq_plan"))) return rc;
+    a.Xa = reinterpret_cast<const double2*>(I_epsR);
+    a.Y = reinterpret_cast<const double2*>(I_epsA);
+    a.jobs_a = nFreq;
+    const int BMd = contract_row_tile(f32, false, nb);
+    const long long tiles = (long long)nFreq * ((nb + BMd - 1) / BMd);
+    if (tiles > INT32_MAX) return fail(ctx, BGW_EINVAL, "frequency sum too large");
+    rc = contract_launch(ctx, a, nFreq, BMd, (int)tiles, true, (int)tiles, "contract_ff_achD");
+    if (rc) return rc;
+    launch_replicate(a.out, nfreqeval, s);
+    return check_launch(ctx, "replicate");
   }
   void *pbj, *pbw;
   if ((rc = pool_get(ctx, "ws_ff_band_job", (size_t)nb * sizeof(int), &pbj))) return rc;
@@ -266,12 +308,13 @@ int upload(bgw_ctx* ctx, const char* slot, const T* src, size_t count, const T**
 int ff_host(bgw_ctx* ctx, int op, int nb, int nvband, int nfreqeval, int ncouls, int ngpown, int nFreq,
             double fmin, double fstep, double occ, const double* ekq, const double* grid, const int* inv,
             const int* indinv, const double* aqsm, const double* aqsn, const double* vcoul,
-            const double* R, const double* A, double* out) {
+            const double* R, const double* A, double* out, const double* pref = nullptr) {
   int rc = ff_host_indices(ctx, ngpown, ncouls, inv, indinv);
   if (rc) return rc;
   const size_t npg = (size_t)ngpown * ncouls;
   const int *d_inv, *d_indinv;
-  const double *d_ekq = nullptr, *d_grid = nullptr, *d_am, *d_an, *d_vc, *d_R = nullptr, *d_A = nullptr;
+  const double *d_ekq = nullptr, *d_grid = nullptr, *d_pref = nullptr, *d_am, *d_an, *d_vc, *d_R = nullptr,
+               *d_A = nullptr;
   if ((rc = upload(ctx, "h_inv", inv, (size_t)ngpown, &d_inv))) return rc;
   if ((rc = upload(ctx, "h_indinv", indinv, (size_t)ncouls, &d_indinv))) return rc;
   // asxDtemp reads only the valence rows n1 < nvband of aqsm / aqsn (its row lists
@@ -289,12 +332,14 @@ int ff_host(bgw_ctx* ctx, int op, int nb, int nvband, int nfreqeval, int ncouls,
   } else {
     if ((rc = upload(ctx, "h_ekq", ekq, (size_t)(op == 1 ? nvband : nb), &d_ekq))) return rc;
     if ((rc = upload(ctx, "h_grid", grid, (size_t)nFreq, &d_grid))) return rc;
+    if (op == 3 && (rc = upload(ctx, "h_pref", pref, (size_t)nFreq, &d_pref))) return rc;
     // Only the slices the device plan (k_ff_plan) will read are transferred.  The
     // slice of a band is ifreq (and ifreq + 1) of compute_fact, the same function
     // on host and device, and wx is formed with the same two exact-order operations,
     // so the sets agree; untouched slices of the device buffer are never read.
-    std::vector<char> needR(nFreq, 0), needA(nFreq, 0);
-    const int nbands = op == 1 ? nvband : nb;
+    // The frequency sum (op 3) reads every slice.
+    std::vector<char> needR(nFreq, op == 3), needA(nFreq, op == 3);
+    const int nbands = op == 1 ? nvband : (op == 2 ? nb : 0);
     for (int n1 = 0; n1 < nbands; ++n1) {
       const double wx = fmin - ekq[n1] + fstep;
       double f1 = 0.0, f2 = 0.0;
@@ -312,8 +357,9 @@ int ff_host(bgw_ctx* ctx, int op, int nb, int nvband, int nfreqeval, int ncouls,
     void *pR, *pA = nullptr;
     if ((rc = pool_get(ctx, "h_epsR", 2 * npg * nFreq * sizeof(double), &pR))) return rc;
     const size_t slice = npg * 16;
-    if (op == 2) {
-      // achDtemp_cor reads its slices only as I_epsR - I_epsA (FF/ff.cpp:399-404, :436-441):
+    if (op == 2 || op == 3) {
+      // achDtemp_cor and the frequency sum read their slices only as I_epsR - I_epsA
+      // (FF/ff.cpp:399-404, :436-441):
       // the host forms the difference while staging, so half the bytes cross the bus, and
       // the contraction runs on it in its plain mode (the same IEEE subtraction as on the
       // device, so the result equals the device entry point's bit for bit)
@@ -331,12 +377,12 @@ int ff_host(bgw_ctx* ctx, int op, int nb, int nvband, int nfreqeval, int ncouls,
       }
     }
     d_R = static_cast<const double*>(pR);
-    d_A = static_cast<const double*>(pA);  // null for achDtemp_cor: d_R holds R - A
+    d_A = static_cast<const double*>(pA);  // null for achDtemp_cor and achDtemp: d_R holds R - A
   }
   void* d_out;
   if ((rc = pool_get(ctx, "h_ffout", (size_t)(nfreqeval > 1 ? nfreqeval : 1) * 16, &d_out))) return rc;
   rc = ff_device(ctx, op, nb, nvband, nfreqeval, ncouls, ngpown, nFreq, fmin, fstep, occ, d_ekq, d_grid,
-                 d_inv, d_indinv, d_am, d_an, d_vc, d_R, d_A, static_cast<double*>(d_out));
+                 d_inv, d_indinv, d_am, d_an, d_vc, d_R, d_A, static_cast<double*>(d_out), d_pref);
   if (rc) {
     (void)hipStreamSynchronize(ctx->stream);
     return rc;
@@ -456,6 +502,35 @@ int bgw_achDtemp_cor_Kernel(bgw_ctx* ctx, int number_bands, int nvband, int nfre
   return ff_host(ctx, 2, number_bands, nvband, nfreqeval, ncouls, ngpown, nFreq, freqevalmin, freqevalstep, 0.0,
                  ekq, dFreqGrid, inv_igp_index, indinv, aqsmtemp, aqsntemp, vcoul, I_epsR_array, I_epsA_array,
                  achDtemp_cor);
+}
+
+int bgw_achDtemp_Kernel(bgw_ctx* ctx, int number_bands, int nvband, int nfreqeval, int ncouls, int ngpown,
+                        int nFreq, double freqevalmin, double freqevalstep, const double* ekq,
+                        const double* dFreqGrid, const double* pref, const int* inv_igp_index, const int* indinv,
+                        const double* aqsmtemp, const double* aqsntemp, const double* vcoul,
+                        const double* I_epsR_array, const double* I_epsA_array, double* achDtemp) {
+  int rc = ff_validate(ctx, 3, number_bands, nvband, nfreqeval, ncouls, ngpown, nFreq,
+                       {ekq, dFreqGrid, pref, inv_igp_index, indinv, aqsmtemp, aqsntemp, vcoul, I_epsR_array,
+                        I_epsA_array, achDtemp});
+  if (rc) return rc;
+  return ff_host(ctx, 3, number_bands, nvband, nfreqeval, ncouls, ngpown, nFreq, freqevalmin, freqevalstep, 0.0,
+                 ekq, dFreqGrid, inv_igp_index, indinv, aqsmtemp, aqsntemp, vcoul, I_epsR_array, I_epsA_array,
+                 achDtemp, pref);
+}
+
+int bgw_achDtemp_Kernel_device(bgw_ctx* ctx, int number_bands, int nvband, int nfreqeval, int ncouls, int ngpown,
+                               int nFreq, double freqevalmin, double freqevalstep, const double* ekq,
+                               const double* dFreqGrid, const double* pref, const int* inv_igp_index,
+                               const int* indinv, const double* aqsmtemp, const double* aqsntemp,
+                               const double* vcoul, const double* I_epsR_array, const double* I_epsA_array,
+                               double* achDtemp) {
+  int rc = ff_validate(ctx, 3, number_bands, nvband, nfreqeval, ncouls, ngpown, nFreq,
+                       {ekq, dFreqGrid, pref, inv_igp_index, indinv, aqsmtemp, aqsntemp, vcoul, I_epsR_array,
+                        I_epsA_array, achDtemp});
+  if (rc) return rc;
+  return ff_device(ctx, 3, number_bands, nvband, nfreqeval, ncouls, ngpown, nFreq, freqevalmin, freqevalstep, 0.0,
+                   ekq, dFreqGrid, inv_igp_index, indinv, aqsmtemp, aqsntemp, vcoul, I_epsR_array, I_epsA_array,
+                   achDtemp, pref);
 }
 
 int bgw_achsDtemp_Kernel_device(bgw_ctx* ctx, int number_bands, int ngpown, int ncouls, const int* inv_igp_index,

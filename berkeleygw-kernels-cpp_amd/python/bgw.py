@@ -101,6 +101,9 @@ def _declare(L: C.CDLL) -> None:
     cor = [_P, _I, _I, _I, _I, _I, _I, _D, _D, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P]
     L.bgw_achDtemp_cor_Kernel.argtypes = cor
     L.bgw_achDtemp_cor_Kernel_device.argtypes = cor
+    achD = [_P, _I, _I, _I, _I, _I, _I, _D, _D, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P]
+    L.bgw_achDtemp_Kernel.argtypes = achD
+    L.bgw_achDtemp_Kernel_device.argtypes = achD
     band = [_P, _P, _P, _P, _P, _P, _P, _P, _P, _I, _I, _I, _I, _D, _D]
     for k in ("1", "2"):
         getattr(L, f"bgw_schDttt_corKernel{k}").argtypes = band
@@ -270,13 +273,28 @@ def gpp_config_inputs(config: int, nb: int | None = None) -> dict:
     return gpp_inputs(nb or c["nb"], c["nvband"], c["ncouls"], ngpown, seed_base(config))
 
 
+def ff_pref(dFreqGrid) -> np.ndarray:
+    """The reference driver's trapezoid weights of the frequency grid (FF/ff.cpp:619-626):
+    pref[f] = (grid[f+1] - grid[f]) / 3.14, the last repeating the one before, both ends
+    halved.  The input of bgw_achDtemp_Kernel."""
+    g = np.asarray(dFreqGrid, np.float64)
+    if g.size < 2:
+        raise ValueError("the frequency grid needs at least two points")
+    p = np.empty(g.size)
+    p[:-1] = (g[1:] - g[:-1]) / 3.14
+    p[-1] = p[-2]
+    p[0] *= 0.5
+    p[-1] *= 0.5
+    return p
+
+
 def ff_inputs(nb: int, nvband: int, ncouls: int, ngpown: int, nFreq: int, nfreqeval: int, seed: int,
               kind: str = "random") -> dict:
     """FF inputs.  kind="reference" reproduces FF/ff.cpp:584-633 (inv/indinv identity,
     ekq = -10 + n1, aqs 0.5+0.5i, vcoul 1, I_epsR 0.5+0.5i, I_epsA 0.5-0.5i,
     dFreqGrid = 2f, freqevalmin 0, freqevalstep 0.5, occ 1).  kind="random": aqs as
     the GPP inputs, I_eps{R,A} re U[0,.5], im U[-.1,.1], ekq U[-1,1], dFreqGrid
-    uniform on [0,2], same freqevalmin/step/occ."""
+    uniform on [0,2], same freqevalmin/step/occ.  Both carry pref = ff_pref(dFreqGrid)."""
     npg = nFreq * ngpown * ncouls
     if kind == "reference":
         return dict(nb=nb, nvband=nvband, ncouls=ncouls, ngpown=ngpown, nFreq=nFreq, nfreqeval=nfreqeval,
@@ -285,7 +303,7 @@ def ff_inputs(nb: int, nvband: int, ncouls: int, ngpown: int, nFreq: int, nfreqe
                     aqsn=np.full(nb * ncouls, 0.5 + 0.5j), vcoul=np.ones(ncouls),
                     I_epsR=np.full(npg, 0.5 + 0.5j), I_epsA=np.full(npg, 0.5 - 0.5j),
                     dFreqGrid=2.0 * np.arange(nFreq, dtype=np.float64), freqevalmin=0.0, freqevalstep=0.5,
-                    occ=1.0)
+                    occ=1.0, pref=ff_pref(2.0 * np.arange(nFreq, dtype=np.float64)))
     return dict(nb=nb, nvband=nvband, ncouls=ncouls, ngpown=ngpown, nFreq=nFreq, nfreqeval=nfreqeval,
                 inv=synth_index(seed + 6, ncouls, ngpown), indinv=synth_permutation(seed + 7, ncouls),
                 ekq=synth_uniform(seed + 8, -1.0, 1.0, nb),
@@ -295,7 +313,7 @@ def ff_inputs(nb: int, nvband: int, ncouls: int, ngpown: int, nFreq: int, nfreqe
                 I_epsR=synth_complex(seed + 9, (0, 0.5), (-0.1, 0.1), npg),
                 I_epsA=synth_complex(seed + 10, (0, 0.5), (-0.1, 0.1), npg),
                 dFreqGrid=2.0 * np.arange(nFreq, dtype=np.float64) / (nFreq - 1), freqevalmin=0.0,
-                freqevalstep=0.5, occ=1.0)
+                freqevalstep=0.5, occ=1.0, pref=ff_pref(2.0 * np.arange(nFreq, dtype=np.float64) / (nFreq - 1)))
 
 
 def ff_config4_inputs(nb: int | None = None, ngpown: int | None = None, ncouls: int | None = None) -> dict:
@@ -604,6 +622,21 @@ class Context:
             a["indinv"], a["aqsm"], a["aqsn"], a["vcoul"], a["R"], a["A"], _ptr(out)))
         return out
 
+    def achDtemp_Kernel(self, f: dict):
+        """The real-axis frequency sum (bgw_achDtemp_Kernel); f["pref"] defaults to
+        ff_pref(f["dFreqGrid"])."""
+        out = np.zeros(f["nfreqeval"], np.complex128)
+        h = _Hold()
+        nb, nF = f["nb"], f["nFreq"]
+        a = _ff_arrays(h, f, nb, nF)
+        pref = f["pref"] if f.get("pref") is not None else ff_pref(f["dFreqGrid"])
+        self._check(lib().bgw_achDtemp_Kernel(
+            self._h, nb, f["nvband"], f["nfreqeval"], f["ncouls"], f["ngpown"], nF, f["freqevalmin"],
+            f["freqevalstep"], h.f64(f["ekq"], nb, "ekq"), h.f64(f["dFreqGrid"], nF, "dFreqGrid"),
+            h.f64(pref, nF, "pref"), a["inv"], a["indinv"], a["aqsm"], a["aqsn"], a["vcoul"], a["R"], a["A"],
+            _ptr(out)))
+        return out
+
     def schDttt_corKernel(self, kernel: int, f: dict, n1: int, ifreq: int, fact1: float, fact2: float) -> complex:
         """schDttt_corKernel1 (kernel=1) or schDttt_corKernel2 (kernel=2) on host arrays:
         band n1's term of achDtemp_cor on slices ifreq, ifreq + 1."""
@@ -633,6 +666,12 @@ class Context:
                                               ptrs["ekq"], ptrs["dFreqGrid"], ptrs["inv"], ptrs["indinv"],
                                               ptrs["aqsm"], ptrs["aqsn"], ptrs["vcoul"], ptrs["I_epsR"],
                                               ptrs["I_epsA"], out_ptr)
+        elif op == "achDtemp":
+            rc = L.bgw_achDtemp_Kernel_device(self._h, f["nb"], f["nvband"], f["nfreqeval"], f["ncouls"],
+                                              f["ngpown"], f["nFreq"], f["freqevalmin"], f["freqevalstep"],
+                                              ptrs["ekq"], ptrs["dFreqGrid"], ptrs["pref"], ptrs["inv"],
+                                              ptrs["indinv"], ptrs["aqsm"], ptrs["aqsn"], ptrs["vcoul"],
+                                              ptrs["I_epsR"], ptrs["I_epsA"], out_ptr)
         else:
             rc = L.bgw_achDtemp_cor_Kernel_device(self._h, f["nb"], f["nvband"], f["nfreqeval"], f["ncouls"],
                                                   f["ngpown"], f["nFreq"], f["freqevalmin"], f["freqevalstep"],

@@ -47,7 +47,9 @@ extern "C" {
 
 /* ---- arithmetic forms (bgw_set_precision) ------------------------------- */
 #define BGW_FP64       0  /* plain FP64 throughout (default)                           */
-#define BGW_FP32_BULK  1  /* bulk complex products in FP32 (GPP) / split-FP32 MFMA (FF),
+#define BGW_FP32_BULK  1  /* bulk complex products in FP32 (GPP) / FP32-operand MFMA with
+                             FP64 stage folds (FF contraction: operands rounded once to
+                             FP32, FP32 sums over 16 G-vectors folded into FP64),
                              FP64 pole denominators and FP64 compensated reductions   */
 
 typedef struct bgw_ctx bgw_ctx;
@@ -215,6 +217,34 @@ int bgw_achDtemp_cor_Kernel_device(bgw_ctx* ctx, int number_bands, int nvband, i
                                    const double* aqsmtemp, const double* aqsntemp,
                                    const double* vcoul, const double* I_epsR_array,
                                    const double* I_epsA_array, double* achDtemp_cor);
+
+/* The real-axis frequency sum of the FF self-energy, the term the reference prepares in its
+ * driver and never applies: its trapezoid weights pref (FF/ff.cpp:564-565,619-626), the
+ * spectral normalisation pref_zb = 0.5/3.14 (:468) and the orphan achDtemp timers (:503).
+ * No reference counterpart; defined here as
+ *   achDtemp = sum_{n1<number_bands} sum_{p<ngpown} sum_{g<ncouls} aqsn(n1,g) conj(aqsm(n1,igp))
+ *              vcoul(igp) sum_{f<nFreq} -i/2 pref(f) (R - A)(f,p,g) / (wx(n1) - s(n1) dFreqGrid(f)),
+ *   wx(n1) = freqevalmin - ekq(n1) + freqevalstep  (as in the reference's FF kernels, :245,:337),
+ *   s(n1) = -1 for occupied bands n1 < nvband (pole at -dFreqGrid), +1 otherwise,
+ * R - A = I_epsR_array - I_epsA_array (the spectral function times 2i, so -i/2 pref(f) is
+ * -i pref_zb h_f with h_f the trapezoid step), a term with an exactly zero denominator left
+ * out.  pref[nFreq] is the reference driver's array: (dFreqGrid(f+1) - dFreqGrid(f))/3.14,
+ * the last entry repeating the one before, both ends halved.  achDtemp c128[nfreqeval], all
+ * entries equal (wx has no iw term); the other arguments as bgw_achDtemp_cor_Kernel's. */
+int bgw_achDtemp_Kernel(bgw_ctx* ctx, int number_bands, int nvband, int nfreqeval, int ncouls,
+                        int ngpown, int nFreq, double freqevalmin, double freqevalstep,
+                        const double* ekq, const double* dFreqGrid, const double* pref,
+                        const int* inv_igp_index, const int* indinv,
+                        const double* aqsmtemp, const double* aqsntemp,
+                        const double* vcoul, const double* I_epsR_array,
+                        const double* I_epsA_array, double* achDtemp);
+int bgw_achDtemp_Kernel_device(bgw_ctx* ctx, int number_bands, int nvband, int nfreqeval,
+                               int ncouls, int ngpown, int nFreq, double freqevalmin,
+                               double freqevalstep, const double* ekq, const double* dFreqGrid,
+                               const double* pref, const int* inv_igp_index, const int* indinv,
+                               const double* aqsmtemp, const double* aqsntemp,
+                               const double* vcoul, const double* I_epsR_array,
+                               const double* I_epsA_array, double* achDtemp);
 
 /* Replace schDttt_corKernel1 (FF/ff.cpp:381-416) and schDttt_corKernel2 (:418-451),
  * declared in common/Defines.h:36-48, with the reference's argument order:

@@ -810,3 +810,94 @@ void oracle_schDttt_cor(int kernel, int ncouls, int ngpown, int ifreq, int n1, d
   schDttt_cor[1] = dd_val(a.im);
   free(part);
 }
+
+/* sum_g aqsn(n1,g) (X - Y)(f,p,g) over one frequency slice, compensated. */
+static cdd_t ff_slice_dot(const double* an_row, const double* X, const double* Y, int f, size_t p,
+                          size_t ngpown, size_t ncouls) {
+  const double* x = X + 2 * (((size_t)f * ngpown + p) * ncouls);
+  const double* y = Y + 2 * (((size_t)f * ngpown + p) * ncouls);
+  lanes_t ar, ai;
+  lanes_zero(&ar);
+  lanes_zero(&ai);
+  size_t g0 = 0;
+  for (; g0 + LANES <= ncouls; g0 += LANES) {
+    for (int l = 0; l < LANES; ++l) {
+      const size_t g = g0 + l;
+      const double er = x[2 * g] - y[2 * g], ei = x[2 * g + 1] - y[2 * g + 1];
+      const double nr = an_row[2 * g], ni = an_row[2 * g + 1];
+      const double tr = nr * er - ni * ei, ti = nr * ei + ni * er;
+      double s1, e1, s2, e2;
+      two_sum(ar.h[l], tr, &s1, &e1);
+      ar.h[l] = s1;
+      ar.l[l] += e1;
+      two_sum(ai.h[l], ti, &s2, &e2);
+      ai.h[l] = s2;
+      ai.l[l] += e2;
+    }
+  }
+  cdd_t r = {lanes_fold(&ar), lanes_fold(&ai)};
+  for (; g0 < ncouls; ++g0) {
+    const double er = x[2 * g0] - y[2 * g0], ei = x[2 * g0 + 1] - y[2 * g0 + 1];
+    const double nr = an_row[2 * g0], ni = an_row[2 * g0 + 1];
+    dd_add(&r.re, nr * er - ni * ei);
+    dd_add(&r.im, nr * ei + ni * er);
+  }
+  return r;
+}
+
+void oracle_achDtemp(int nb, int nvband, int nfreqeval, int ncouls, int ngpown, int nFreq,
+                     double freqevalmin, double freqevalstep, const double* ekq,
+                     const double* dFreqGrid, const double* pref, const int* inv_igp_index,
+                     const int* indinv, const double* aqsm, const double* aqsn, const double* vcoul,
+                     const double* I_epsR, const double* I_epsA, double* achDtemp, int nthreads) {
+  /* The real-axis frequency sum that the reference prepares and does not apply: the
+   * trapezoid weights pref (FF/ff.cpp:619-626) and the spectral normalisation pref_zb =
+   * 0.5/3.14 (:468).  achDtemp = sum_{n1,p,g} aqsn(n1,g) conj(aqsm(n1,igp)) vcoul(igp)
+   * sum_f w(n1,f) (R - A)(f,p,g), with w(n1,f) = -i/2 pref(f) / (wx(n1) - s(n1) dFreqGrid(f)),
+   * wx(n1) = freqevalmin - ekq(n1) + freqevalstep as in the reference's kernels (:245,
+   * :337), s = -1 for occupied bands (n1 < nvband), +1 otherwise, and w = 0 where the
+   * denominator is exactly zero.  No iw dependence: one sum, replicated over iw. */
+  const int nt = nthreads_of(nthreads);
+  double* wim = (double*)calloc((size_t)nb * nFreq, sizeof(double));
+  for (int n1 = 0; n1 < nb; ++n1) {
+    const double wx = freqevalmin - ekq[n1] + freqevalstep;
+    for (int f = 0; f < nFreq; ++f) {
+      const double den = n1 < nvband ? wx + dFreqGrid[f] : wx - dFreqGrid[f];
+      wim[(size_t)n1 * nFreq + f] = den == 0.0 ? 0.0 : -0.5 * (pref[f] / den);
+    }
+  }
+  cdd_t* part = (cdd_t*)calloc((size_t)nt, sizeof(cdd_t));
+#pragma omp parallel num_threads(nt)
+  {
+#ifdef _OPENMP
+    const int tid = omp_get_thread_num();
+#else
+    const int tid = 0;
+#endif
+#pragma omp for schedule(static)
+    for (long lp = 0; lp < (long)ngpown; ++lp) {
+      const int igp = igp_of(inv_igp_index, indinv, (size_t)lp);
+      for (int n1 = 0; n1 < nb; ++n1) {
+        const double complex am = aqsm[2 * ((size_t)n1 * ncouls + igp)] + I * aqsm[2 * ((size_t)n1 * ncouls + igp) + 1];
+        const double complex c = conj(am) * vcoul[igp];
+        for (int f = 0; f < nFreq; ++f) {
+          const double wf = wim[(size_t)n1 * nFreq + f];
+          if (wf == 0.0) continue;
+          cdd_t M = ff_slice_dot(aqsn + 2 * (size_t)n1 * ncouls, I_epsR, I_epsA, f, (size_t)lp,
+                                 (size_t)ngpown, (size_t)ncouls);
+          const double complex cw = c * (I * wf);
+          cdd_add_cprod_dd(&part[tid], creal(cw), cimag(cw), M);
+        }
+      }
+    }
+  }
+  cdd_t a;
+  memset(&a, 0, sizeof(a));
+  for (int t = 0; t < nt; ++t) cdd_add(&a, part[t]);
+  for (int iw = 0; iw < nfreqeval; ++iw) {
+    achDtemp[2 * iw] = dd_val(a.re);
+    achDtemp[2 * iw + 1] = dd_val(a.im);
+  }
+  free(part);
+  free(wim);
+}

@@ -68,6 +68,14 @@ void oracle_achDtemp_cor(int nb, int nvband, int nfreqeval, int ncouls, int ngpo
                          const double* aqsm, const double* aqsn, const double* vcoul,
                          const double* I_epsR, const double* I_epsA, double* achDtemp_cor,
                          int nthreads);
+/* The real-axis frequency sum with the reference's unused weights pref (FF/ff.cpp:619-626)
+ * and pref_zb (:468): achDtemp = sum_{n1,p,g} aqsn conj(aqsm(igp)) vcoul(igp)
+ * sum_f -i/2 pref(f) (R - A)(f,p,g) / (wx(n1) - s(n1) dFreqGrid(f)), s = -1 occupied. */
+void oracle_achDtemp(int nb, int nvband, int nfreqeval, int ncouls, int ngpown, int nFreq,
+                     double freqevalmin, double freqevalstep, const double* ekq,
+                     const double* dFreqGrid, const double* pref, const int* inv_igp_index,
+                     const int* indinv, const double* aqsm, const double* aqsn, const double* vcoul,
+                     const double* I_epsR, const double* I_epsA, double* achDtemp, int nthreads);
 /* schDttt_corKernel1 (kernel 1, FF/ff.cpp:381-416) / schDttt_corKernel2 (kernel 2,
  * :418-451, factor -1/2): band n1 on slices ifreq, ifreq+1 of I_epsR - I_epsA. */
 void oracle_schDttt_cor(int kernel, int ncouls, int ngpown, int ifreq, int n1, double fact1,

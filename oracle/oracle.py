@@ -52,6 +52,8 @@ def olib() -> C.CDLL:
         L.oracle_asxDtemp.restype = None
         L.oracle_achDtemp_cor.argtypes = [_I] + ff + [_P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _I]
         L.oracle_achDtemp_cor.restype = None
+        L.oracle_achDtemp.argtypes = [_I] + ff + [_P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _I]
+        L.oracle_achDtemp.restype = None
         L.oracle_schDttt_cor.argtypes = [_I, _I, _I, _I, _I, _D, _D] + [_P] * 8 + [_I]
         L.oracle_schDttt_cor.restype = None
         _o = L
@@ -160,6 +162,31 @@ def achDtemp_cor(f: dict, nthreads: int | None = None) -> np.ndarray:
                                f["freqevalmin"], f["freqevalstep"], _p(f["ekq"]), _p(f["dFreqGrid"]), _p(f["inv"]),
                                _p(f["indinv"]), _p(f["aqsm"]), _p(f["aqsn"]), _p(f["vcoul"]), _p(f["I_epsR"]),
                                _p(f["I_epsA"]), _p(out), nthreads_default() if nthreads is None else nthreads)
+    return out
+
+
+def pref_weights(dFreqGrid) -> np.ndarray:
+    """The reference driver's trapezoid weights, FF/ff.cpp:619-626: (grid[f+1]-grid[f])/3.14,
+    the last repeating the one before, both ends halved."""
+    g = np.asarray(dFreqGrid, np.float64)
+    n = g.size
+    p = np.empty(n)
+    for f in range(n):
+        p[f] = (g[f + 1] - g[f]) / 3.14 if f < n - 1 else p[f - 1]
+    p[0] *= 0.5
+    p[n - 1] *= 0.5
+    return p
+
+
+def achDtemp(f: dict, nthreads: int | None = None) -> np.ndarray:
+    """The real-axis frequency sum with the weights pref (oracle_achDtemp)."""
+    out = np.zeros(f["nfreqeval"], np.complex128)
+    pref = np.ascontiguousarray(f["pref"] if "pref" in f else pref_weights(f["dFreqGrid"]), np.float64)
+    olib().oracle_achDtemp(f["nb"], f["nvband"], f["nfreqeval"], f["ncouls"], f["ngpown"], f["nFreq"],
+                           f["freqevalmin"], f["freqevalstep"], _p(f["ekq"]), _p(f["dFreqGrid"]), _p(pref),
+                           _p(f["inv"]), _p(f["indinv"]), _p(f["aqsm"]), _p(f["aqsn"]), _p(f["vcoul"]),
+                           _p(f["I_epsR"]), _p(f["I_epsA"]), _p(out),
+                           nthreads_default() if nthreads is None else nthreads)
     return out
 
 

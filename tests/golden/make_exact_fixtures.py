@@ -7,9 +7,12 @@ are the exact sums rounded once to double.  Formulas (reference lines):
   achtemp[iw]  = sum_{n1,p,g} s(n1,p) I(p,g) conj(wd)/|wd|^2, wd = wx(n1,iw) - wt(p,g),
                  s = conj(aqsm(n1,igp)) aqsn(n1,igp) 0.5 vcoul(igp) wt(p,igp)   GPP/gpp.cpp:362-390
   acht_n1_loc  = per-band split of achtemp
-  achstemp     = -1/2 sum_{n1,p} vcoul(igp) conj(aqsm(n1,igp)) sum_g aqsn(n1,g) I(p,g)  (|I|>to1 holds)
+  achstemp     = -1/2 sum_{n1,p} vcoul(igp) conj(aqsm(n1,igp)) sum_g aqsn(n1,g) I(p,g) [|I(p,g)| > to1]
   achsDtemp    = sum_{n1,p} 0.5 vcoul(igp) (-sum_g aqsn(n1,g) conj(aqsm(n1,igp)) R(1,p,g))  FF/ff.cpp:191-208
   achDtemp_cor = FF/ff.cpp:332-451 with fact1/fact2 from compute_fact (exact rationals)
+  achDtemp     = sum_{n1,p,g} aqsn(n1,g) conj(aqsm(n1,igp)) vcoul(igp) sum_f -i/2 pref(f) (R-A)(f,p,g)
+                 / (wx(n1) - s(n1) dFreqGrid(f)), s = -1 for n1 < nvband, +1 otherwise; pref the
+                 reference driver's weights FF/ff.cpp:619-626, evaluated in double as it does
 
 Run:  python tests/golden/make_exact_fixtures.py
 """
@@ -91,13 +94,31 @@ def ssx_exact(wx, wt, I):
     return ssx
 
 
-def gpp_case(seed, nb, nvband, ngpown, ncouls, near_pole=False):
+# I_eps values of the cut case: zero, below to1 and above it, each at least 5% from
+# the boundary |I| = to1 (so the double comparison cannot differ from the exact one)
+TINY_I = [Cx(0), Cx(F(1, 2**21)), Cx(0, F(1, 2**21)), Cx(F(1, 2**21), F(1, 2**21)), Cx(F(15, 2**24)),
+          Cx(F(9, 2**23)), Cx(0, F(-9, 2**23)), Cx(F(1, 2**19), F(1, 2**20)), Cx(F(3, 2**20), F(-1, 2**20)),
+          Cx(F(-7, 2**23), F(1, 2**22)), Cx(F(-5, 2**22), F(3, 2**22))]
+
+
+def cut_keeps(I):
+    """The static-COH cut |I_eps| > to1, exactly (on squares)."""
+    return I.norm2() > TO1 * TO1
+
+
+def gpp_case(seed, nb, nvband, ngpown, ncouls, near_pole=False, tiny_I=False):
     rng = random.Random(seed)
     aqsm = [Cx(dy(rng, -1, 1), dy(rng, -1, 1)) for _ in range(nb * ncouls)]
     aqsn = [Cx(dy(rng, -1, 1), dy(rng, -1, 1)) for _ in range(nb * ncouls)]
     I = [Cx(dy(rng, 0, 0.5), dy(rng, -0.05, 0.05)) for _ in range(ngpown * ncouls)]
-    # |I| > to1 for every entry (re >= 1/64), so the static-COH cut never applies
+    # |I| > to1 for every entry (re >= 1/64), so the static-COH cut never applies ...
     I = [Cx(max(x.re, F(1, 64)), x.im) for x in I]
+    if tiny_I:
+        # ... except here: every entry is zero or within a factor 4 of to1, on both sides
+        I = [TINY_I[k % len(TINY_I)] for k in range(ngpown * ncouls)]
+        for x in I:
+            r = x.norm2() / (TO1 * TO1)
+            assert x.norm2() == 0 or not F(9, 10) < r < F(11, 10)
     wt = [Cx(dy(rng, 0.5, 3), dy(rng, -0.1, 0)) for _ in range(ngpown * ncouls)]
     vcoul = [dy(rng, 0.1, 1) for _ in range(ncouls)]
     inv = [rng.randrange(ncouls + 1) for _ in range(ngpown)]  # may reach ncouls
@@ -128,7 +149,8 @@ def gpp_case(seed, nb, nvband, ngpown, ncouls, near_pole=False):
                 ach[n1][iw] = ach[n1][iw] + s * acc
             m = Cx(0)
             for g in range(ncouls):
-                m = m + aqsn[n1 * ncouls + g] * I[p * ncouls + g]
+                if cut_keeps(I[p * ncouls + g]):
+                    m = m + aqsn[n1 * ncouls + g] * I[p * ncouls + g]
             achs = achs + F(-1, 2) * vcoul[igp] * aqsm[n1 * ncouls + igp].conj() * m
     achtemp = [sum((ach[n1][iw] for n1 in range(nb)), Cx(0)) for iw in range(3)]
     asx = [Cx(0) for _ in range(3)]
@@ -237,18 +259,40 @@ def ff_case(seed, nb, nvband, ngpown, ncouls, nFreq, nfreqeval):
             for g in range(ncouls):
                 e = (sl(R, fi, p, g) - sl(A, fi, p, g)) * f1 + (sl(R, fi + 1, p, g) - sl(A, fi + 1, p, g)) * f2
                 cor = cor + aqsn[n1 * ncouls + g] * aqsm[n1 * ncouls + igp].conj() * e * vcoul[igp] * sc
+    # the real-axis frequency sum: grid and weights as the double values the kernels see
+    grid_d = [F(float(x)) for x in grid]
+    pref_d = []
+    for f in range(nFreq):
+        pref_d.append((float(grid_d[f + 1]) - float(grid_d[f])) / 3.14 if f < nFreq - 1 else pref_d[f - 1])
+    pref_d[0] *= 0.5
+    pref_d[nFreq - 1] *= 0.5
+    achD = Cx(0)
+    for n1 in range(nb):
+        wx = fmin - ekq[n1] + fstep
+        for f in range(nFreq):
+            den = wx + grid_d[f] if n1 < nvband else wx - grid_d[f]
+            if den == 0:
+                continue
+            w = Cx(0, F(-1, 2) * F(pref_d[f]) / den)
+            for p in range(ngpown):
+                igp = indinv[inv[p]]
+                m = Cx(0)
+                for g in range(ncouls):
+                    m = m + aqsn[n1 * ncouls + g] * (sl(R, f, p, g) - sl(A, f, p, g))
+                achD = achD + m * aqsm[n1 * ncouls + igp].conj() * vcoul[igp] * w
     return {
         "nb": nb, "nvband": nvband, "ngpown": ngpown, "ncouls": ncouls, "nFreq": nFreq, "nfreqeval": nfreqeval,
         "aqsm": [c.out() for c in aqsm], "aqsn": [c.out() for c in aqsn], "I_epsR": [c.out() for c in R],
         "I_epsA": [c.out() for c in A], "vcoul": [float(v) for v in vcoul], "inv": inv, "indinv": indinv,
         "ekq": [float(v) for v in ekq], "dFreqGrid": [float(v) for v in grid], "freqevalmin": 0.0,
-        "freqevalstep": 0.5, "occ": 1.0,
-        "expect": {"achsDtemp": achs.out(), "asxDtemp": asx.out(), "achDtemp_cor": cor.out()},
+        "freqevalstep": 0.5, "occ": 1.0, "pref": pref_d,
+        "expect": {"achsDtemp": achs.out(), "asxDtemp": asx.out(), "achDtemp_cor": cor.out(), "achDtemp": achD.out()},
     }
 
 
 def main():
-    cases = [gpp_case(11, 3, 1, 4, 8), gpp_case(12, 5, 2, 3, 7), gpp_case(13, 4, 2, 3, 6, near_pole=True)]
+    cases = [gpp_case(11, 3, 1, 4, 8), gpp_case(12, 5, 2, 3, 7), gpp_case(13, 4, 2, 3, 6, near_pole=True),
+             gpp_case(14, 4, 2, 3, 11, tiny_I=True)]
     (OUT / "exact_gpp.json").write_text(json.dumps(cases))
     ff = [ff_case(21, 6, 2, 3, 8, 5, 2), ff_case(22, 9, 3, 2, 6, 6, 3)]
     (OUT / "exact_ff.json").write_text(json.dumps(ff))

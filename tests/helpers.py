@@ -36,7 +36,7 @@ def load_exact_ff():
                  I_epsA=_c(c["I_epsA"]), vcoul=np.asarray(c["vcoul"], np.float64), inv=np.asarray(c["inv"], np.int32),
                  indinv=np.asarray(c["indinv"], np.int32), ekq=np.asarray(c["ekq"], np.float64),
                  dFreqGrid=np.asarray(c["dFreqGrid"], np.float64), freqevalmin=c["freqevalmin"],
-                 freqevalstep=c["freqevalstep"], occ=c["occ"])
+                 freqevalstep=c["freqevalstep"], occ=c["occ"], pref=np.asarray(c["pref"], np.float64))
         e = c["expect"]
         out.append((f, {k: complex(*v) for k, v in e.items()}))
     return out
@@ -72,6 +72,23 @@ def ff_reference_closed_form(nb, nvband, ncouls, ngpown, nFreq):
     k2 = np.sum(~(wx > 0) & occ)
     cor = ngpown * ncouls * (k1 * 0.5 * (-0.5) * 1j + k2 * 0.5 * (-0.5) * 1j)
     return achs, asx, cor
+
+
+def ff_reference_achDtemp(nb, nvband, ncouls, ngpown, nFreq):
+    """The real-axis frequency sum at the FF/ff.cpp:584-633 constants: aqsn conj(aqsm) = 0.5,
+    vcoul = 1 and R - A = i make every (n1, p, g, f) term 0.5 (-i/2 pref(f)/den) i
+    = pref(f) / (4 den), den = wx(n1) - s(n1) 2f with wx = 10.5 - n1 and s = -1 for
+    n1 < nvband; pref from FF/ff.cpp:619-626 on the grid 2f (spacing 2, so 2/3.14 with
+    halved ends).  Real-valued."""
+    pref = np.full(nFreq, 2.0 / 3.14)
+    pref[0] *= 0.5
+    pref[-1] *= 0.5
+    tot = 0.0
+    for n1 in range(nb):
+        wx = 10.5 - n1
+        s = -1.0 if n1 < nvband else 1.0
+        tot += np.sum(pref / (wx - s * 2.0 * np.arange(nFreq)))
+    return ngpown * ncouls * tot / 4.0
 
 
 def ff_with_kernel2_band(f):

@@ -7,6 +7,7 @@ import pytest
 import bgw
 import oracle as O
 from conftest import rel_err
+from helpers import load_exact_gpp
 
 pytestmark = pytest.mark.gpu
 TOL32 = 1e-6
@@ -25,7 +26,7 @@ def test_config0_fp32_bulk_vs_oracle(ctx32):
     o = O.gpp_forward(d)
     assert rel_err(r["achtemp"], o["achtemp"]) < TOL32
     assert rel_err(r["acht_n1_loc"], o["acht_n1_loc"]) < TOL32
-    assert rel_err(r["achstemp"], o["achstemp"]) < TOL32  # split-FP32 MFMA contraction
+    assert rel_err(r["achstemp"], o["achstemp"]) < TOL32  # FP32-operand MFMA, FP64 stage folds
     assert rel_err(r["asxtemp"], o["asxtemp"]) < TOL32
     # and it is not silently the FP64 path: the error is visibly above FP64 rounding
     assert rel_err(r["achtemp"], o["achtemp"]) > 1e-13
@@ -37,6 +38,18 @@ def test_band_counts_fp32_bulk(ctx32, nb):
     r = ctx32.gpp_solver(d, outputs=("achtemp", "acht_n1_loc"))
     o = O.gpp_forward(d, want=("achtemp", "acht_n1_loc"))
     assert rel_err(r["acht_n1_loc"], o["acht_n1_loc"]) < TOL32
+
+
+def test_achstemp_cut_fixture_fp32(ctx32):
+    """The exact case whose I_eps sit on both sides of achstemp's |I_eps| > to1 cut: the
+    FP32-operand contraction applies the cut like the FP64 one (a dropped or inverted
+    cut changes achstemp by order one)."""
+    d, exp = load_exact_gpp()[3]
+    r = ctx32.gpp_solver(d, occ=1.0)
+    o = O.gpp_forward(d, occ=1.0)
+    for k in ("achtemp", "acht_n1_loc", "achstemp", "asxtemp"):
+        assert rel_err(r[k], exp[k]) < TOL32, k
+        assert rel_err(r[k], o[k]) < TOL32, k
 
 
 def test_near_pole_tiles_use_fp64(ctx32):
@@ -78,7 +91,7 @@ def test_config2_full_fp32_bulk_vs_fp64(ctx):
     assert rel_err(r32["achtemp"], r64["achtemp"]) > 1e-13
 
 
-def test_ff_split_fp32_vs_oracle(ctx32):
+def test_ff_fp32_operand_vs_oracle(ctx32):
     f = bgw.ff_inputs(130, 5, 1000, 65, 15, 3, 1030)
     achs = ctx32.achsDtemp_Kernel(f)
     asx = ctx32.asxDtemp_Kernel(f)
@@ -91,7 +104,7 @@ def test_ff_split_fp32_vs_oracle(ctx32):
     assert rel_err(cor, O.achDtemp_cor(f)) > 1e-13
 
 
-def test_ff_split_fp32_long_valence_lists(ctx32):
+def test_ff_fp32_operand_long_valence_lists(ctx32):
     """nvband above 64: asxDtemp's slice lists can exceed the square row tile, so the
     FP32 form takes the 80-row tile for it too (as for achDtemp_cor)."""
     f = bgw.ff_inputs(100, 70, 600, 40, 15, 2, 4711)
@@ -114,7 +127,7 @@ def test_band_kernels_streaming_form_in_fp32_mode(ctx32):
 
 
 @pytest.mark.slow
-def test_ff_config4_shape_split_fp32(ctx32):
+def test_ff_config4_shape_fp32_operand(ctx32):
     f = bgw.ff_config4_inputs(ncouls=2048)
     achs = ctx32.achsDtemp_Kernel(f)
     cor = ctx32.achDtemp_cor_Kernel(f)

@@ -21,7 +21,7 @@ def _check_all(r, o, tol=TOL):
     assert rel_err(r["asxtemp"], o["asxtemp"]) < tol
 
 
-@pytest.mark.parametrize("case", range(3))
+@pytest.mark.parametrize("case", range(4))  # case 3: I_eps on both sides of achstemp's |I_eps| > to1 cut
 def test_exact_fixture(ctx, case):
     d, exp = load_exact_gpp()[case]
     r = ctx.gpp_solver(d, occ=1.0)

@@ -15,13 +15,13 @@ import pytest
 import bgw
 import oracle as O
 from conftest import rel_err
-from helpers import (ff_band_terms, ff_reference_closed_form, ff_with_kernel2_band, gpp_reference_achstemp,
-                     gpp_reference_closed_form, load_exact_ff, load_exact_gpp)
+from helpers import (ff_band_terms, ff_reference_achDtemp, ff_reference_closed_form, ff_with_kernel2_band,
+                     gpp_reference_achstemp, gpp_reference_closed_form, load_exact_ff, load_exact_gpp)
 
 # ------------------------------------------------------------------------- GPP
 
 
-@pytest.mark.parametrize("case", range(3))
+@pytest.mark.parametrize("case", range(4))
 def test_gpp_oracle_exact_fixture(case):
     d, exp = load_exact_gpp()[case]
     r = O.gpp_forward(d)
@@ -29,6 +29,33 @@ def test_gpp_oracle_exact_fixture(case):
     assert rel_err(r["acht_n1_loc"], exp["acht_n1_loc"]) < 1e-14
     assert rel_err(r["achstemp"], exp["achstemp"]) < 1e-14
     assert rel_err(r["asxtemp"], exp["asxtemp"]) < 1e-14  # case 2 covers near-pole and cut-off
+
+
+def _achstemp_numpy(d, keep):
+    """-1/2 sum_{n1,p} vcoul(igp) conj(aqsm(n1,igp)) sum_g aqsn(n1,g) I(p,g) keep(I(p,g))."""
+    nb, ng, nc = d["nb"], d["ngpown"], d["ncouls"]
+    igp = d["indinv"][d["inv"]]
+    I = d["I_eps"].reshape(ng, nc)
+    Im = np.where(keep(I), I, 0.0)
+    am, an = d["aqsm"].reshape(nb, nc), d["aqsn"].reshape(nb, nc)
+    m = an @ Im.T  # [nb, ng]
+    return -0.5 * np.sum(d["vcoul"][igp][None, :] * np.conj(am[:, igp]) * m)
+
+
+def test_achstemp_cut_fixture_discriminates():
+    """Case 3 of the exact GPP fixtures has every I_eps at zero or near the static-COH cut
+    |I_eps| > to1.  Its achstemp pins the cut: the right mask reproduces it, and a removed
+    mask, an inverted one, or one that compares |I|^2 with to1 instead of to1^2 miss it
+    by far more than the FP32 form's tolerance."""
+    d, exp = load_exact_gpp()[3]
+    to1 = 1e-6
+    n2 = lambda I: I.real ** 2 + I.imag ** 2  # noqa: E731
+    right = _achstemp_numpy(d, lambda I: n2(I) > to1 * to1)
+    assert rel_err([right], exp["achstemp"]) < 1e-14
+    kept = np.sum(n2(d["I_eps"]) > to1 * to1)
+    assert 0 < kept < np.count_nonzero(d["I_eps"])  # both sides of the cut are present
+    for wrong in (lambda I: np.ones(I.shape, bool), lambda I: ~(n2(I) > to1 * to1), lambda I: n2(I) > to1):
+        assert rel_err([_achstemp_numpy(d, wrong)], exp["achstemp"]) > 1e-2
 
 
 @pytest.mark.parametrize("nb,ngpown,ncouls", [(1, 1, 1), (3, 5, 17), (7, 33, 130), (12, 64, 1000)])
@@ -208,6 +235,8 @@ def test_ff_oracle_exact_fixture(case):
     assert np.all(np.abs(asx - exp["asxDtemp"]) <= 1e-14 * abs(exp["asxDtemp"]))
     cor = O.achDtemp_cor(f)
     assert np.all(np.abs(cor - exp["achDtemp_cor"]) <= 1e-14 * abs(exp["achDtemp_cor"]))
+    achD = O.achDtemp(f)
+    assert np.all(np.abs(achD - exp["achDtemp"]) <= 1e-14 * abs(exp["achDtemp"]))
 
 
 @pytest.mark.parametrize("nb,nvband,ncouls,ngpown,nFreq", [(20, 4, 64, 16, 15), (40, 25, 33, 7, 6), (9, 9, 10, 10, 3)])
@@ -217,6 +246,38 @@ def test_ff_oracle_closed_form_reference_constants(nb, nvband, ncouls, ngpown, n
     assert abs(O.achsDtemp(f) - achs) <= 1e-15 * abs(achs)
     assert np.all(np.abs(O.asxDtemp(f) - asx) <= 1e-15 * abs(asx))
     assert np.all(np.abs(O.achDtemp_cor(f) - cor) <= 1e-15 * abs(cor))
+    achD = ff_reference_achDtemp(nb, nvband, ncouls, ngpown, nFreq)
+    assert np.all(np.abs(O.achDtemp(f) - achD) <= 1e-13 * abs(achD))
+
+
+def test_pref_weights():
+    """FF/ff.cpp:619-626 on a uniform grid of spacing 1/2 and on an uneven one: interior
+    weights are the spacing over 3.14, the last point repeats the spacing before it, the
+    two ends are halved; the binding's helper and the oracle's agree."""
+    g = np.array([0.0, 0.5, 1.0, 1.5, 2.0])
+    assert np.array_equal(O.pref_weights(g), np.array([0.25, 0.5, 0.5, 0.5, 0.25]) / 3.14)
+    u = np.array([0.0, 0.25, 1.0, 3.0])
+    assert np.allclose(O.pref_weights(u), np.array([0.125, 0.75, 2.0, 1.0]) / 3.14, rtol=1e-15, atol=0)
+    assert np.array_equal(bgw.ff_pref(u), O.pref_weights(u))
+    with pytest.raises(ValueError):
+        bgw.ff_pref([1.0])
+
+
+def test_achDtemp_occupation_sign():
+    """The pole of an occupied band sits at -dFreqGrid, of an unoccupied one at
+    +dFreqGrid: with one frequency slice carrying R - A, moving band 0 between
+    occupied and unoccupied changes its term by exactly the ratio of its denominators."""
+    f = bgw.ff_inputs(3, 0, 16, 4, 4, 1, 31)
+    f["I_epsA"] = f["I_epsA"].copy()
+    npg = 4 * 16
+    for k in (0, 1, 3):  # only slice 2 has R != A
+        f["I_epsA"][k * npg:(k + 1) * npg] = f["I_epsR"][k * npg:(k + 1) * npg]
+    sub = dict(f, nb=1, aqsm=f["aqsm"][:16].copy(), aqsn=f["aqsn"][:16].copy(), ekq=f["ekq"][:1].copy())
+    unocc = O.achDtemp(dict(sub, nvband=0))[0]
+    occ = O.achDtemp(dict(sub, nvband=1))[0]
+    wx = f["freqevalmin"] - f["ekq"][0] + f["freqevalstep"]
+    w2 = f["dFreqGrid"][2]
+    assert abs(occ * (wx + w2) - unocc * (wx - w2)) <= 1e-14 * abs(unocc * (wx - w2))
 
 
 def test_ff_reference_default_goldens_vs_closed_form():
