@@ -300,6 +300,12 @@ def main():
                          "(for profiling one kernel)")
     ap.add_argument("--ff-only", action="store_true",
                     help="only run the FF leg (config 4) and print its results as one JSON line")
+    ap.add_argument("--adjoint-only", action="store_true",
+                    help="only run the adjoint leg (config 3: forward with all outputs, backward) and print one "
+                         "JSON line")
+    ap.add_argument("--full-outputs-only", action="store_true",
+                    help="only time the GPP step with all four outputs (achstemp and asxtemp included) at the "
+                         "benchmark workload and print one JSON line")
     ap.add_argument("--host-api-only", action="store_true",
                     help="only time the host-pointer entry points (GPP benchmark workload, FF config 4), "
                          "PCIe transfers included, and print one JSON line")
@@ -322,6 +328,15 @@ def main():
         print(json.dumps({"host_api": host_api_leg(bgw)}), flush=True)
         return
 
+    if args.adjoint_only:
+        stream = torch.cuda.Stream()
+        torch.cuda.set_stream(stream)
+        ctx = bgw.Context(0)
+        ctx.set_stream(stream.cuda_stream)
+        print(json.dumps({"adjoint_config3": adjoint_leg(bgw, torch, dev, stream, ctx)}), flush=True)
+        ctx.close()
+        return
+
     if args.ff_only:
         stream = torch.cuda.Stream()
         torch.cuda.set_stream(stream)
@@ -333,7 +348,7 @@ def main():
 
     log("generating inputs")
     d = bgw.gpp_config_inputs(1, nb=args.nb)
-    if args.form:
+    if args.form or args.full_outputs_only:
         args.no_extras = True
         args.no_cpu_baseline = True
     nb, nvband, ngpown, ncouls = d["nb"], d["nvband"], d["ngpown"], d["ncouls"]
@@ -369,6 +384,19 @@ def main():
 
     flops = bgw.gpp_flops(nb, ngpown, ncouls)
     res = {}
+    if args.full_outputs_only:
+        step(True)
+        ctx.reset_timing()
+        ctx.set_timing(True)
+        full_ms = timed(args.steps, full=True)
+        ctx.set_timing(False)
+        kt = {}
+        for key, kname in (("gpp_main", "gpp_main"), ("achstemp", "contract_achs"), ("asxtemp", "gpp_asx")):
+            kms, kn = ctx.kernel_time(kname)
+            kt[f"{key}_kernel_ms"] = kms / max(kn, 1)
+        print(json.dumps({"full_outputs_ms_per_step": full_ms, **kt}), flush=True)
+        ctx.close()
+        return
     for prec, name in ((bgw.BGW_FP64, "fp64"), (bgw.BGW_FP32_BULK, "fp32_bulk")):
         if args.form and name != args.form:
             continue
