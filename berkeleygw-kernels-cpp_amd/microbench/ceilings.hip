@@ -89,9 +89,14 @@ __global__ __launch_bounds__(256) void k_mfma_f64(const double* __restrict__ in,
   f64x4 acc[NACC];
 #pragma unroll
   for (int j = 0; j < NACC; ++j) acc[j] = (f64x4){0.0, 0.0, 0.0, 0.0};
-  for (int i = 0; i < iters; ++i) {
+  // 16 rounds per loop iteration: the compiler moves the accumulators between AGPRs and
+  // VGPRs at the loop's back edge (2 x 8 NACC v_accvgpr moves), which a short body would
+  // measure instead of the MFMAs
+  for (int i = 0; i < iters; i += 16) {
 #pragma unroll
-    for (int j = 0; j < NACC; ++j) acc[j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc[j], 0, 0, 0);
+    for (int u = 0; u < 16; ++u)
+#pragma unroll
+      for (int j = 0; j < NACC; ++j) acc[j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc[j], 0, 0, 0);
   }
   double s = 0.0;
 #pragma unroll
@@ -106,9 +111,11 @@ __global__ __launch_bounds__(256) void k_mfma4_f64(const double* __restrict__ in
   double acc[NACC];
 #pragma unroll
   for (int j = 0; j < NACC; ++j) acc[j] = 0.0;
-  for (int i = 0; i < iters; ++i) {
+  for (int i = 0; i < iters; i += 16) {  // 16 rounds per iteration, as in k_mfma_f64
 #pragma unroll
-    for (int j = 0; j < NACC; ++j) acc[j] = __builtin_amdgcn_mfma_f64_4x4x4f64(a, b, acc[j], 0, 0, 0);
+    for (int u = 0; u < 16; ++u)
+#pragma unroll
+      for (int j = 0; j < NACC; ++j) acc[j] = __builtin_amdgcn_mfma_f64_4x4x4f64(a, b, acc[j], 0, 0, 0);
   }
   double s = 0.0;
 #pragma unroll
