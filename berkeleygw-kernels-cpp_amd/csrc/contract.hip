@@ -75,18 +75,30 @@ __global__ __launch_bounds__(NT, 2) void k_contract(ContractArgs a) {
 
   // Work items are (G' tile ct, live row tile t, K split ks), ct fastest.  The live
   // row tiles of all jobs are numbered densely by tile_start (prefix of
-  // ceil(cnt/BM)), so empty jobs and empty row-tile tails take no workgroup.
-  // XCD-aware order: the hardware deals workgroup L to XCD L % 8; chunks of
-  // a.chunk consecutive items (the G' tiles of one row tile for dense row lists,
-  // of two row tiles for sparse plans) are dealt round-robin over the XCDs.  The
-  // G' tiles of a row tile (sharing its A rows) and the row tiles of a slice
-  // (sharing its B tile) then run side by side on one XCD and meet in its L2,
-  // while every XCD gets an even share of the work.
-  const int slot = (int)(blockIdx.x / kXcds);
-  const int item = ((slot / a.chunk) * kXcds + (int)(blockIdx.x % kXcds)) * a.chunk + slot % a.chunk;
+  // ceil(cnt/BM)), so empty jobs and empty row-tile tails take no workgroup.  The
+  // hardware deals workgroup L to XCD L % 8; the order of the items over the XCDs
+  // decides what meets in each XCD's L2.
+  //   Dense row lists (a.job_tiles row tiles per job, K splits a multiple of 8):
+  //   XCD x takes the K splits ks = x (mod 8), its items run row tile fastest, then
+  //   G' tile, then job.  The row tiles that share a B tile and the G' tiles that
+  //   share an A row block then run side by side on one XCD, and each XCD reads only
+  //   its K range of A and of the slices.
+  //   Sparse plans: chunks of a.chunk consecutive items (the G' tiles of two row
+  //   tiles, one or two frequency slices) are dealt round-robin over the XCDs, so the
+  //   G' tiles of a row tile (sharing its A rows) meet in one L2.
+  const int ntn = (a.N + BN - 1) / BN;
+  int item;
+  if (a.job_tiles) {
+    const int x = (int)(blockIdx.x % kXcds), q = (int)(blockIdx.x / kXcds), per_ks = a.tiles * ntn;
+    const int ks_ = (q / per_ks) * kXcds + x, r = q % per_ks;
+    const int rt_ = r % a.job_tiles, ct_ = (r / a.job_tiles) % ntn, job_ = r / a.job_tiles / ntn;
+    item = ct_ + ntn * (job_ * a.job_tiles + rt_ + a.tiles * ks_);
+  } else {
+    const int slot = (int)(blockIdx.x / kXcds);
+    item = ((slot / a.chunk) * kXcds + (int)(blockIdx.x % kXcds)) * a.chunk + slot % a.chunk;
+  }
   if (item >= a.items) return;  // grid padding to whole chunks
   const size_t pid = (size_t)item;
-  const int ntn = (a.N + BN - 1) / BN;
   const int ct = item % ntn;
   const int t = (item / ntn) % a.tiles;
   const int ks = item / ntn / a.tiles;
@@ -385,16 +397,16 @@ int resident_slots(bgw_ctx* ctx, const void* kern) {
 
 // K split minimising the estimated makespan: whole rounds of `slots` resident
 // workgroups, each item costing its bk-wide stages plus ~2 stages of epilogue.
-int pick_ksplit(int K, int BK, size_t tiles_x_ntn, int slots) {
-  int best = 1;
+int pick_ksplit(int K, int BK, size_t tiles_x_ntn, int slots, int step) {
+  int best = step;
   double best_cost = 0.0;
-  const int max_ks = std::max(1, std::min(64, K / BK));
-  for (int ks = 1; ks <= max_ks; ++ks) {
+  const int max_ks = std::max(step, std::min(64, K / BK));
+  for (int ks = step; ks <= max_ks; ks += step) {
     const double items = (double)tiles_x_ntn * ks;
     const double rounds = std::ceil(items / slots);
     const double stages = std::ceil(std::ceil((double)K / ks) / BK);
     const double cost = rounds * (stages + 2.0);
-    if (ks == 1 || cost < best_cost * 0.999) {
+    if (ks == step || cost < best_cost * 0.999) {
       best = ks;
       best_cost = cost;
     }
@@ -528,8 +540,13 @@ int contract_launch(bgw_ctx* ctx, ContractArgs a, int njobs, int bm, int max_til
   const int bk = BK;
   const int slots = resident_slots(ctx, (const void*)kern);
   int ksplit;
+  a.job_tiles = 0;
   if (exact_tiles) {
-    ksplit = pick_ksplit(a.K, bk, (size_t)ntn * max_tiles, slots);
+    // dense lists: K splits in multiples of the XCD count when K allows (one split
+    // per XCD in turn, see k_contract's work order)
+    const bool per_xcd = a.K / bk >= kXcds && max_tiles % njobs == 0;
+    ksplit = pick_ksplit(a.K, bk, (size_t)ntn * max_tiles, slots, per_xcd ? kXcds : 1);
+    if (per_xcd) a.job_tiles = max_tiles / njobs;
   } else {
     const long long est = std::max(1, std::min(max_tiles, est_tiles));
     const long long want = (4LL * slots + (long long)ntn * est - 1) / ((long long)ntn * est);
