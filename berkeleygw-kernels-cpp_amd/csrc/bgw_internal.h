@@ -158,10 +158,14 @@ int gpp_backward_device(bgw_ctx* ctx, size_t nb, size_t nvband, size_t ngpown, s
                         double* dsigma, double* dwt, double* dI);
 
 // FF: op 0 = achsDtemp, 1 = asxDtemp, 2 = achDtemp_cor, 3 = achDtemp (frequency sum,
-// reads pref[nFreq]; I_epsA null means I_epsR holds R - A)
+// reads pref[nFreq]; I_epsA null means I_epsR holds R - A).  slice_map (device, ops 1
+// and 2, nullable): the slice of I_epsR / I_epsA each plan job reads (asxDtemp: jobs
+// [0, nFreq) on I_epsR, [nFreq, 2 nFreq) on I_epsA), for arrays that hold only the
+// touched slices, packed; null: job j reads slice j % nFreq.
 int ff_device(bgw_ctx* ctx, int op, int nb, int nvband, int nfreqeval, int ncouls, int ngpown,
               int nFreq, double freqevalmin, double freqevalstep, double occ, const double* ekq,
               const double* dFreqGrid, const int* inv, const int* indinv, const double* aqsm,
               const double* aqsn, const double* vcoul, const double* I_epsR,
-              const double* I_epsA, double* out, const double* pref = nullptr);
+              const double* I_epsA, double* out, const double* pref = nullptr,
+              const int* slice_map = nullptr);
 }  // namespace bgw

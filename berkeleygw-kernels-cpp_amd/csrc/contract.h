@@ -59,9 +59,10 @@ int contract_launch(bgw_ctx* ctx, ContractArgs a, int njobs, int bm, int max_til
 // (registers), which makes short FP32 lists faster in the square tile.
 int contract_row_tile(bool f32, bool sparse, int max_job_rows);
 // Sparse plans with at most kStreamRows rows per job (and no mask): the streaming
-// form, FP64 in both precision forms (it is bound by reading the slices).
+// form, FP64 in both precision forms (it is bound by reading the slices).  live_bound:
+// a host bound on the jobs with rows; the grid covers that many jobs, not njobs.
 constexpr int kStreamRows = 4;
-int contract_stream(bgw_ctx* ctx, ContractArgs a, int njobs, int max_job_rows, const char* tname);
+int contract_stream(bgw_ctx* ctx, ContractArgs a, int njobs, int max_job_rows, int live_bound, const char* tname);
 
 // rows[i] = i, wts[i] = w, cnt[0] = n, slice[0] = slice.
 void launch_identity_rows(int* rows, double2* wts, int* cnt, int* slice, int n, double2 w, int slice_idx,

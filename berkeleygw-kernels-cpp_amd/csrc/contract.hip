@@ -424,11 +424,29 @@ int pick_ksplit(int K, int BK, size_t tiles_x_ntn, int slots, int step) {
 // MFMA epilogue does.  One double-double partial per workgroup, zero for empty jobs.
 constexpr int kStreamNT = 256, kStreamUnroll = 4, kStreamPPer = 2;
 
+//
+// Workgroup (live job li, pair pb): the live jobs (cnt > 0) are numbered densely by
+// a.tile_start (the prefix of [cnt > 0]), so a plan of many frequency slices with a few
+// live ones launches workgroups for the live ones only; slots beyond the live count write
+// a zero partial, which keeps the reduction's order fixed.
 __global__ __launch_bounds__(kStreamNT) void k_contract_stream(ContractArgs a, int pblocks) {
-  const int job = blockIdx.x / pblocks, pb = blockIdx.x - job * pblocks;
+  const int li = blockIdx.x / pblocks, pb = blockIdx.x - li * pblocks;
   const int tid = threadIdx.x;
   __shared__ cdd red[kStreamNT / 64];
-  const int nrows = a.cnt[job];
+  int job = 0, nrows = 0;
+  if (li < a.tile_start[a.njobs]) {
+    // job = the largest j with tile_start[j] <= li (jobs without rows repeat a value)
+    int lo = 0, hi = a.njobs;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (a.tile_start[mid] <= li)
+        lo = mid;
+      else
+        hi = mid - 1;
+    }
+    job = lo;
+    nrows = a.cnt[job];
+  }
   cdd acc = cdd_zero();
   if (nrows > 0) {
     const int slice = a.slice_of_job[job];
@@ -585,19 +603,27 @@ int contract_launch(bgw_ctx* ctx, ContractArgs a, int njobs, int bm, int max_til
   return check_launch(ctx, "reduce");
 }
 
-int contract_stream(bgw_ctx* ctx, ContractArgs a, int njobs, int max_job_rows, const char* tname) {
+int contract_stream(bgw_ctx* ctx, ContractArgs a, int njobs, int max_job_rows, int live_bound, const char* tname) {
   if (max_job_rows > kStreamRows || a.thr2 > 0.0)
     return fail(ctx, BGW_EINVAL, "streaming contraction: a job has more rows than it handles");
   const int pblocks = (a.N + kStreamPPer - 1) / kStreamPPer;
-  const long long W = (long long)njobs * pblocks;
+  const int live = std::max(0, std::min(njobs, live_bound));
+  const long long W = (long long)live * pblocks;
   if (W < 1) {
     fill_zero(a.out, sizeof(double2), ctx->stream);
     return check_launch(ctx, "fill");
   }
   if (W > INT32_MAX) return fail(ctx, BGW_EINVAL, "contraction grid too large");
-  void* pp;
-  int rc = pool_get(ctx, "ws_contract", (size_t)W * sizeof(double4), &pp);
+  void *pts, *pp;
+  int rc = pool_get(ctx, "ws_tile_start", (size_t)(njobs + 1) * sizeof(int), &pts);
+  if (!rc) rc = pool_get(ctx, "ws_contract", (size_t)W * sizeof(double4), &pp);
   if (rc) return rc;
+  // live-job numbering: the tile prefix with one tile per job that has rows
+  hipLaunchKernelGGL(k_tile_prefix, dim3(1), dim3(kPrefixThreads), 0, ctx->stream, a.cnt, njobs, 1 << 30,
+                     static_cast<int*>(pts));
+  if ((rc = check_launch(ctx, "k_tile_prefix"))) return rc;
+  a.tile_start = static_cast<const int*>(pts);
+  a.njobs = njobs;
   a.partial = static_cast<double4*>(pp);
   {
     KernelTimer kt(ctx, tname);

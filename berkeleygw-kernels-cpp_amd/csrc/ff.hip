@@ -30,7 +30,7 @@ __global__ __launch_bounds__(256) void k_ff_plan(int op, int nb, int nvband, int
                                                  const double* __restrict__ grid, int* __restrict__ rows,
                                                  double2* __restrict__ wts, int* __restrict__ cnt,
                                                  int* __restrict__ slice_of_job, int* __restrict__ band_job,
-                                                 double2* __restrict__ band_w) {
+                                                 double2* __restrict__ band_w, const int* __restrict__ slice_map) {
   const int njobs = op == 1 ? 2 * nFreq : nFreq;
   const int nbands = op == 1 ? nvband : nb;
   // 1) compute_fact for every band in parallel: band_job = (first job, or -1), band_w = (w1, w2)
@@ -82,7 +82,7 @@ __global__ __launch_bounds__(256) void k_ff_plan(int op, int nb, int nvband, int
     }
     if (lane == 0) {
       cnt[j] = c;
-      slice_of_job[j] = j % nFreq;
+      slice_of_job[j] = slice_map ? slice_map[j] : j % nFreq;
     }
   }
 }
@@ -129,7 +129,8 @@ __global__ void k_cor_band_plan(int* __restrict__ rows, double2* __restrict__ wt
 int ff_device(bgw_ctx* ctx, int op, int nb, int nvband, int nfreqeval, int ncouls, int ngpown, int nFreq,
               double freqevalmin, double freqevalstep, double occ, const double* ekq, const double* dFreqGrid,
               const int* inv, const int* indinv, const double* aqsm, const double* aqsn, const double* vcoul,
-              const double* I_epsR, const double* I_epsA, double* out, const double* pref) {
+              const double* I_epsR, const double* I_epsA, double* out, const double* pref,
+              const int* slice_map) {
   hipStream_t s = ctx->stream;
   launch_check_indices(inv, (size_t)ngpown, indinv, (size_t)ncouls, ncouls - 1, ncouls, ctx->d_flag, s);
   ContractArgs a{};
@@ -192,13 +193,15 @@ int ff_device(bgw_ctx* ctx, int op, int nb, int nvband, int nfreqeval, int ncoul
   if ((rc = pool_get(ctx, "ws_ff_band_w", (size_t)nb * 16, &pbw))) return rc;
   hipLaunchKernelGGL(k_ff_plan, dim3(1), dim3(256), 0, s, op, nb, nvband, nFreq, freqevalmin, freqevalstep, occ,
                      ekq, dFreqGrid, static_cast<int*>(prow), static_cast<double2*>(pw), static_cast<int*>(pcnt),
-                     static_cast<int*>(pslice), static_cast<int*>(pbj), static_cast<double2*>(pbw));
+                     static_cast<int*>(pslice), static_cast<int*>(pbj), static_cast<double2*>(pbw), slice_map);
   if ((rc = check_launch(ctx, "k_ff_plan"))) return rc;
-  // every listed band is a row of exactly two jobs, so sum_j ceil(cnt_j/BM) is at
-  // most njobs + ceil(2*maxrows/BM), and never above njobs * ceil(maxrows/BM)
+  // every listed band (at most maxrows of them) is a row of exactly two jobs, so at most
+  // min(njobs, 2*maxrows) jobs have rows and sum_j ceil(cnt_j/BM) is at most that plus
+  // ceil(2*maxrows/BM), and never above njobs * ceil(maxrows/BM)
   const int BMr = contract_row_tile(f32, true, maxrows);
+  const long long live_jobs = std::min<long long>(njobs, 2LL * maxrows);
   const long long tiles_dense = (long long)njobs * ((maxrows + BMr - 1) / BMr);
-  const long long tiles_rows = (long long)njobs + (2LL * maxrows + BMr - 1) / BMr;
+  const long long tiles_rows = live_jobs + (2LL * maxrows + BMr - 1) / BMr;
   const int max_tiles = (int)(tiles_dense < tiles_rows ? tiles_dense : tiles_rows);
   // live tiles: at most one per listed row when rows are few (2*maxrows rows in all)
   const int est_tiles = (int)std::min<long long>(max_tiles, 2LL * maxrows);
@@ -208,7 +211,7 @@ int ff_device(bgw_ctx* ctx, int op, int nb, int nvband, int nfreqeval, int ncoul
     a.jobs_a = nFreq;
     // a job's rows are valence bands: few of them make the slices' reading the bound
     if (nvband <= kStreamRows)
-      rc = contract_stream(ctx, a, njobs, nvband, "contract_ff_asx");
+      rc = contract_stream(ctx, a, njobs, nvband, std::min(njobs, 2 * nvband), "contract_ff_asx");
     else
       rc = contract_launch(ctx, a, njobs, BMr, max_tiles, false, est_tiles, "contract_ff_asx");
   } else {
@@ -260,7 +263,7 @@ int cor_band_device(bgw_ctx* ctx, double scale, int ncouls, int ngpown, int ifre
   a.ncouls = ncouls;
   a.thr2 = 0.0;
   a.out = reinterpret_cast<double2*>(out);
-  return contract_stream(ctx, a, 2, 1, "contract_ff_corband");
+  return contract_stream(ctx, a, 2, 1, 2, "contract_ff_corband");
 }
 
 }  // namespace bgw
@@ -315,6 +318,7 @@ int ff_host(bgw_ctx* ctx, int op, int nb, int nvband, int nfreqeval, int ncouls,
   const int *d_inv, *d_indinv;
   const double *d_ekq = nullptr, *d_grid = nullptr, *d_pref = nullptr, *d_am, *d_an, *d_vc, *d_R = nullptr,
                *d_A = nullptr;
+  const int* d_map = nullptr;
   if ((rc = upload(ctx, "h_inv", inv, (size_t)ngpown, &d_inv))) return rc;
   if ((rc = upload(ctx, "h_indinv", indinv, (size_t)ncouls, &d_indinv))) return rc;
   // asxDtemp reads only the valence rows n1 < nvband of aqsm / aqsn (its row lists
@@ -336,8 +340,7 @@ int ff_host(bgw_ctx* ctx, int op, int nb, int nvband, int nfreqeval, int ncouls,
     // Only the slices the device plan (k_ff_plan) will read are transferred.  The
     // slice of a band is ifreq (and ifreq + 1) of compute_fact, the same function
     // on host and device, and wx is formed with the same two exact-order operations,
-    // so the sets agree; untouched slices of the device buffer are never read.
-    // The frequency sum (op 3) reads every slice.
+    // so the sets agree.  The frequency sum (op 3) reads every slice.
     std::vector<char> needR(nFreq, op == 3), needA(nFreq, op == 3);
     const int nbands = op == 1 ? nvband : (op == 2 ? nb : 0);
     for (int n1 = 0; n1 < nbands; ++n1) {
@@ -355,26 +358,45 @@ int ff_host(bgw_ctx* ctx, int op, int nb, int nvband, int nfreqeval, int ncouls,
       }
     }
     void *pR, *pA = nullptr;
-    if ((rc = pool_get(ctx, "h_epsR", 2 * npg * nFreq * sizeof(double), &pR))) return rc;
     const size_t slice = npg * 16;
-    if (op == 2 || op == 3) {
-      // achDtemp_cor and the frequency sum read their slices only as I_epsR - I_epsA
-      // (FF/ff.cpp:399-404, :436-441):
+    if (op == 3) {
+      // the frequency sum reads every slice, as I_epsR - I_epsA (FF/ff.cpp:436-441 form):
       // the host forms the difference while staging, so half the bytes cross the bus, and
       // the contraction runs on it in its plain mode (the same IEEE subtraction as on the
       // device, so the result equals the device entry point's bit for bit)
+      if ((rc = pool_get(ctx, "h_epsR", slice * nFreq, &pR))) return rc;
       for (int f = 0; f < nFreq; ++f)
-        if (needR[f] && (rc = upload_host_diff(ctx, static_cast<double*>(pR) + 2 * npg * f, R + 2 * npg * f,
-                                               A + 2 * npg * f, 2 * npg)))
+        if ((rc = upload_host_diff(ctx, static_cast<double*>(pR) + 2 * npg * f, R + 2 * npg * f, A + 2 * npg * f,
+                                   2 * npg)))
           return rc;
     } else {
-      if ((rc = pool_get(ctx, "h_epsA", 2 * npg * nFreq * sizeof(double), &pA))) return rc;
+      // The touched slices are staged packed, in ascending order, so the device holds
+      // only them; slice_map gives every plan job its packed slice (a job whose slice is
+      // untouched has no rows and reads nothing).  asxDtemp stages R and A slices apart;
+      // achDtemp_cor reads its slices only as I_epsR - I_epsA (FF/ff.cpp:399-404), which
+      // the host forms while staging, as for the frequency sum.
+      std::vector<int> map(op == 1 ? 2 * (size_t)nFreq : (size_t)nFreq, 0);
+      int nR = 0, nA = 0;
+      for (int f = 0; f < nFreq; ++f)
+        if (needR[f]) map[f] = nR++;
+      if (op == 1)
+        for (int f = 0; f < nFreq; ++f)
+          if (needA[f]) map[nFreq + f] = nA++;
+      if ((rc = pool_get(ctx, "h_epsR", slice * std::max(nR, 1), &pR))) return rc;
+      if (op == 1 && (rc = pool_get(ctx, "h_epsA", slice * std::max(nA, 1), &pA))) return rc;
       for (int f = 0; f < nFreq; ++f) {
-        if (needR[f] && (rc = upload_host(ctx, static_cast<char*>(pR) + f * slice, R + 2 * npg * f, slice)))
-          return rc;
-        if (needA[f] && (rc = upload_host(ctx, static_cast<char*>(pA) + f * slice, A + 2 * npg * f, slice)))
-          return rc;
+        if (!needR[f]) continue;
+        char* dst = static_cast<char*>(pR) + map[f] * slice;
+        rc = op == 2 ? upload_host_diff(ctx, reinterpret_cast<double*>(dst), R + 2 * npg * f, A + 2 * npg * f, 2 * npg)
+                     : upload_host(ctx, dst, R + 2 * npg * f, slice);
+        if (rc) return rc;
       }
+      if (op == 1)
+        for (int f = 0; f < nFreq; ++f)
+          if (needA[f] &&
+              (rc = upload_host(ctx, static_cast<char*>(pA) + map[nFreq + f] * slice, A + 2 * npg * f, slice)))
+            return rc;
+      if ((rc = upload(ctx, "h_smap", map.data(), map.size(), &d_map))) return rc;
     }
     d_R = static_cast<const double*>(pR);
     d_A = static_cast<const double*>(pA);  // null for achDtemp_cor and achDtemp: d_R holds R - A
@@ -382,7 +404,7 @@ int ff_host(bgw_ctx* ctx, int op, int nb, int nvband, int nfreqeval, int ncouls,
   void* d_out;
   if ((rc = pool_get(ctx, "h_ffout", (size_t)(nfreqeval > 1 ? nfreqeval : 1) * 16, &d_out))) return rc;
   rc = ff_device(ctx, op, nb, nvband, nfreqeval, ncouls, ngpown, nFreq, fmin, fstep, occ, d_ekq, d_grid,
-                 d_inv, d_indinv, d_am, d_an, d_vc, d_R, d_A, static_cast<double*>(d_out), d_pref);
+                 d_inv, d_indinv, d_am, d_an, d_vc, d_R, d_A, static_cast<double*>(d_out), d_pref, d_map);
   if (rc) {
     (void)hipStreamSynchronize(ctx->stream);
     return rc;
