@@ -38,6 +38,7 @@ struct ContractArgs {
   const int* tile_start;   // set by contract_launch: [njobs+1] prefix of ceil(cnt/row tile)
   int items;               // set by contract_launch: workgroup tasks (= partials)
   int chunk;               // set by contract_launch: consecutive items kept on one XCD
+  int xcd_ks;              // set by contract_launch: sparse plan dealt one K split per XCD
   int job_tiles;           // set by contract_launch: row tiles per job of a dense plan dealt
                            //   one K split per XCD (0: chunked order)
   double4* partial;        // set by contract_launch

@@ -83,9 +83,11 @@ __global__ __launch_bounds__(NT, 2) void k_contract(ContractArgs a) {
   //   G' tile, then job.  The row tiles that share a B tile and the G' tiles that
   //   share an A row block then run side by side on one XCD, and each XCD reads only
   //   its K range of A and of the slices.
-  //   Sparse plans: chunks of a.chunk consecutive items (the G' tiles of two row
-  //   tiles, one or two frequency slices) are dealt round-robin over the XCDs, so the
-  //   G' tiles of a row tile (sharing its A rows) meet in one L2.
+  //   Sparse plans (a.xcd_ks, K splits a multiple of 8): XCD x takes the K splits
+  //   ks = x (mod 8), its items G' tile fastest, then row tile, so the G' tiles of a row
+  //   tile (sharing its A rows) meet in one L2 and each XCD reads its K range only.
+  //   Otherwise chunks of a.chunk consecutive items (the G' tiles of two row tiles) are
+  //   dealt round-robin over the XCDs.
   const int ntn = (a.N + BN - 1) / BN;
   int item;
   if (a.job_tiles) {
@@ -93,6 +95,9 @@ __global__ __launch_bounds__(NT, 2) void k_contract(ContractArgs a) {
     const int ks_ = (q / per_ks) * kXcds + x, r = q % per_ks;
     const int rt_ = r % a.job_tiles, ct_ = (r / a.job_tiles) % ntn, job_ = r / a.job_tiles / ntn;
     item = ct_ + ntn * (job_ * a.job_tiles + rt_ + a.tiles * ks_);
+  } else if (a.xcd_ks) {
+    const int x = (int)(blockIdx.x % kXcds), q = (int)(blockIdx.x / kXcds), per_ks = a.tiles * ntn;
+    item = q % per_ks + per_ks * ((q / per_ks) * kXcds + x);
   } else {
     const int slot = (int)(blockIdx.x / kXcds);
     item = ((slot / a.chunk) * kXcds + (int)(blockIdx.x % kXcds)) * a.chunk + slot % a.chunk;
@@ -559,6 +564,7 @@ int contract_launch(bgw_ctx* ctx, ContractArgs a, int njobs, int bm, int max_til
   const int slots = resident_slots(ctx, (const void*)kern);
   int ksplit;
   a.job_tiles = 0;
+  a.xcd_ks = 0;
   if (exact_tiles) {
     // dense lists: K splits in multiples of the XCD count when K allows (one split
     // per XCD in turn, see k_contract's work order)
@@ -570,6 +576,12 @@ int contract_launch(bgw_ctx* ctx, ContractArgs a, int njobs, int bm, int max_til
     const long long want = (4LL * slots + (long long)ntn * est - 1) / ((long long)ntn * est);
     ksplit = (int)std::max<long long>(a.K / 1024, want);
     ksplit = std::max(1, std::min({ksplit, 64, std::max(1, a.K / bk)}));
+    // one K split per XCD in turn when K allows (achDtemp_cor at config 4: 4% faster,
+    // 6.39e9 B fetched instead of 6.51e9)
+    if (a.K / bk >= kXcds) {
+      ksplit = std::min(64, (ksplit + kXcds - 1) / kXcds * kXcds);
+      a.xcd_ks = 1;
+    }
   }
   a.ksplit = ksplit;
   a.tiles = max_tiles;
