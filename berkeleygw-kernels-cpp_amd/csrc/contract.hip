@@ -587,8 +587,10 @@ int contract_launch(bgw_ctx* ctx, ContractArgs a, int njobs, int bm, int max_til
   a.tiles = max_tiles;
   a.njobs = njobs;
   const size_t W = (size_t)ntn * ksplit * max_tiles;
-  // dense lists: one row tile's G' tiles per chunk (measured 6-8% faster for
-  // achsDtemp than two); sparse plans: two row tiles (one or two frequency slices)
+  // the chunked order (K below 8 stages): one row tile's G' tiles per chunk for dense
+  // lists (measured 6-8% faster for achsDtemp than two), two row tiles (one or two
+  // frequency slices) for sparse plans.  Under the per-XCD orders the grid is rounded
+  // the same way; blocks past the items return at once.
   a.chunk = (exact_tiles ? 1 : 2) * ntn;
   const size_t per_round = (size_t)kXcds * a.chunk;
   const size_t G = (W + per_round - 1) / per_round * per_round;
