@@ -93,6 +93,23 @@ def test_config4_slices_reduced_bands(ctx):
     assert rel_err(ctx.achDtemp_Kernel(sub), O.achDtemp(sub)) < TOL
 
 
+@pytest.mark.slow
+def test_config4_full_vs_oracle(ctx):
+    """Config 4 at full size (512 bands, 15 slices of 819 x 16384): both precision forms of
+    the frequency sum against the oracle on the same inputs (the oracle's compensated FP64
+    sum takes a few tens of seconds on the host cores)."""
+    f = bgw.ff_config4_inputs()
+    ref = O.achDtemp(f)
+    assert rel_err(ctx.achDtemp_Kernel(f), ref) < TOL
+    ctx.set_precision(bgw.BGW_FP32_BULK)
+    try:
+        got32 = ctx.achDtemp_Kernel(f)
+    finally:
+        ctx.set_precision(bgw.BGW_FP64)
+    assert rel_err(got32, ref) < 1e-6
+    assert rel_err(got32, ref) > 1e-13  # the FP32-operand path ran
+
+
 def test_fp32_operand_form(ctx):
     f = bgw.ff_inputs(130, 5, 1000, 65, 15, 3, 2030)
     ref = O.achDtemp(f)

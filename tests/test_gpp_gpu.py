@@ -196,6 +196,15 @@ def test_config1_full_properties(ctx):
 
 
 @pytest.mark.slow
+def test_config1_full_all_outputs_vs_oracle(ctx):
+    """The benchmark workload at full size with all four outputs (the static COH term's
+    masked MFMA contraction and the SX term included) against the oracle on the same
+    inputs; the oracle's compensated sums take a few tens of seconds on the host cores."""
+    d = bgw.gpp_config_inputs(1)
+    _check_all(ctx.gpp_solver(d), O.gpp_forward(d))
+
+
+@pytest.mark.slow
 @pytest.mark.skipif(not O.REF_OMP.exists(), reason="reference build (oracle/_ref) absent")
 def test_config1_full_vs_reference_build(ctx):
     """The benchmark workload at full size (512 bands, ngpown 1638, ncouls 32768, wx per
