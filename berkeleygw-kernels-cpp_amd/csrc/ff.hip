@@ -319,6 +319,7 @@ int ff_host(bgw_ctx* ctx, int op, int nb, int nvband, int nfreqeval, int ncouls,
   const double *d_ekq = nullptr, *d_grid = nullptr, *d_pref = nullptr, *d_am, *d_an, *d_vc, *d_R = nullptr,
                *d_A = nullptr;
   const int* d_map = nullptr;
+  std::vector<int> map;  // host copy of the packed-slice map
   if ((rc = upload(ctx, "h_inv", inv, (size_t)ngpown, &d_inv))) return rc;
   if ((rc = upload(ctx, "h_indinv", indinv, (size_t)ncouls, &d_indinv))) return rc;
   // asxDtemp reads only the valence rows n1 < nvband of aqsm / aqsn (its row lists
@@ -375,7 +376,7 @@ int ff_host(bgw_ctx* ctx, int op, int nb, int nvband, int nfreqeval, int ncouls,
       // untouched has no rows and reads nothing).  asxDtemp stages R and A slices apart;
       // achDtemp_cor reads its slices only as I_epsR - I_epsA (FF/ff.cpp:399-404), which
       // the host forms while staging, as for the frequency sum.
-      std::vector<int> map(op == 1 ? 2 * (size_t)nFreq : (size_t)nFreq, 0);
+      map.assign(op == 1 ? 2 * (size_t)nFreq : (size_t)nFreq, 0);
       int nR = 0, nA = 0;
       for (int f = 0; f < nFreq; ++f)
         if (needR[f]) map[f] = nR++;
@@ -397,6 +398,10 @@ int ff_host(bgw_ctx* ctx, int op, int nb, int nvband, int nfreqeval, int ncouls,
               (rc = upload_host(ctx, static_cast<char*>(pA) + map[nFreq + f] * slice, A + 2 * npg * f, slice)))
             return rc;
       if ((rc = upload(ctx, "h_smap", map.data(), map.size(), &d_map))) return rc;
+      // the map (4 B per plan job) comes from a host vector: wait for its copy here, so no later
+      // early return can leave the copy reading freed memory
+      const hipError_t e = hipStreamSynchronize(ctx->stream);
+      if (e != hipSuccess) return hip_fail(ctx, e, "slice map upload");
     }
     d_R = static_cast<const double*>(pR);
     d_A = static_cast<const double*>(pA);  // null for achDtemp_cor and achDtemp: d_R holds R - A
