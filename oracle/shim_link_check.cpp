@@ -1,8 +1,8 @@
 // shim_link_check.cpp -- link check of host/bgw_reference_api.hpp (TEST INFRASTRUCTURE).
 //
-// A translation unit shaped like the reference's GPP/gpp.cpp and FF/ff.cpp after the
-// integration of INTEGRATION.md: it includes the reference's common/Defines.h, then the
-// shim, and calls every kernel of the reference API with the reference's ArrayMD types.
+// A translation unit shaped like the reference's GPP/gpp.cpp and FF/ff.cpp once they call
+// this library through the header shim: it includes the reference's common/Defines.h, then
+// the shim, and calls every kernel of the reference API with the reference's ArrayMD types.
 // oracle/Makefile (target shim-check) compiles and links it against libbgw in both of
 // the reference's complex modes; it is never run (no GPU needed).
 #include "common/Defines.h"
