@@ -13,6 +13,9 @@
 //          sizes): the attainable HBM read bandwidth.
 //   dual   FP32 VALU dual issue: independent vs dependent streams, transcendental and
 //          FP64 instructions mixed in, at 1..8 waves per SIMD.
+//   mfmalds v_mfma_f64_16x16x4f64 fed by LDS operand reads in the FF contraction's
+//          pattern (re+im sums from VALU or from a third plane, with or without a
+//          barrier per stage) at 1 and 2 waves per SIMD.
 //   copy   one 1 GiB device-to-device copy by a plain kernel (16-byte loads and
 //          stores): the reference dispatch for scaling rocprofv3 FETCH_SIZE/WRITE_SIZE.
 //
@@ -20,6 +23,7 @@
 //   build/ceilings [valu|fixed|mfma|dual|hbm|copy ...]      (no argument: all but copy)
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -121,6 +125,58 @@ __global__ __launch_bounds__(256) void k_mfma4_f64(const double* __restrict__ in
 #pragma unroll
   for (int j = 0; j < NACC; ++j) s += acc[j];
   out[blockIdx.x * 256 + threadIdx.x] = s;
+}
+
+// MFMA fed from LDS in the FF contraction's pattern: a wave owns one 16-column block of
+// RB 16-row blocks and per k-step reads its operands from LDS (8-byte reads, the
+// contraction's XOR-swizzled layout: A[k][row ^ swz(k)], B[k][col ^ swz(k)]) and issues
+// the 3M scheme's three v_mfma_f64_16x16x4f64 per row block; 4 k-steps per 16-G stage.
+// SUMS: re and im planes only, the sums re + im formed by VALU adds as the contraction
+// does (else read from a third plane).  BARRIER: a workgroup barrier after every stage.
+// Waves per SIMD are set by the LDS each workgroup reserves (registers capped for two
+// workgroups per CU, as in the contraction).  The LDS holds constants:
+// the operands never change, only the issue pattern is measured.
+template <int RB, bool SUMS, bool BARRIER>
+__global__ __launch_bounds__(256, 2) void k_mfma_lds(const double* __restrict__ in, double* __restrict__ out,
+                                                  int stages) {
+  constexpr int LW = (RB * 16 + 31) / 32 * 32, BK = 16, BN = 64;
+  extern __shared__ double lds[];
+  double* As = lds;                 // [3][BK][LW]
+  double* Bs = lds + 3 * BK * LW;   // [3][BK][BN]
+  for (int i = threadIdx.x; i < 3 * BK * (LW + BN); i += 256) lds[i] = in[i & 15] + 1e-3 * (i & 7);
+  __syncthreads();
+  const int lane = threadIdx.x & 63, wc = (threadIdx.x >> 6) * 16, cl = lane & 15;
+  auto swz = [](int k) { return 16 * (k & 1) + k; };
+  f64x4 de[RB], df[RB], dh[RB];
+#pragma unroll
+  for (int i = 0; i < RB; ++i) de[i] = df[i] = dh[i] = (f64x4){0.0, 0.0, 0.0, 0.0};
+  for (int s = 0; s < stages; ++s) {
+#pragma unroll
+    for (int kk = 0; kk < BK; kk += 4) {
+      const int kr = kk + (lane >> 4), sk = swz(kr);
+      double ar[RB], ai[RB], sa[RB];
+#pragma unroll
+      for (int i = 0; i < RB; ++i) {
+        ar[i] = As[(0 * BK + kr) * LW + ((i * 16 + cl) ^ sk)];
+        ai[i] = As[(1 * BK + kr) * LW + ((i * 16 + cl) ^ sk)];
+        sa[i] = SUMS ? ar[i] + ai[i] : As[(2 * BK + kr) * LW + ((i * 16 + cl) ^ sk)];
+      }
+      const double br = Bs[(0 * BK + kr) * BN + ((wc + cl) ^ sk)];
+      const double bi = Bs[(1 * BK + kr) * BN + ((wc + cl) ^ sk)];
+      const double sb = SUMS ? br + bi : Bs[(2 * BK + kr) * BN + ((wc + cl) ^ sk)];
+#pragma unroll
+      for (int i = 0; i < RB; ++i) {
+        de[i] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[i], br, de[i], 0, 0, 0);
+        df[i] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai[i], bi, df[i], 0, 0, 0);
+        dh[i] = __builtin_amdgcn_mfma_f64_16x16x4f64(sa[i], sb, dh[i], 0, 0, 0);
+      }
+    }
+    if (BARRIER) __syncthreads();
+  }
+  double t = 0.0;
+#pragma unroll
+  for (int i = 0; i < RB; ++i) t += de[i][0] + df[i][1] + dh[i][2] + de[i][3];
+  out[blockIdx.x * 256 + threadIdx.x] = t;
 }
 
 // Read stream: every element of p[0, n) is read once (grid-stride, 4 loads of 16 B
@@ -332,6 +388,30 @@ void mfma4_sweep(int cus, const double* in, double* out) {
   }
 }
 
+template <int RB, bool SUMS, bool BARRIER>
+void mfma_lds_case(int cus, const double* in, double* out) {
+  constexpr int LW = (RB * 16 + 31) / 32 * 32;
+  const size_t need = (size_t)3 * 16 * (LW + 64) * sizeof(double);
+  const int stages = 2000;
+  for (int per_cu = 1; per_cu <= 2; ++per_cu) {
+    // reserve LDS so that exactly per_cu workgroups (per_cu waves per SIMD) fit a CU's 160 KiB
+    const size_t lds = per_cu == 1 ? std::max(need, (size_t)100 << 10) : std::max(need, (size_t)64 << 10);
+    auto k = k_mfma_lds<RB, SUMS, BARRIER>;
+    HIP_OK(hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    int resident = 0;
+    HIP_OK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&resident, k, 256, lds));
+    if (resident != per_cu) {
+      std::printf("mfma from LDS, %d row blocks: %d workgroups/CU resident, wanted %d\n", RB, resident, per_cu);
+      continue;
+    }
+    const int blocks = cus * per_cu;
+    const float ms = time_ms([&] { hipLaunchKernelGGL(k, dim3(blocks), dim3(256), lds, 0, in, out, stages); });
+    const double flop = (double)blocks * 4 * stages * 4 * 3 * RB * 2048.0;  // 4 waves, 4 k-steps, 3 MFMAs per block
+    std::printf("mfma from LDS, %d row blocks, %s, %s barrier/stage, %d wave(s)/SIMD %8.3f ms  %6.2f TFLOP/s\n",
+                RB, SUMS ? "VALU sums " : "sum plane ", BARRIER ? "   " : "no ", per_cu, ms, flop / ms / 1e9);
+  }
+}
+
 }  // namespace
 
 int main(int argc, char** argv) {
@@ -379,6 +459,14 @@ int main(int argc, char** argv) {
     mfma_sweep<4>(cus, in, out);
     mfma_sweep<8>(cus, in, out);
     mfma_sweep<16>(cus, in, out);
+  }
+  if (want(argc, argv, "mfmalds")) {
+    mfma_lds_case<4, true, true>(cus, in, out);
+    mfma_lds_case<4, false, true>(cus, in, out);
+    mfma_lds_case<4, false, false>(cus, in, out);
+    mfma_lds_case<5, true, true>(cus, in, out);
+    mfma_lds_case<5, false, true>(cus, in, out);
+    mfma_lds_case<5, false, false>(cus, in, out);
   }
   if (want(argc, argv, "fixed")) {
     // fixed-register forms, 8 blocks per CU (8 waves per SIMD); per iteration 32 VALU
