@@ -122,7 +122,7 @@ __global__ __launch_bounds__(BT, 3) void k_gpp_bwd(const double2* __restrict__ w
         const double wr = w[k].x, wi = w[k].y, wi2 = wi * wi;
         const double u0 = x0 - wr, u1 = x1 - wr, u2 = x2 - wr;
         const double d0 = fma(u0, u0, wi2), d1 = fma(u1, u1, wi2), d2 = fma(u2, u2, wi2);
-        double r0, r1, r2;
+        double r0, r1, r2, q0r, q0i, q1r, q1i, q2r, q2i;
         if (fast[k]) {
           const double m01 = d0 * d1, m = m01 * d2;
           double r = __builtin_amdgcn_rcp(m);
@@ -131,20 +131,28 @@ __global__ __launch_bounds__(BT, 3) void k_gpp_bwd(const double2* __restrict__ w
           const double q = r * d2;
           r0 = q * d1;
           r1 = q * d0;
+          // inv^2 = r^2 (u^2 - wi^2, 2 u wi) = (r - 2 wi^2 r^2, 2 wi u r^2), as u^2 + wi^2 = d = 1/r;
+          // every d here lies in [2^-300, 2^300], so r^2 stays a normal double
+          const double m2wi2 = -2.0 * wi2, twi = 2.0 * wi;
+          const double t0 = r0 * r0, t1 = r1 * r1, t2 = r2 * r2;
+          q0r = fma(m2wi2, t0, r0), q0i = twi * (u0 * t0);
+          q1r = fma(m2wi2, t1, r1), q1i = twi * (u1 * t1);
+          q2r = fma(m2wi2, t2, r2), q2i = twi * (u2 * t2);
         } else {
           r0 = 1.0 / d0;
           r1 = 1.0 / d1;
           r2 = 1.0 / d2;
+          // inv^2 = (a^2 - b^2, 2ab) with (a, b) = (u, wi) r: no r^2, which can leave the
+          // double range for the denominators this path takes
+          const double a0 = u0 * r0, b0_ = wi * r0, a1 = u1 * r1, b1 = wi * r1, a2 = u2 * r2, b2 = wi * r2;
+          q0r = fma(a0, a0, -b0_ * b0_), q0i = 2.0 * a0 * b0_;
+          q1r = fma(a1, a1, -b1 * b1), q1i = 2.0 * a1 * b1;
+          q2r = fma(a2, a2, -b2 * b2), q2i = 2.0 * a2 * b2;
         }
-        // inv_iw = 1/(wx - wtilde) = (u, wi) * r
-        const double a0 = u0 * r0, b0_ = wi * r0, a1 = u1 * r1, b1 = wi * r1, a2 = u2 * r2, b2 = wi * r2;
-        const double sr = a0 + a1 + a2, si = b0_ + b1 + b2;
+        // inv_iw = 1/(wx - wtilde) = (u, wi) * r; the band's sum over iw
+        const double sr = fma(u0, r0, fma(u1, r1, u2 * r2)), si = wi * (r0 + r1 + r2);
         G[k][0] = fma(c.x, sr, fma(-c.y, si, G[k][0]));
         G[k][1] = fma(c.x, si, fma(c.y, sr, G[k][1]));
-        // inv^2 = (a^2 - b^2, 2ab)
-        const double q0r = fma(a0, a0, -b0_ * b0_), q0i = 2.0 * a0 * b0_;
-        const double q1r = fma(a1, a1, -b1 * b1), q1i = 2.0 * a1 * b1;
-        const double q2r = fma(a2, a2, -b2 * b2), q2i = 2.0 * a2 * b2;
         H[k][0][0] = fma(c.x, q0r, fma(-c.y, q0i, H[k][0][0]));
         H[k][0][1] = fma(c.x, q0i, fma(c.y, q0r, H[k][0][1]));
         H[k][1][0] = fma(c.x, q1r, fma(-c.y, q1i, H[k][1][0]));
