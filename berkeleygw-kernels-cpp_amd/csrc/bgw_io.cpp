@@ -10,7 +10,8 @@
 // Layout (little-endian):
 //   [0,32)     "BGWINPUT", u32 version (=1), u32 kind, u32 count, u32 0, u64 directory offset (=32)
 //   directory  count entries of 96 bytes:
-//                char name[32] (NUL-padded), u32 dtype, u32 ndim, u64 dims[4] (unused = 1),
+//                char name[32] (1-31 printable ASCII characters other than space,
+//                NUL-padded), u32 dtype, u32 ndim, u64 dims[4] (unused = 1),
 //                u64 offset (multiple of 4096), u64 nbytes, u64 checksum
 //   payload    each array at its offset, row-major, complex as interleaved (re,im)
 // checksum = sum_i mix64(w_i ^ (i * 0x9E3779B97F4A7C15)) mod 2^64 over the payload's
@@ -52,6 +53,15 @@ size_t elem_size(int dtype) {
     case BGW_DT_I64: return 8;
     default: return 0;
   }
+}
+
+// Entry names: 1 to 31 printable ASCII characters other than space.
+bool valid_name(const char* s) {
+  const size_t n = std::strlen(s);
+  if (n == 0 || n >= 32) return false;
+  for (size_t i = 0; i < n; ++i)
+    if ((unsigned char)s[i] < 0x21 || (unsigned char)s[i] > 0x7E) return false;
+  return true;
 }
 
 inline uint64_t mix64(uint64_t z) {
@@ -202,7 +212,7 @@ int bgw_inputs_write(const char* path, int kind, const bgw_array_desc* arrays, i
     const bgw_array_desc& d = arrays[i];
     size_t n = 0;
     const size_t es = elem_size(d.dtype);
-    if (!d.name || !*d.name || std::strlen(d.name) >= 32 || !es || !desc_elems(d, &n) || n > SIZE_MAX / es ||
+    if (!d.name || !valid_name(d.name) || !es || !desc_elems(d, &n) || n > SIZE_MAX / es ||
         (!d.data && n)) {
       g_err = "bgw_inputs_write: bad descriptor #" + std::to_string(i);
       return BGW_EINVAL;
@@ -285,7 +295,7 @@ int bgw_inputs_open(const char* path, int verify, bgw_inputs** out) {
     const size_t e = dir + (size_t)i * kEntry;
     char name[33] = {0};
     std::memcpy(name, b + e, 32);
-    if (!name[0] || std::strlen(name) >= 32) return bad("entry #" + std::to_string(i) + " has no valid name");
+    if (!valid_name(name)) return bad("entry #" + std::to_string(i) + " has no valid name");
     for (const auto& prev : in->names)
       if (prev == name) return bad(std::string("duplicate entry '") + name + "'");
     bgw_array_desc d{};

@@ -376,7 +376,7 @@ def write_input_set(path, kind: int, arrays: dict) -> None:
     arr = (_ArrayDesc * len(descs))(*descs)
     rc = lib().bgw_inputs_write(str(path).encode(), kind, arr, len(descs))
     if rc != BGW_OK:
-        raise BgwError(rc, lib().bgw_io_last_error().decode())
+        raise BgwError(rc, lib().bgw_io_last_error().decode(errors="replace"))
 
 
 class _Mapping:
@@ -401,7 +401,7 @@ class InputSet:
         h = _P()
         rc = lib().bgw_inputs_open(str(path).encode(), int(verify), C.byref(h))
         if rc != BGW_OK:
-            raise BgwError(rc, lib().bgw_io_last_error().decode())
+            raise BgwError(rc, lib().bgw_io_last_error().decode(errors="replace"))
         self._m = _Mapping(h)
         self.kind = lib().bgw_inputs_kind(h)
         self.arrays = {}
@@ -506,7 +506,7 @@ class Context:
 
     def _check(self, rc: int) -> None:
         if rc != BGW_OK:
-            raise BgwError(rc, lib().bgw_last_error(self._h).decode())
+            raise BgwError(rc, lib().bgw_last_error(self._h).decode(errors="replace"))
 
     def set_stream(self, stream_ptr: int | None) -> None:
         self._check(lib().bgw_set_stream(self._h, stream_ptr))

@@ -298,7 +298,8 @@ void bgw_synth_permutation(unsigned long long seed, int* out, size_t n);
  * Layout (little-endian):
  *   bytes [0,32)  "BGWINPUT", u32 version = 1, u32 kind, u32 count, u32 0,
  *                 u64 directory offset = 32
- *   directory     count entries of 96 bytes: char name[32] (NUL-padded),
+ *   directory     count entries of 96 bytes: char name[32] (1-31 printable
+ *                 ASCII characters other than space, NUL-padded),
  *                 u32 dtype, u32 ndim (0..4, 0 = one-element scalar),
  *                 u64 dims[4] (unused = 1), u64 offset (multiple of 4096),
  *                 u64 nbytes, u64 checksum

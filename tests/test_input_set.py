@@ -178,6 +178,49 @@ def test_malformed_files_are_refused(tmp_path):
     _expect_eio(tmp_path / "does_not_exist.bgwin")
 
 
+def test_mutated_files_are_refused_or_read_in_bounds(tmp_path):
+    """Seeded random damage to the header and directory (byte overwrites, bit flips,
+    large and small 64-bit values, truncation): every file is either refused with
+    BGW_EIO or opens with each entry inside the file, whose bytes then read back as
+    the file holds them."""
+    good = _gpp_file(tmp_path).read_bytes()
+    rng = np.random.default_rng(20261204)
+    head = 32 + 96 * 13  # header + directory of the GPP set
+    q = tmp_path / "mut.bgwin"
+    opened = 0
+    for _ in range(400):
+        raw = bytearray(good)
+        for _ in range(int(rng.integers(1, 5))):
+            pos = int(rng.integers(0, head))
+            kind = int(rng.integers(0, 4))
+            if kind == 0:
+                raw[pos] = int(rng.integers(0, 256))
+            elif kind == 1:
+                raw[pos] ^= 1 << int(rng.integers(0, 8))
+            elif kind == 2 and pos + 8 <= len(raw):
+                v = (2**64 - 1) >> int(rng.integers(0, 64)) if rng.integers(0, 2) else int(rng.integers(0, 100000))
+                raw[pos:pos + 8] = struct.pack("<Q", v)
+            else:
+                raw[pos] = 0
+        if rng.integers(0, 10) == 0:
+            raw = raw[:int(rng.integers(0, len(raw)))]
+        q.write_bytes(bytes(raw))
+        try:
+            s = bgw.InputSet(q, verify=bool(rng.integers(0, 2)))
+        except bgw.BgwError as e:
+            assert e.code == bgw.BGW_EIO
+            continue
+        opened += 1
+        assert all(k.isascii() and k.isprintable() and " " not in k for k in s.arrays)
+        for v in s.arrays.values():
+            a = np.atleast_1d(v)
+            b = np.frombuffer(a.tobytes(), dtype=np.uint8)
+            assert b.size == a.nbytes and b.size <= len(raw)
+            assert bytes(b) in bytes(raw)
+        s.close()
+    assert opened < 400  # the damage was seen at least somewhere
+
+
 def test_kind_checks_required_entries_and_shapes(tmp_path):
     d = bgw.gpp_inputs(4, 1, 24, 3, 5)
     base = {"number_bands": np.int64(4), "nvband": np.int64(1), "ncouls": np.int64(24), "ngpown": np.int64(3),
@@ -211,6 +254,8 @@ def test_kind_checks_required_entries_and_shapes(tmp_path):
 def test_writer_rejects_bad_descriptors(tmp_path):
     p = tmp_path / "w.bgwin"
     for arrays in ({"x" * 32: np.zeros(2)},             # name too long
+                   {"a b": np.zeros(2)},                # names are printable ASCII without spaces
+                   {"a\u00e9": np.zeros(2)},
                    {"a": np.zeros(2, np.float32)},      # unsupported dtype
                    {"a": np.zeros((1, 1, 1, 1, 2))}):   # rank above 4
         with pytest.raises((bgw.BgwError, TypeError, ValueError)):
