@@ -9,6 +9,7 @@
 // --dump FILE writes the generated inputs as an input set (see bgw.h); --input FILE
 // reads one instead of generating inputs (no size arguments).
 #include <chrono>
+#include <climits>
 #include <cmath>
 #include <complex>
 #include <cstdio>
@@ -19,6 +20,7 @@
 
 #include "bgw.h"
 #include "bgw_reference_api.hpp"
+#include "driver_args.hpp"
 #include "host_array.hpp"
 #include "input_set.hpp"
 
@@ -88,25 +90,28 @@ static int run(int argc, char** argv) {
   std::string input_path, dump_path;
   for (int i = 1; i < argc; ++i) {
     std::string a = argv[i];
-    if (a == "--input" && i + 1 < argc)
+    if (a == "--input" && i + 1 < argc) {
       input_path = argv[++i];
-    else if (a == "--dump" && i + 1 < argc)
+    } else if (a == "--dump" && i + 1 < argc) {
       dump_path = argv[++i];
-    else if (a == "--random" && i + 1 < argc)
-      seed = std::atoll(argv[++i]);
-    else if (a == "--precision" && i + 1 < argc) {
+    } else if (a == "--random" && i + 1 < argc) {
+      if (!bgw::parse_int(argv[++i], 0, LLONG_MAX, &seed)) {
+        usage();
+        return 1;
+      }
+    } else if (a == "--precision" && i + 1 < argc) {
       const std::string p = argv[++i];
       if (p != "fp32" && p != "fp64") {
         usage();
         return 1;
       }
       precision = p == "fp32" ? BGW_FP32_BULK : BGW_FP64;
-    }
-    else if (a == "-h" || a == "--help") {
+    } else if (a == "-h" || a == "--help") {
       usage();
       return 0;
-    } else
+    } else {
       pos.push_back(a);
+    }
   }
   if (!input_path.empty()) {
     if (!pos.empty() || seed >= 0 || !dump_path.empty()) {
@@ -117,12 +122,19 @@ static int run(int argc, char** argv) {
   }
   int number_bands = 15023, nvband = 1998, ncouls = 22401, ngpown = 66, nFreq = 15, nfreqeval = 10;
   if (pos.size() == 6) {
-    number_bands = std::atoi(pos[0].c_str());
-    nvband = std::atoi(pos[1].c_str());
-    ncouls = std::atoi(pos[2].c_str());
-    ngpown = std::atoi(pos[3].c_str());
-    nFreq = std::atoi(pos[4].c_str());
-    nfreqeval = std::atoi(pos[5].c_str());
+    long long v[6];
+    for (int k = 0; k < 6; ++k)
+      if (!bgw::parse_int(pos[k], INT_MIN, INT_MAX, &v[k])) {
+        std::printf("Incorrect Parameters!!! The correct form is\n");
+        usage();
+        return 1;
+      }
+    number_bands = (int)v[0];
+    nvband = (int)v[1];
+    ncouls = (int)v[2];
+    ngpown = (int)v[3];
+    nFreq = (int)v[4];
+    nfreqeval = (int)v[5];
   } else if (!pos.empty()) {
     std::printf("Incorrect Parameters!!! The correct form is\n");
     usage();
@@ -142,6 +154,14 @@ static int run(int argc, char** argv) {
   const double freqevalmin = 0.0, freqevalstep = 0.5, occ = 1.0;
   std::printf(" number_bands = %d\n nvband = %d\n ncouls = %d\n ngpown = %d\n nFreq = %d\n nfreqeval = %d\n",
               number_bands, nvband, ncouls, ngpown, nFreq, nfreqeval);
+  // aqsmtemp, aqsntemp, I_epsR_array and I_epsA_array dominate the host arrays
+  const double mem = (2.0 * number_bands * ncouls + 2.0 * nFreq * (double)ngpown * ncouls) * 16;
+  const double host_mem = bgw::host_memory_bytes();
+  if (host_mem > 0 && mem > host_mem) {
+    std::fprintf(stderr, "the arrays need %g GB, more than the host's %g GB of memory\n", mem / std::pow(1024, 3),
+                 host_mem / std::pow(1024, 3));
+    return 1;
+  }
   auto t0 = clk::now();
   HostArray<int, 1> inv_igp_index(ngpown), indinv(ncouls);
   HostArray<double, 1> vcoul(ncouls), ekq(number_bands), dFreqGrid(nFreq);

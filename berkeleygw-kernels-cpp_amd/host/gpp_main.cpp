@@ -10,6 +10,7 @@
 // --dump FILE (writes the generated inputs as an input set, see bgw.h) and
 // --input FILE (reads an input set instead of generating inputs; no size arguments).
 #include <chrono>
+#include <climits>
 #include <cmath>
 #include <complex>
 #include <cstdio>
@@ -21,6 +22,7 @@
 
 #include "bgw.h"
 #include "bgw_reference_api.hpp"
+#include "driver_args.hpp"
 #include "host_array.hpp"
 #include "input_set.hpp"
 
@@ -36,14 +38,14 @@ static void usage() {
 static void print_extras(int number_bands, int nvband, int ngpown, int ncouls, const int* inv, const int* indinv,
                          const double* wx_all, const double* wtilde, const double* aqsm, const double* aqsn,
                          const double* I_eps, const double* vcoul, double occ) {
-  std::vector<cplx> loc(number_bands * 3);
+  std::vector<cplx> loc((size_t)number_bands * 3);
   cplx ach[3], achs, asx[3];
   bgw::check(bgw_gpp_solver(bgw::default_context(), number_bands, nvband, ngpown, ncouls, inv, indinv, wx_all,
                             wtilde, aqsm, aqsn, I_eps, vcoul, occ, reinterpret_cast<double*>(ach),
                             reinterpret_cast<double*>(loc.data()), reinterpret_cast<double*>(&achs),
                             reinterpret_cast<double*>(asx)));
   cplx sum = 0;
-  for (int n1 = 0; n1 < number_bands; ++n1) sum += loc[n1 * 3];
+  for (int n1 = 0; n1 < number_bands; ++n1) sum += loc[(size_t)n1 * 3];
   std::printf(" sum_n1 acht_n1_loc[n1][0] = (%f,%f)\n achstemp = (%f,%f)\n asxtemp[0] = (%f,%f)\n", sum.real(),
               sum.imag(), achs.real(), achs.imag(), asx[0].real(), asx[0].imag());
 }
@@ -62,7 +64,7 @@ static int run_input_set(const char* path, int precision, bool all) {
   std::vector<double> wx_all((size_t)number_bands * 3);
   const double* w = static_cast<const double*>(wx.data);
   for (int n1 = 0; n1 < number_bands; ++n1)
-    for (int iw = 0; iw < 3; ++iw) wx_all[n1 * 3 + iw] = wx.ndim == 1 ? w[iw] : w[n1 * 3 + iw];
+    for (int iw = 0; iw < 3; ++iw) wx_all[(size_t)n1 * 3 + iw] = wx.ndim == 1 ? w[iw] : w[(size_t)n1 * 3 + iw];
   bgw::check(bgw_set_precision(bgw::default_context(), precision));
   cplx achtemp[3];
   auto k_start = clk::now();
@@ -96,7 +98,10 @@ static int run(int argc, char** argv) {
     } else if (a == "--dump" && i + 1 < argc) {
       dump_path = argv[++i];
     } else if (a == "--random" && i + 1 < argc) {
-      seed = std::atoll(argv[++i]);
+      if (!bgw::parse_int(argv[++i], 0, LLONG_MAX, &seed)) {
+        usage();
+        return 1;
+      }
     } else if (a == "--precision" && i + 1 < argc) {
       const std::string p = argv[++i];
       if (p != "fp32" && p != "fp64") {
@@ -123,10 +128,16 @@ static int run(int argc, char** argv) {
   int number_bands = 512, nvband = 2, ncouls = 32768, nodes_per_group = 20;
   const int npes = 1;
   if (pos.size() == 4) {
-    number_bands = std::atoi(pos[0].c_str());
-    nvband = std::atoi(pos[1].c_str());
-    ncouls = std::atoi(pos[2].c_str());
-    nodes_per_group = std::atoi(pos[3].c_str());
+    long long v[4];
+    for (int k = 0; k < 4; ++k)
+      if (!bgw::parse_int(pos[k], INT_MIN, INT_MAX, &v[k])) {
+        usage();
+        return 1;
+      }
+    number_bands = (int)v[0];
+    nvband = (int)v[1];
+    ncouls = (int)v[2];
+    nodes_per_group = (int)v[3];
   } else if (!pos.empty()) {
     usage();
     return 1;
@@ -145,14 +156,20 @@ static int run(int argc, char** argv) {
 
   std::printf("number_bands = %d\t nvband = %d\t ncouls = %d\t nodes_per_group  = %d\t ngpown = %d\t nend = 3\t "
               "nstart = 0\n", number_bands, nvband, ncouls, nodes_per_group, ngpown);
-  HostArray<cplx, 2> aqsmtemp(number_bands, ncouls), aqsntemp(number_bands, ncouls);
-  HostArray<cplx, 2> I_eps_array(ngpown, ncouls), wtilde_array(ngpown, ncouls);
-  HostArray<double, 1> vcoul(ncouls), wx3(3), wx_all(number_bands * 3);
-  HostArray<int, 1> inv_igp_index(ngpown), indinv(ncouls + 1);
-  HostArray<cplx, 1> achtemp(3);
   const double mem = (2.0 * number_bands * ncouls + 2.0 * ngpown * ncouls) * 16 + ncouls * 8.0 + ngpown * 4.0 +
                      (ncouls + 1) * 4.0;
   std::printf("Memory Foot Print = %g GBs\n", mem / std::pow(1024, 3));
+  const double host_mem = bgw::host_memory_bytes();
+  if (host_mem > 0 && mem > host_mem) {
+    std::fprintf(stderr, "the arrays need %g GB, more than the host's %g GB of memory\n", mem / std::pow(1024, 3),
+                 host_mem / std::pow(1024, 3));
+    return 1;
+  }
+  HostArray<cplx, 2> aqsmtemp(number_bands, ncouls), aqsntemp(number_bands, ncouls);
+  HostArray<cplx, 2> I_eps_array(ngpown, ncouls), wtilde_array(ngpown, ncouls);
+  HostArray<double, 1> vcoul(ncouls), wx3(3), wx_all((size_t)number_bands * 3);
+  HostArray<int, 1> inv_igp_index(ngpown), indinv((size_t)ncouls + 1);
+  HostArray<cplx, 1> achtemp(3);
 
   if (seed < 0) {  // reference constants
     const cplx expr(0.5, 0.5);
@@ -167,7 +184,7 @@ static int run(int argc, char** argv) {
       if (wx3(iw) < to1) wx3(iw) = to1;
     }
     for (int n1 = 0; n1 < number_bands; ++n1)
-      for (int iw = 0; iw < 3; ++iw) wx_all(n1 * 3 + iw) = wx3(iw);
+      for (int iw = 0; iw < 3; ++iw) wx_all((size_t)n1 * 3 + iw) = wx3(iw);
   } else {  // benchmark distributions (same streams as the Python tooling)
     const unsigned long long s = (unsigned long long)seed;
     bgw_synth_complex(s + 1, -1, 1, -1, 1, reinterpret_cast<double*>(aqsmtemp.dptr), aqsmtemp.size);
@@ -183,7 +200,7 @@ static int run(int argc, char** argv) {
     for (int n1 = 0; n1 < number_bands; ++n1)
       for (int iw = 0; iw < 3; ++iw) {
         double w = e_lk - ek[n1] + dw * (iw - 1);
-        wx_all(n1 * 3 + iw) = w < to1 ? to1 : w;
+        wx_all((size_t)n1 * 3 + iw) = w < to1 ? to1 : w;
       }
     for (int iw = 0; iw < 3; ++iw) wx3(iw) = wx_all(iw);
   }
