@@ -48,7 +48,7 @@ struct TileShape {
   static constexpr int WR = TALL ? 1 : 2, WC = TALL ? 4 : 2;
   static constexpr int RB = TALL ? 5 : 2, CB = TALL ? 1 : 2;
   static constexpr int BM = WR * RB * 16;
-  static constexpr int LW = (BM + 31) / 32 * 32;  // LDS row width: the swizzle permutes within 32
+  static constexpr int LW = (BM + 31) / 32 * 32;  // FP32 LDS row width: its swizzle permutes within 32
 };
 constexpr int BN = 64;   // columns (G') per workgroup tile
 constexpr int BK = 16;   // G-vectors per LDS stage (double-buffered)
@@ -66,12 +66,17 @@ __global__ __launch_bounds__(NT, 2) void k_contract(ContractArgs a) {
   constexpr int BM = TS::BM, RB = TS::RB, CB = TS::CB;
   constexpr int AE = BM * BK / NT;  // A elements staged per thread and stage
   constexpr int BE = BN * BK / NT;  // B elements staged per thread and stage
-  // [buffer][re/im][k][row ^ swz(k)]: two stages in flight, one barrier per stage.
-  // The XOR swizzle swz(k) = 16*(k&1) + k keeps both the staging writes (16 k x 2
-  // rows per half-wave) and the MFMA operand reads (16 rows x 2 k per half-wave)
-  // on distinct LDS banks.
-  __shared__ __align__(16) T As[2][2][BK][TS::LW];
-  __shared__ __align__(16) T Bs[2][2][BK][BN];
+  // [buffer][k][row ^ swz(k)] of (re, im) pairs: two stages in flight, one barrier per
+  // stage; a stage element is one 16-byte (FP64) or 8-byte (FP32) write and every
+  // operand pair one read.  The XOR swizzle keeps both the staging writes (16 k of a
+  // row per 16 lanes) and the MFMA operand reads (16 rows of a k per 16 lanes) on
+  // distinct LDS banks: FP64 pairs fill the 64 banks 16 at a time, so swz(k) = k
+  // permutes within rows of 16; FP32 pairs take two 16-lane groups per pass, so
+  // swz(k) = 16*(k&1) + k permutes within 32 and alternates the halves.
+  typedef typename std::conditional<F32, float2, double2>::type T2;
+  constexpr int LWP = F32 ? TS::LW : (BM + 15) / 16 * 16;  // LDS row width
+  __shared__ __align__(16) T2 As[2][BK][LWP];
+  __shared__ __align__(16) T2 Bs[2][BK][BN];
 
   // Work items are (G' tile ct, live row tile t, K split ks), ct fastest.  The live
   // row tiles of all jobs are numbered densely by tile_start (prefix of
@@ -211,14 +216,16 @@ __global__ __launch_bounds__(NT, 2) void k_contract(ContractArgs a) {
       }
     }
   };
-  auto swz = [](int k) { return 16 * (k & 1) + k; };
+  auto swz = [](int k) { return F32 ? 16 * (k & 1) + k : k; };
   auto stage_store = [&](int buf, int kb) {
 #pragma unroll
     for (int i = 0; i < AE; ++i) {
       const int idx = tid + NT * i, k = idx % BK, col = (idx / BK) ^ swz(k);
       const bool ok = aok[i] && kb + k < k1;
-      As[buf][0][k][col] = ok ? (T)av[i].x : (T)0;
-      As[buf][1][k][col] = ok ? (T)av[i].y : (T)0;
+      T2 w;
+      w.x = ok ? (T)av[i].x : (T)0;
+      w.y = ok ? (T)av[i].y : (T)0;
+      As[buf][k][col] = w;
     }
 #pragma unroll
     for (int i = 0; i < BE; ++i) {
@@ -231,8 +238,10 @@ __global__ __launch_bounds__(NT, 2) void k_contract(ContractArgs a) {
         if (!(v.x * v.x + v.y * v.y > a.thr2)) v = make_double2(0.0, 0.0);
       }
       const bool ok = bok[i] && kb + k < k1;
-      Bs[buf][0][k][col] = ok ? (T)v.x : (T)0;
-      Bs[buf][1][k][col] = ok ? (T)v.y : (T)0;
+      T2 w;
+      w.x = ok ? (T)v.x : (T)0;
+      w.y = ok ? (T)v.y : (T)0;
+      Bs[buf][k][col] = w;
     }
   };
   if (k0 < k1) {
@@ -254,14 +263,16 @@ __global__ __launch_bounds__(NT, 2) void k_contract(ContractArgs a) {
         double ar[RB], ai[RB], sa[RB], br[CB], bi[CB], sb[CB];
 #pragma unroll
         for (int i = 0; i < RB; ++i) {
-          ar[i] = As[buf][0][kr][(wr + i * 16 + cl) ^ sk];
-          ai[i] = As[buf][1][kr][(wr + i * 16 + cl) ^ sk];
+          const T2 av2 = As[buf][kr][(wr + i * 16 + cl) ^ sk];
+          ar[i] = av2.x;
+          ai[i] = av2.y;
           sa[i] = ar[i] + ai[i];
         }
 #pragma unroll
         for (int j = 0; j < CB; ++j) {
-          br[j] = Bs[buf][0][kr][(wc + j * 16 + cl) ^ sk];
-          bi[j] = Bs[buf][1][kr][(wc + j * 16 + cl) ^ sk];
+          const T2 bv2 = Bs[buf][kr][(wc + j * 16 + cl) ^ sk];
+          br[j] = bv2.x;
+          bi[j] = bv2.y;
           sb[j] = br[j] + bi[j];
         }
 #pragma unroll
@@ -293,14 +304,16 @@ __global__ __launch_bounds__(NT, 2) void k_contract(ContractArgs a) {
         float ar[RB], ai[RB], sa[RB], br[CB], bi[CB], sb[CB];
 #pragma unroll
         for (int i = 0; i < RB; ++i) {
-          ar[i] = As[buf][0][kr][(wr + i * 16 + cl) ^ sk];
-          ai[i] = As[buf][1][kr][(wr + i * 16 + cl) ^ sk];
+          const T2 av2 = As[buf][kr][(wr + i * 16 + cl) ^ sk];
+          ar[i] = av2.x;
+          ai[i] = av2.y;
           sa[i] = ar[i] + ai[i];
         }
 #pragma unroll
         for (int j = 0; j < CB; ++j) {
-          br[j] = Bs[buf][0][kr][(wc + j * 16 + cl) ^ sk];
-          bi[j] = Bs[buf][1][kr][(wc + j * 16 + cl) ^ sk];
+          const T2 bv2 = Bs[buf][kr][(wc + j * 16 + cl) ^ sk];
+          br[j] = bv2.x;
+          bi[j] = bv2.y;
           sb[j] = br[j] + bi[j];
         }
 #pragma unroll
@@ -362,8 +375,8 @@ __global__ __launch_bounds__(NT, 2) void k_contract(ContractArgs a) {
     }
   }
   // the stage buffers are free after the loop's last barrier: the reduction
-  // scratch lives in them (two FP64 workgroups fill the 160 KB of LDS of a CU)
-  cdd tot = cdd_block_reduce(acc, reinterpret_cast<cdd*>(&As[0][0][0][0]));
+  // scratch lives in them (two FP64 80-row workgroups take 144 of the CU's 160 KB of LDS)
+  cdd tot = cdd_block_reduce(acc, reinterpret_cast<cdd*>(&As[0][0][0]));
   if (tid == 0) cdd_store(&a.partial[pid], tot);
 }
 
