@@ -361,7 +361,7 @@ void* bgw_get_stream(bgw_ctx* ctx) { return ctx ? static_cast<void*>(ctx->stream
 
 int bgw_set_precision(bgw_ctx* ctx, int precision) {
   if (!ctx) return BGW_EINVAL;
-  if (precision != BGW_FP64 && precision != BGW_FP32_BULK)
+  if (precision != BGW_FP64 && precision != BGW_FP32_BULK && precision != BGW_FP64_MOMENTS)
     return fail(ctx, BGW_EINVAL, "unknown precision");
   ctx->precision = precision;
   return BGW_OK;

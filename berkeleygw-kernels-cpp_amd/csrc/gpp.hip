@@ -15,6 +15,7 @@
 // per-workgroup partials [W][nb][3]; a fixed-order double-double pass produces
 // acht_n1_loc and achtemp (run-to-run bitwise reproducible).
 #include <algorithm>
+#include <cfloat>
 #include <cmath>
 #include <mutex>
 #include <vector>
@@ -275,10 +276,342 @@ __device__ inline void tile_f32(const TileEntry32* __restrict__ tile, int n, con
   }
 }
 
+// ------------------------------------------------------------ moment (far-field) form
+// The G-sum of one work item (p, G-chunk) is a sum of simple poles: conj(x - w)/|x - w|^2
+// = 1/(x - w) (GPP/gpp.cpp:376-380), so with w = wtilde(p,g) and I = I_eps(p,g)
+//   T(x) = sum_g I_g / (x - w_g).
+// With every pole within R of a centre c and every band frequency x at least D from c,
+// 1/(x - w) = sum_{m>=0} z^m / (x - c)^(m+1) with z = w - c, and the tail of a pole beyond
+// m = M - 1 is exactly (z/(x - c))^M / (x - w).  Hence
+//   T(x) = t sum_{m<M} nu_m (s t)^m + E,   t = 1/(x - c),   nu_m = sum_g I_g (z_g/s)^m,
+//   |E| <= (R/D)^M sum_g |I_g / (x - w_g)|,
+// where s is a power of two near the poles' extent (an exact scaling that keeps the powers
+// in range).  A chunk is "far" when R <= kFarRho D, which holds E below 2^-54 times the sum
+// of the magnitudes of its terms -- under the rounding error bound of the direct sum.  The
+// far chunks of a G' row are re-expanded about one centre and added, so the row is
+// evaluated once per band frequency; every other chunk (poles near a band frequency, a
+// non-finite pole, weight or moment) takes the direct sum.  The moments cost O(M) per
+// (p, g) instead of O(3 nb).
+constexpr int kMom = 28;
+constexpr double kFarRho = 0.26;  // 0.26^28 = 4.1e-17 < 2^-54
+constexpr int kMomSample = 256;   // the first poles of a chunk, whose bounding box gives its centre and scale
+
+constexpr int kChunkDirect = 0;  // the direct sum
+constexpr int kChunkFar = 1;     // re-expanded into its row's moments, evaluated with the row
+constexpr int kChunkEval = 2;    // far, and evaluates its row's moments
+constexpr int kChunkSelf = 3;    // far, but evaluated from its own moments (outside the row's disc)
+
+struct __align__(16) MomChunk {
+  double cx, cy;  // centre c
+  double s;       // scale, a power of two
+  double R;       // bound of |w - c| over the chunk's poles; +inf: not usable
+  int mode, pad_[3];
+  double2 nu[kMom];
+};
+struct __align__(16) MomRow {
+  double cx, cy, s, pad_;
+  double2 nu[kMom];  // sum over the row's kChunkFar/kChunkEval chunks, about (cx, cy) with scale s
+};
+
+// Sums of the N doubles v[] over the 64 lanes of a wave, reduce-scatter style: while N
+// is even each exchange halves the values a lane carries (a lane keeps the lower or the
+// upper half by its lane bit and adds its partner's copy of that half), then the rest is
+// summed across the remaining lane bits.  On return v[0..scatter_len(N)) of a lane holds
+// the wave totals of components [base, base + scatter_len(N)); `writer` is set in one lane
+// of each component block.  The order of every addition is fixed, so the sums are bitwise
+// reproducible.
+constexpr int scatter_len(int n, int mask = 32) {
+  return mask == 0 ? n : (n % 2 == 0 ? scatter_len(n / 2, mask / 2) : n);
+}
+
+template <int N, int MASK>
+__device__ __forceinline__ void wave_sum_scatter(double* v, int lane, int& base, bool& writer) {
+  if constexpr (MASK > 0) {
+    if constexpr (N % 2 == 0) {
+      const bool up = (lane & MASK) != 0;
+#pragma unroll
+      for (int j = 0; j < N / 2; ++j) {
+        const double send = up ? v[j] : v[j + N / 2];
+        const double keep = up ? v[j + N / 2] : v[j];
+        v[j] = keep + __shfl_xor(send, MASK, 64);
+      }
+      if (up) base += N / 2;
+      wave_sum_scatter<N / 2, MASK / 2>(v, lane, base, writer);
+    } else {
+      // a + b in one lane and b + a in its partner: the same value (addition commutes)
+#pragma unroll
+      for (int j = 0; j < N; ++j) v[j] += __shfl_xor(v[j], MASK, 64);
+      if (lane & MASK) writer = false;
+      wave_sum_scatter<N, MASK / 2>(v, lane, base, writer);
+    }
+  }
+}
+
+// One wave per work item (p, G-chunk [gb, ge)), four items per workgroup: the chunk's
+// centre and scale from the bounding box of its first kMomSample poles (any centre is
+// valid: R is measured over every pole), its moments and R.  A non-finite pole or weight
+// makes a moment non-finite (NaN and Inf survive every sum and product), which marks the
+// chunk as not usable.
+constexpr int kMomWaves = 4;
+
+__global__ __launch_bounds__(64 * kMomWaves) void k_gpp_moments(const double2* __restrict__ wt,
+                                                                const double2* __restrict__ Ieps, int ncouls,
+                                                                int chunk, int nchunks, long item0, long item1,
+                                                                MomChunk* __restrict__ tab) {
+  const int lane = threadIdx.x & 63;
+  const long item = item0 + (long)blockIdx.x * kMomWaves + (threadIdx.x >> 6);
+  if (item >= item1) return;  // the whole wave: the kernel has no workgroup barrier
+  const int p = (int)(item / nchunks);
+  const int gb = (int)(item - (long)p * nchunks) * chunk;
+  const int ge = min(ncouls, gb + chunk);
+  const double2* __restrict__ wrow = wt + (size_t)p * ncouls;
+  const double2* __restrict__ irow = Ieps + (size_t)p * ncouls;
+
+  double lr = INFINITY, hr = -INFINITY, li = INFINITY, hi = -INFINITY;
+#pragma unroll
+  for (int k = 0; k < kMomSample / 64; ++k) {
+    const int g = gb + lane + 64 * k;
+    if (g < ge) {
+      const double2 w = wrow[g];
+      lr = fmin(lr, w.x);
+      hr = fmax(hr, w.x);
+      li = fmin(li, w.y);
+      hi = fmax(hi, w.y);
+    }
+  }
+#pragma unroll
+  for (int m = 32; m > 0; m >>= 1) {
+    lr = fmin(lr, __shfl_xor(lr, m, 64));
+    hr = fmax(hr, __shfl_xor(hr, m, 64));
+    li = fmin(li, __shfl_xor(li, m, 64));
+    hi = fmax(hi, __shfl_xor(hi, m, 64));
+  }
+  double cx = 0.5 * (lr + hr), cy = 0.5 * (li + hi);
+  if (!(fabs(cx) <= 0x1p+1000 && fabs(cy) <= 0x1p+1000)) cx = cy = 0.0;  // a sample of NaN/Inf poles
+  const double half = 0.5 * fmax(hr - lr, hi - li);
+  int e = 0;
+  if (half >= 0x1p-1000 && half <= 0x1p+1000) (void)frexp(half, &e);  // half < 2^e
+  const double s = ldexp(1.0, e), is = ldexp(1.0, -e);
+
+  double v[2 * kMom];
+#pragma unroll
+  for (int j = 0; j < 2 * kMom; ++j) v[j] = 0.0;
+  double r2 = 0.0;
+  // one pole ahead: the loads of the next pole are in flight while this one is summed
+  // (the last index is re-read, in bounds, past the end)
+  int g = gb + lane;
+  double2 wn = wrow[min(g, ge - 1)], In = irow[min(g, ge - 1)];
+  for (; g < ge; g += 64) {
+    const double2 w = wn, I = In;
+    const int gn = min(g + 64, ge - 1);
+    wn = wrow[gn];
+    In = irow[gn];
+    // keeps the loads issued here, ahead of this pole's arithmetic (no wait: the
+    // barrier has no register operands), instead of sunk to their use
+    asm volatile("" ::: "memory");
+    const double zr = w.x - cx, zi = w.y - cy;
+    r2 = fmax(r2, fma(zr, zr, zi * zi));
+    const double ur = zr * is, ui = zi * is;  // exact: is is a power of two
+    double pr = I.x, pi = I.y;
+#pragma unroll
+    for (int m = 0; m < kMom; ++m) {
+      v[2 * m] += pr;
+      v[2 * m + 1] += pi;
+      if (m + 1 < kMom) {
+        const double t = fma(pr, ur, -pi * ui);
+        pi = fma(pr, ui, pi * ur);
+        pr = t;
+      }
+    }
+  }
+  int base = 0;
+  bool writer = true;
+  wave_sum_scatter<2 * kMom, 32>(v, lane, base, writer);
+  constexpr int NF = scatter_len(2 * kMom);
+  bool fin = true;
+#pragma unroll
+  for (int j = 0; j < NF; ++j) fin &= fabs(v[j]) <= DBL_MAX;
+#pragma unroll
+  for (int m = 32; m > 0; m >>= 1) r2 = fmax(r2, __shfl_xor(r2, m, 64));
+  const bool ok = __all(fin);
+  MomChunk* __restrict__ me = tab + item;
+  double* __restrict__ out = reinterpret_cast<double*>(me->nu);
+  if (writer) {
+#pragma unroll
+    for (int j = 0; j < NF; ++j) out[base + j] = v[j];
+  }
+  if (lane == 0) {
+    me->cx = cx;
+    me->cy = cy;
+    me->s = s;
+    // sqrt(r2) is within a few ulp of max |w - c|: the factor makes R a bound
+    me->R = ok ? sqrt(r2) * (1.0 + 0x1p-40) : INFINITY;
+  }
+}
+
+// A chunk's disc, centre (cx, cy) radius R, is far from the band frequencies' range
+// [wxmin, wxmax] when R <= kFarRho D, D the distance from the centre to the range, and
+// every |x - c|^2 stays inside the batched inversion's [2^-150, 2^150] (false for NaN).
+__device__ inline bool disc_far(double cx, double cy, double R, double wxmin, double wxmax) {
+  const double dx = cx < wxmin ? wxmin - cx : (cx > wxmax ? cx - wxmax : 0.0);
+  const double D = sqrt(fma(dx, dx, cy * cy)) * (1.0 - 0x1p-40);
+  const double e0 = wxmin - cx, e1 = wxmax - cx;
+  const double dmax2 = fmax(e0 * e0, e1 * e1) + cy * cy;
+  return R <= kFarRho * D && D * D >= kDMin && dmax2 <= kDMax;
+}
+
+// Row pass, one workgroup per G' row p0 + blockIdx.x.  The row's centre and scale are
+// those of its first far chunk, taken from chunk floor(p nchunks / W) mod nchunks on, which
+// becomes kChunkEval: with W the main kernel's grid width (item i runs on workgroup i mod W),
+// this spreads the rows' evaluations over all of its workgroups.  Each other far chunk whose
+// disc, widened by the distance between the centres, is still far about the row's centre
+// is re-expanded there and added (kChunkFar); the rest evaluate their own moments
+// (kChunkSelf) or take the direct sum.  Re-expansion from centre c, scale s to c', s':
+//   nu'_m = sum_{j<=m} C(m,j) (s/s')^j ((c - c')/s')^(m-j) nu_j,
+// the binomial expansion of ((w - c) + (c - c'))^m, exact in exact arithmetic, formed in
+// lane m by a Taylor shift in (c - c')/s' (no binomial coefficients, no division).  Wave
+// w takes the chunks c = w mod kRowWaves in chunk order, and the waves' sums are added in
+// wave order.
+constexpr int kRowWaves = 4;
+
+__global__ __launch_bounds__(64 * kRowWaves) void k_gpp_moments_row(int nchunks, int p0, int W,
+                                                                   const double* __restrict__ wxrange,
+                                                                   MomChunk* __restrict__ tab,
+                                                                   MomRow* __restrict__ rows) {
+  __shared__ double2 part[kRowWaves][kMom];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int p = p0 + blockIdx.x;
+  const double wxmin = wxrange[0], wxmax = wxrange[1];
+  MomChunk* __restrict__ ch = tab + (size_t)p * nchunks;
+  // the first far chunk from chunk `first` on, 64 chunks tested at a time (every wave alike)
+  const int first = (int)(((long)p * nchunks / W) % nchunks);
+  int d = -1;
+  for (int k0 = 0; k0 < nchunks && d < 0; k0 += 64) {
+    const int k = k0 + lane;
+    const int c = first + k < nchunks ? first + k : first + k - nchunks;
+    const bool far = k < nchunks && disc_far(ch[c].cx, ch[c].cy, ch[c].R, wxmin, wxmax);
+    const unsigned long long b = __ballot(far);
+    if (b) {
+      const int kk = k0 + __builtin_ctzll(b);
+      d = first + kk < nchunks ? first + kk : first + kk - nchunks;
+    }
+  }
+  if (d < 0) {  // no far chunk: every chunk takes the direct sum
+    for (int c = threadIdx.x; c < nchunks; c += 64 * kRowWaves) ch[c].mode = kChunkDirect;
+    return;
+  }
+  const double pcx = ch[d].cx, pcy = ch[d].cy, ps = ch[d].s, ips = 1.0 / ps;  // exact
+  const int m = lane < kMom ? lane : kMom - 1;
+  double sr = 0.0, si = 0.0;
+  for (int c = wave; c < nchunks; c += kRowWaves) {
+    const double cx = ch[c].cx, cy = ch[c].cy, R = ch[c].R;
+    int mode = kChunkDirect;
+    if (c == d) {
+      mode = kChunkEval;
+      sr += ch[c].nu[m].x;
+      si += ch[c].nu[m].y;
+    } else if (disc_far(cx, cy, R, wxmin, wxmax)) {
+      const double br = (cx - pcx) * ips, bi = (cy - pcy) * ips;  // (c - c')/s'
+      const double Rw = (R + sqrt(fma(cx - pcx, cx - pcx, (cy - pcy) * (cy - pcy)))) * (1.0 + 0x1p-40);
+      mode = kChunkSelf;
+      if (disc_far(pcx, pcy, Rw, wxmin, wxmax)) {
+        // lane j: mu_j = (s/s')^j nu_j (exact: s/s' = 2^ea), then the Taylor shift,
+        // kMom - 1 rounds of mu_m += b mu_(m-1) over m >= k (old values), which leaves
+        // mu_m = sum_{j<=m} C(m,j) b^(m-j) mu_j in lane m
+        int ea = 0;
+        (void)frexp(ch[c].s * ips, &ea);  // s/s' = 2^(ea - 1)
+        const double2 nj = ch[c].nu[m];
+        double ur = ldexp(nj.x, m * (ea - 1)), ui = ldexp(nj.y, m * (ea - 1));
+        for (int k = 1; k < kMom; ++k) {
+          const double qr = __shfl_up(ur, 1, 64), qi = __shfl_up(ui, 1, 64);
+          if (lane >= k && lane < kMom) {
+            const double nr = fma(br, qr, fma(-bi, qi, ur));
+            ui = fma(br, qi, fma(bi, qr, ui));
+            ur = nr;
+          }
+        }
+        if (__all(fabs(ur) <= DBL_MAX && fabs(ui) <= DBL_MAX)) {
+          sr += ur;
+          si += ui;
+          mode = kChunkFar;
+        }
+      }
+    }
+    if (lane == 0) ch[c].mode = mode;
+  }
+  if (lane < kMom) part[wave][lane] = make_double2(sr, si);
+  __syncthreads();
+  if (threadIdx.x < kMom) {
+    double2 t = part[0][threadIdx.x];
+#pragma unroll
+    for (int w = 1; w < kRowWaves; ++w) {
+      t.x += part[w][threadIdx.x].x;
+      t.y += part[w][threadIdx.x].y;
+    }
+    rows[p].nu[threadIdx.x] = t;
+  }
+  if (threadIdx.x == 0) {
+    rows[p].cx = pcx;
+    rows[p].cy = pcy;
+    rows[p].s = ps;
+  }
+}
+
+// Far-field value of a set of moments (a row's summed far chunks, or one chunk) for a
+// lane's 3*BPL frequencies: acc[b][iw] = T(x[b][iw]).  nu: the moments (LDS).  The 3*BPL reciprocals
+// come from one batched inversion, as in the direct form.
+template <int BPL>
+__device__ inline void row_far(const double2* __restrict__ nu, double cx, double cy, double s,
+                               const double (&x)[BPL][3], double (&acc)[BPL][3][2]) {
+  constexpr int K = BPL * 3;
+  double ar[K], ai[K];  // tau = s t
+  {
+    double a[K], d[K], r[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      a[k] = x[k / 3][k % 3] - cx;
+      d[k] = fma(a[k], a[k], cy * cy);
+    }
+    batch_inverse(d, r);
+    // 1/(x - c) = 1/((x - cx) - i cy) = ((x - cx) + i cy) / |x - c|^2
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      ar[k] = s * (a[k] * r[k]);
+      ai[k] = s * (cy * r[k]);
+    }
+  }
+  double hr[K], hi[K];
+  const double2 top = nu[kMom - 1];
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    hr[k] = top.x;
+    hi[k] = top.y;
+  }
+  for (int m = kMom - 2; m >= 0; --m) {
+    const double2 c = nu[m];
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      const double nr = fma(hr[k], ar[k], fma(-hi[k], ai[k], c.x));
+      hi[k] = fma(hr[k], ai[k], fma(hi[k], ar[k], c.y));
+      hr[k] = nr;
+    }
+  }
+  // T = h tau / s
+  const double is = 1.0 / s;  // exact
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    acc[k / 3][k % 3][0] = (hr[k] * ar[k] - hi[k] * ai[k]) * is;
+    acc[k / 3][k % 3][1] = (hr[k] * ai[k] + hi[k] * ar[k]) * is;
+  }
+}
+
 // Waves per SIMD: FP64 four (128 VGPRs); FP32 eight (64 VGPRs), since FP32 VALU
 // instructions of two waves of a SIMD issue together and the pair rate grows with the
-// number of waves.
-template <int NT, int BPL, bool F32>
+// number of waves.  FAR: the moment form (FP64): an item of a chunk re-expanded into its
+// row adds nothing, the chunk that evaluates its row adds the row's far-field value
+// (`mrows`), a far chunk outside the row's disc its own (`mom`); the other items take the
+// direct FP64 sum exactly as without FAR.
+template <int NT, int BPL, bool F32, bool FAR>
 __global__ __launch_bounds__(NT, F32 ? 8 : 4) void k_gpp_main(const double2* __restrict__ wt,
                                                  const double2* __restrict__ Ieps,
                                                  const double2* __restrict__ S,
@@ -286,10 +619,14 @@ __global__ __launch_bounds__(NT, F32 ? 8 : 4) void k_gpp_main(const double2* __r
                                                  const double* __restrict__ wxrange, int nb,
                                                  int ngpown, int ncouls, int chunk, int nchunks,
                                                  long item0, long item1, int carry,
+                                                 const MomChunk* __restrict__ mom,
+                                                 const MomRow* __restrict__ mrows,
                                                  double2* __restrict__ partial) {
   static_assert(!F32 || BPL == 1, "the FP32-bulk form runs one band per lane");
+  static_assert(!(F32 && FAR), "the moment form is FP64");
   __shared__ TileEntry tile[NT];
   __shared__ TileEntry32 tile32[F32 ? NT : 1];
+  __shared__ double2 nu_s[FAR ? kMom : 1];
   const int tid = threadIdx.x;
   const int band0 = blockIdx.y * NT * BPL;
   int n1[BPL];
@@ -321,12 +658,24 @@ __global__ __launch_bounds__(NT, F32 ? 8 : 4) void k_gpp_main(const double2* __r
 #pragma unroll
     for (int iw = 0; iw < 3; ++iw) x32[bb][iw] = (float)x[bb][iw];
 
-  for (long item = item0 + blockIdx.x; item < nitems; item += gridDim.x) {
+  // FAR: the chunk modes of the workgroup's next 64 items are loaded at once (lane k:
+  // item + k * gridDim.x) and read lane by lane
+  int modes = kChunkDirect, k = 0;
+  for (long item = item0 + blockIdx.x; item < nitems; item += gridDim.x, k = (k + 1) & 63) {
     const int p = (int)(item / nchunks);
     const int c = (int)(item - (long)p * nchunks);
     const int gb = c * chunk;
     const int ge = min(ncouls, gb + chunk);
     const size_t rowoff = (size_t)p * ncouls;
+    int mode = kChunkDirect;
+    if constexpr (FAR) {
+      if (k == 0) {
+        const long it = item + (long)(tid & 63) * gridDim.x;
+        modes = it < nitems ? mom[it].mode : kChunkDirect;
+      }
+      mode = __builtin_amdgcn_readlane(modes, k);  // uniform: the item's
+      if (mode == kChunkFar) continue;  // re-expanded into its row's moments, evaluated with the row
+    }
     // sch_store1(n1,p) of this lane's bands (zero for idle lanes)
     double2 sv[BPL];
 #pragma unroll
@@ -339,7 +688,24 @@ __global__ __launch_bounds__(NT, F32 ? 8 : 4) void k_gpp_main(const double2* __r
 #pragma unroll
       for (int iw = 0; iw < 3; ++iw) acc[bb][iw][0] = acc[bb][iw][1] = 0.0;
 
-    for (int g0 = gb; g0 < ge; g0 += NT) {
+    bool far = false;
+    if constexpr (FAR) {
+      if (mode == kChunkEval || mode == kChunkSelf) {
+        // uniform: the row's summed moments, or the chunk's own
+        const MomRow* __restrict__ rw = mrows + p;
+        const MomChunk* __restrict__ mc = mom + item;
+        const bool row = mode == kChunkEval;
+        const double2* __restrict__ src = row ? rw->nu : mc->nu;
+        const double cx = row ? rw->cx : mc->cx, cy = row ? rw->cy : mc->cy, s = row ? rw->s : mc->s;
+        __syncthreads();  // the previous item's readers of nu_s are done
+        if (tid < kMom) nu_s[tid] = src[tid];
+        __syncthreads();
+        row_far<BPL>(nu_s, cx, cy, s, x, acc);
+        far = true;
+      }
+    }
+
+    for (int g0 = gb; g0 < (far ? gb : ge); g0 += NT) {
       const int g = g0 + tid;
       bool ok;
       TileEntry e;
@@ -498,14 +864,14 @@ __global__ __launch_bounds__(NT) void k_gpp_asx(const int* __restrict__ inv, con
   }
 }
 
-template <int NT, int BPL, bool F32>
+template <int NT, int BPL, bool F32, bool FAR = false>
 int launch_main(bgw_ctx* ctx, const double2* wt, const double2* I, const double2* S,
                 const double* wxb, const double* range, int nb, int ngpown, int ncouls,
                 double2* partial_ws, int* W_out, int chunk, long item0, long item1) {
   const int nchunks = (ncouls + chunk - 1) / chunk;
   const int btiles = (nb + NT * BPL - 1) / (NT * BPL);
   int per_cu = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_gpp_main<NT, BPL, F32>, NT, 0) != hipSuccess ||
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_gpp_main<NT, BPL, F32, FAR>, NT, 0) != hipSuccess ||
       per_cu < 1)
     per_cu = 1;
   long items = (long)ngpown * nchunks;
@@ -514,10 +880,36 @@ int launch_main(bgw_ctx* ctx, const double2* wt, const double2* I, const double2
   if (W > items) W = items;
   *W_out = (int)W;
   if (!partial_ws) return BGW_OK;  // sizing query
-  KernelTimer kt(ctx, "gpp_main");
   if (item1 < 0) item1 = items;
-  hipLaunchKernelGGL((k_gpp_main<NT, BPL, F32>), dim3((unsigned)W, btiles), dim3(NT), 0, ctx->stream, wt, I, S,
-                     wxb, range, nb, ngpown, ncouls, chunk, nchunks, item0, item1, item0 > 0 ? 1 : 0,
+  MomChunk* mom = nullptr;
+  MomRow* mrows = nullptr;
+  if constexpr (FAR) {
+    // Moments of every chunk of the rows that items [item0, item1) reach and that an
+    // earlier launch of this call has not covered (whole rows: a row's pass needs all of
+    // its chunks), the row pass over them, then the evaluation of the items.
+    void *pm, *pr;
+    int rc = pool_get(ctx, "ws_moments", (size_t)items * sizeof(MomChunk), &pm);
+    if (!rc) rc = pool_get(ctx, "ws_moment_rows", (size_t)ngpown * sizeof(MomRow), &pr);
+    if (rc) return rc;
+    mom = static_cast<MomChunk*>(pm);
+    mrows = static_cast<MomRow*>(pr);
+    const int r0 = item0 > 0 ? (int)((item0 - 1) / nchunks) + 1 : 0;
+    const int r1 = (int)((item1 - 1) / nchunks) + 1;
+    if (r1 > r0) {
+      {
+        KernelTimer kt(ctx, "gpp_moments");
+        const long n = (long)(r1 - r0) * nchunks;
+        hipLaunchKernelGGL(k_gpp_moments, dim3((unsigned)((n + kMomWaves - 1) / kMomWaves)), dim3(64 * kMomWaves), 0,
+                           ctx->stream, wt, I, ncouls, chunk, nchunks, (long)r0 * nchunks, (long)r1 * nchunks, mom);
+      }
+      hipLaunchKernelGGL(k_gpp_moments_row, dim3((unsigned)(r1 - r0)), dim3(64 * kRowWaves), 0, ctx->stream, nchunks,
+                         r0, (int)W, range, mom, mrows);
+      if ((rc = check_launch(ctx, "k_gpp_moments"))) return rc;
+    }
+  }
+  KernelTimer kt(ctx, FAR ? "gpp_far" : "gpp_main");
+  hipLaunchKernelGGL((k_gpp_main<NT, BPL, F32, FAR>), dim3((unsigned)W, btiles), dim3(NT), 0, ctx->stream, wt, I, S,
+                     wxb, range, nb, ngpown, ncouls, chunk, nchunks, item0, item1, item0 > 0 ? 1 : 0, mom, mrows,
                      partial_ws);
   return check_launch(ctx, "k_gpp_main");
 }
@@ -559,6 +951,18 @@ int run_main_t(bgw_ctx* ctx, MainCfg cfg, const double2* wt, const double2* I, c
   }
 }
 
+int run_far(bgw_ctx* ctx, MainCfg cfg, const double2* wt, const double2* I, const double2* S, const double* wxb,
+            const double* range, int nb, int ngpown, int ncouls, double2* ws, int* W, long item0, long item1) {
+  const int chunk = kGChunk;
+  if (cfg.nt == 256 && cfg.bpl == 2)
+    return launch_main<256, 2, false, true>(ctx, wt, I, S, wxb, range, nb, ngpown, ncouls, ws, W, chunk, item0, item1);
+  if (cfg.nt == 128 && cfg.bpl == 2)
+    return launch_main<128, 2, false, true>(ctx, wt, I, S, wxb, range, nb, ngpown, ncouls, ws, W, chunk, item0, item1);
+  if (cfg.nt == 128)
+    return launch_main<128, 1, false, true>(ctx, wt, I, S, wxb, range, nb, ngpown, ncouls, ws, W, chunk, item0, item1);
+  return launch_main<64, 1, false, true>(ctx, wt, I, S, wxb, range, nb, ngpown, ncouls, ws, W, chunk, item0, item1);
+}
+
 // Launches the main kernel over work items [item0, item1) (item1 < 0: all).  A
 // launch with item0 > 0 continues the per-workgroup totals of the launches before
 // it; item0 must then be a multiple of the grid width *W.
@@ -567,7 +971,16 @@ int run_main(bgw_ctx* ctx, MainCfg cfg, const double2* wt, const double2* I, con
              int* W, long item0 = 0, long item1 = -1) {
   if (ctx->precision == BGW_FP32_BULK)
     return run_main_t<true>(ctx, cfg, wt, I, S, wxb, range, nb, ngpown, ncouls, ws, W, item0, item1);
+  if (ctx->precision == BGW_FP64_MOMENTS)
+    return run_far(ctx, cfg, wt, I, S, wxb, range, nb, ngpown, ncouls, ws, W, item0, item1);
   return run_main_t<false>(ctx, cfg, wt, I, S, wxb, range, nb, ngpown, ncouls, ws, W, item0, item1);
+}
+
+// Bytes of the moments table of a forward call (the moment form only).
+size_t moments_bytes(bgw_ctx* ctx, int ngpown, int ncouls) {
+  if (ctx->precision != BGW_FP64_MOMENTS) return 0;
+  return (((size_t)ngpown * ((ncouls + kGChunk - 1) / kGChunk) * sizeof(MomChunk) + 255) & ~size_t(255)) +
+         (size_t)ngpown * sizeof(MomRow);
 }
 
 }  // namespace
@@ -845,7 +1258,8 @@ extern "C" int bgw_gpp_solver_device_noalloc(bgw_ctx* ctx, size_t number_bands, 
   int W = 0;
   run_main(ctx, pick_main(nb), nullptr, nullptr, nullptr, nullptr, nullptr, nb, (int)ngpown, (int)ncouls, nullptr, &W);
   const double need = (double)al256((size_t)W * nb * 48) + (double)al256((size_t)nb * 96) +
-                      (double)al256(16 * number_bands * ngpown) + (double)al256((size_t)nb * 24) + 256.0;
+                      (double)al256(16 * number_bands * ngpown) + (double)al256((size_t)nb * 24) + 256.0 +
+                      (double)al256(moments_bytes(ctx, (int)ngpown, (int)ncouls));
   if (need > (double)kScratchBytes)
     return fail(ctx, BGW_EINVAL, "number_bands x ngpown too large for the 64 MiB scratch of the allocation-free call");
   ctx->arena = static_cast<char*>(ctx->scratch);

@@ -18,7 +18,7 @@ LIB_PATH = PKG / "build" / "libbgw.so"
 HEADER = PKG.parent / "include" / "bgw.h"
 
 BGW_OK, BGW_EINVAL, BGW_ENOMEM, BGW_EHIP, BGW_ENODEV, BGW_EIO = 0, -22, -12, -5, -19, -74
-BGW_FP64, BGW_FP32_BULK = 0, 1
+BGW_FP64, BGW_FP32_BULK, BGW_FP64_MOMENTS = 0, 1, 2
 BGW_INPUTS_GPP, BGW_INPUTS_FF, BGW_INPUTS_OTHER = 1, 2, 3
 BGW_DT_F64, BGW_DT_C128, BGW_DT_I32, BGW_DT_I64 = 1, 2, 3, 4
 

@@ -1,6 +1,6 @@
 """bgw_gpp_solver_device_noalloc: the allocation-free form of bgw_gpp_solver_device.
-Its results are compared with that entry point's on the same device arrays, in both
-arithmetic forms, over two calls in a row (the second one smaller), and the device's free
+Its results are compared with that entry point's on the same device arrays, in every
+arithmetic form, over two calls in a row (the second one smaller), and the device's free
 memory is checked to be unchanged across the allocation-free calls."""
 import numpy as np
 import pytest
@@ -33,7 +33,7 @@ def _run(torch, ctx, d, T, out, noalloc):
     return ach.cpu().numpy(), loc.cpu().numpy()
 
 
-@pytest.mark.parametrize("precision", [bgw.BGW_FP64, bgw.BGW_FP32_BULK])
+@pytest.mark.parametrize("precision", [bgw.BGW_FP64, bgw.BGW_FP32_BULK, bgw.BGW_FP64_MOMENTS])
 def test_noalloc_matches_device_entry_point(precision):
     torch = pytest.importorskip("torch")
     cases = [bgw.gpp_inputs(300, 2, 4000, 160, 501), bgw.gpp_inputs(40, 2, 1100, 37, 502)]  # second call smaller
