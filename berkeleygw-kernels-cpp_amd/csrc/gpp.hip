@@ -279,39 +279,70 @@ __device__ inline void tile_f32(const TileEntry32* __restrict__ tile, int n, con
 // ------------------------------------------------------------ moment (far-field) form
 // The G-sum of one work item (p, G-chunk) is a sum of simple poles: conj(x - w)/|x - w|^2
 // = 1/(x - w) (GPP/gpp.cpp:376-380), so with w = wtilde(p,g) and I = I_eps(p,g)
-//   T(x) = sum_g I_g / (x - w_g).
-// With every pole within R of a centre c and every band frequency x at least D from c,
-// 1/(x - w) = sum_{m>=0} z^m / (x - c)^(m+1) with z = w - c, and the tail of a pole beyond
-// m = M - 1 is exactly (z/(x - c))^M / (x - w).  Hence
-//   T(x) = t sum_{m<M} nu_m (s t)^m + E,   t = 1/(x - c),   nu_m = sum_g I_g (z_g/s)^m,
-//   |E| <= (R/D)^M sum_g |I_g / (x - w_g)|,
-// where s is a power of two near the poles' extent (an exact scaling that keeps the powers
-// in range).  A chunk is "far" when R <= kFarRho D, which holds E below 2^-54 times the sum
-// of the magnitudes of its terms -- under the rounding error bound of the direct sum.  The
-// far chunks of a G' row are re-expanded about one centre and added, so the row is
-// evaluated once per band frequency; every other chunk (poles near a band frequency, a
-// non-finite pole, weight or moment) takes the direct sum.  The moments cost O(M) per
-// (p, g) instead of O(3 nb).
-constexpr int kMom = 28;
-constexpr double kFarRho = 0.26;  // 0.26^28 = 4.1e-17 < 2^-54
-constexpr int kMomSample = 256;   // the first poles of a chunk, whose bounding box gives its centre and scale
+//   T(x) = sum_g I_g / (x - w_g),
+// to be evaluated at the band frequencies x, all in [xmin, xmax].  With e = (xmin + xmax)/2
+// and a centre c of the chunk's poles, the Moebius variable phi(z) = (z - c)/(z - e) gives
+//   phi(x) - phi(w) = (c - e)(x - w) / ((x - e)(w - e)),
+// so 1/(x - w) = (c - e) / ((x - c)(w - e)) sum_{m>=0} (phi(w)/phi(x))^m, and the tail of a
+// pole beyond m = M - 1 is exactly (phi(w)/phi(x))^M times its term.  Hence
+//   T(x) = (c - e)/(x - c) sum_{m<M} nu_m psi^m + E,   psi = 1/phi(x) = (x - e)/(x - c),
+//   nu_m = sum_g I_g phi(w_g)^m / (w_g - e),   |E| <= rho^M sum_g |I_g / (x - w_g)|,
+// with rho = P h / D: P bounds |phi(w)| over the chunk, h = max |x - e| = (xmax - xmin)/2
+// and D = min |x - c| (so |phi(x)| >= D/h).  A chunk is "far" when rho <= kFarRho, which
+// holds E below 2^-54 times the sum of the magnitudes of its terms -- under the rounding
+// error bound of the direct sum.  (Poles 5 away from a frequency range of width 4, as in
+// the benchmark workload, give rho ~ 0.06; the Taylor expansion about c, e at infinity,
+// would give R/D ~ 0.2 and need 28 moments.)  The far chunks of a G' row are re-expanded
+// about one centre and added, so the row is evaluated once per band frequency; every
+// other chunk (poles near a band frequency, a non-finite pole, weight or moment) takes the
+// direct sum.  The moments cost O(M) per (p, g) instead of O(3 nb).
+constexpr int kMom = 16;
+constexpr double kFarRho = 0.095;  // 0.095^16 = 4.4e-17 < 2^-54
+constexpr int kMomSample = 256;    // the first poles of a chunk, whose bounding box gives its centre
 
 constexpr int kChunkDirect = 0;  // the direct sum
 constexpr int kChunkFar = 1;     // re-expanded into its row's moments, evaluated with the row
 constexpr int kChunkEval = 2;    // far, and evaluates its row's moments
-constexpr int kChunkSelf = 3;    // far, but evaluated from its own moments (outside the row's disc)
+constexpr int kChunkSelf = 3;    // far, but evaluated from its own moments (outside the row's bound)
 
 struct __align__(16) MomChunk {
   double cx, cy;  // centre c
-  double s;       // scale, a power of two
-  double R;       // bound of |w - c| over the chunk's poles; +inf: not usable
-  int mode, pad_[3];
+  double P;       // bound of |phi(w)| over the chunk's poles; +inf: not usable
+  int mode, pad_;
   double2 nu[kMom];
 };
 struct __align__(16) MomRow {
-  double cx, cy, s, pad_;
-  double2 nu[kMom];  // sum over the row's kChunkFar/kChunkEval chunks, about (cx, cy) with scale s
+  double cx, cy;
+  double2 nu[kMom];  // sum over the row's kChunkFar/kChunkEval chunks, about centre (cx, cy)
 };
+
+// e, h and the far test of a centre c with bound P, from the range [xmin, xmax] of the band
+// frequencies: rho = P h / D <= kFarRho with D the distance from c to the range, and every
+// |x - c|^2 inside the batched inversion's [2^-150, 2^150] (false for NaN).
+struct FarRange {
+  double e, h, xmin, xmax;
+};
+__device__ inline FarRange far_range(const double* __restrict__ wxrange) {
+  FarRange f;
+  f.xmin = wxrange[0];
+  f.xmax = wxrange[1];
+  f.e = 0.5 * (f.xmin + f.xmax);
+  f.h = fmax(f.xmax - f.e, f.e - f.xmin) * (1.0 + 0x1p-40);
+  return f;
+}
+__device__ inline bool far_ok(double cx, double cy, double P, const FarRange& f) {
+  const double dx = cx < f.xmin ? f.xmin - cx : (cx > f.xmax ? cx - f.xmax : 0.0);
+  const double D = sqrt(fma(dx, dx, cy * cy)) * (1.0 - 0x1p-40);
+  const double e0 = f.xmin - cx, e1 = f.xmax - cx;
+  const double dmax2 = fmax(e0 * e0, e1 * e1) + cy * cy;
+  return P * f.h <= kFarRho * D && D * D >= kDMin && dmax2 <= kDMax;
+}
+
+// Reciprocal for the moments: v_rcp_f64 and one Newton step (within ~11 ulp).
+__device__ inline double rcp_nr(double q) {
+  const double r = __builtin_amdgcn_rcp(q);
+  return fma(r, fma(-q, r, 1.0), r);
+}
 
 // Sums of the N doubles v[] over the 64 lanes of a wave, reduce-scatter style: while N
 // is even each exchange halves the values a lane carries (a lane keeps the lower or the
@@ -348,14 +379,15 @@ __device__ __forceinline__ void wave_sum_scatter(double* v, int lane, int& base,
 }
 
 // One wave per work item (p, G-chunk [gb, ge)), four items per workgroup: the chunk's
-// centre and scale from the bounding box of its first kMomSample poles (any centre is
-// valid: R is measured over every pole), its moments and R.  A non-finite pole or weight
-// makes a moment non-finite (NaN and Inf survive every sum and product), which marks the
-// chunk as not usable.
+// centre from the bounding box of its first kMomSample poles (any centre is valid: P is
+// measured over every pole), its moments and P.  A non-finite pole or weight, or a pole
+// at e, makes a moment non-finite (NaN and Inf survive every sum and product), which
+// marks the chunk as not usable.
 constexpr int kMomWaves = 4;
 
 __global__ __launch_bounds__(64 * kMomWaves) void k_gpp_moments(const double2* __restrict__ wt,
-                                                                const double2* __restrict__ Ieps, int ncouls,
+                                                                const double2* __restrict__ Ieps,
+                                                                const double* __restrict__ wxrange, int ncouls,
                                                                 int chunk, int nchunks, long item0, long item1,
                                                                 MomChunk* __restrict__ tab) {
   const int lane = threadIdx.x & 63;
@@ -366,6 +398,7 @@ __global__ __launch_bounds__(64 * kMomWaves) void k_gpp_moments(const double2* _
   const int ge = min(ncouls, gb + chunk);
   const double2* __restrict__ wrow = wt + (size_t)p * ncouls;
   const double2* __restrict__ irow = Ieps + (size_t)p * ncouls;
+  const double e = 0.5 * (wxrange[0] + wxrange[1]);
 
   double lr = INFINITY, hr = -INFINITY, li = INFINITY, hi = -INFINITY;
 #pragma unroll
@@ -388,15 +421,11 @@ __global__ __launch_bounds__(64 * kMomWaves) void k_gpp_moments(const double2* _
   }
   double cx = 0.5 * (lr + hr), cy = 0.5 * (li + hi);
   if (!(fabs(cx) <= 0x1p+1000 && fabs(cy) <= 0x1p+1000)) cx = cy = 0.0;  // a sample of NaN/Inf poles
-  const double half = 0.5 * fmax(hr - lr, hi - li);
-  int e = 0;
-  if (half >= 0x1p-1000 && half <= 0x1p+1000) (void)frexp(half, &e);  // half < 2^e
-  const double s = ldexp(1.0, e), is = ldexp(1.0, -e);
 
   double v[2 * kMom];
 #pragma unroll
   for (int j = 0; j < 2 * kMom; ++j) v[j] = 0.0;
-  double r2 = 0.0;
+  double f2 = 0.0;  // max |phi|^2
   // one pole ahead: the loads of the next pole are in flight while this one is summed
   // (the last index is re-read, in bounds, past the end)
   int g = gb + lane;
@@ -409,17 +438,20 @@ __global__ __launch_bounds__(64 * kMomWaves) void k_gpp_moments(const double2* _
     // keeps the loads issued here, ahead of this pole's arithmetic (no wait: the
     // barrier has no register operands), instead of sunk to their use
     asm volatile("" ::: "memory");
-    const double zr = w.x - cx, zi = w.y - cy;
-    r2 = fmax(r2, fma(zr, zr, zi * zi));
-    const double ur = zr * is, ui = zi * is;  // exact: is is a power of two
-    double pr = I.x, pi = I.y;
+    const double dr = w.x - e, di = w.y;                 // w - e
+    const double r = rcp_nr(fma(dr, dr, di * di));
+    const double gr = dr * r, gi = -di * r;              // 1/(w - e)
+    const double zr = w.x - cx, zi = w.y - cy;           // w - c
+    const double fr = fma(zr, gr, -zi * gi), fi = fma(zr, gi, zi * gr);  // phi
+    f2 = fmax(f2, fma(fr, fr, fi * fi));
+    double pr = fma(I.x, gr, -I.y * gi), pi = fma(I.x, gi, I.y * gr);    // I/(w - e)
 #pragma unroll
     for (int m = 0; m < kMom; ++m) {
       v[2 * m] += pr;
       v[2 * m + 1] += pi;
       if (m + 1 < kMom) {
-        const double t = fma(pr, ur, -pi * ui);
-        pi = fma(pr, ui, pi * ur);
+        const double t = fma(pr, fr, -pi * fi);
+        pi = fma(pr, fi, pi * fr);
         pr = t;
       }
     }
@@ -432,7 +464,7 @@ __global__ __launch_bounds__(64 * kMomWaves) void k_gpp_moments(const double2* _
 #pragma unroll
   for (int j = 0; j < NF; ++j) fin &= fabs(v[j]) <= DBL_MAX;
 #pragma unroll
-  for (int m = 32; m > 0; m >>= 1) r2 = fmax(r2, __shfl_xor(r2, m, 64));
+  for (int m = 32; m > 0; m >>= 1) f2 = fmax(f2, __shfl_xor(f2, m, 64));
   const bool ok = __all(fin);
   MomChunk* __restrict__ me = tab + item;
   double* __restrict__ out = reinterpret_cast<double*>(me->nu);
@@ -443,35 +475,22 @@ __global__ __launch_bounds__(64 * kMomWaves) void k_gpp_moments(const double2* _
   if (lane == 0) {
     me->cx = cx;
     me->cy = cy;
-    me->s = s;
-    // sqrt(r2) is within a few ulp of max |w - c|: the factor makes R a bound
-    me->R = ok ? sqrt(r2) * (1.0 + 0x1p-40) : INFINITY;
+    // sqrt(f2) is within a few tens of ulp of max |phi|: the factor makes P a bound
+    me->P = ok ? sqrt(f2) * (1.0 + 0x1p-40) : INFINITY;
   }
 }
 
-// A chunk's disc, centre (cx, cy) radius R, is far from the band frequencies' range
-// [wxmin, wxmax] when R <= kFarRho D, D the distance from the centre to the range, and
-// every |x - c|^2 stays inside the batched inversion's [2^-150, 2^150] (false for NaN).
-__device__ inline bool disc_far(double cx, double cy, double R, double wxmin, double wxmax) {
-  const double dx = cx < wxmin ? wxmin - cx : (cx > wxmax ? cx - wxmax : 0.0);
-  const double D = sqrt(fma(dx, dx, cy * cy)) * (1.0 - 0x1p-40);
-  const double e0 = wxmin - cx, e1 = wxmax - cx;
-  const double dmax2 = fmax(e0 * e0, e1 * e1) + cy * cy;
-  return R <= kFarRho * D && D * D >= kDMin && dmax2 <= kDMax;
-}
-
-// Row pass, one workgroup per G' row p0 + blockIdx.x.  The row's centre and scale are
-// those of its first far chunk, taken from chunk floor(p nchunks / W) mod nchunks on, which
-// becomes kChunkEval: with W the main kernel's grid width (item i runs on workgroup i mod W),
-// this spreads the rows' evaluations over all of its workgroups.  Each other far chunk whose
-// disc, widened by the distance between the centres, is still far about the row's centre
-// is re-expanded there and added (kChunkFar); the rest evaluate their own moments
-// (kChunkSelf) or take the direct sum.  Re-expansion from centre c, scale s to c', s':
-//   nu'_m = sum_{j<=m} C(m,j) (s/s')^j ((c - c')/s')^(m-j) nu_j,
-// the binomial expansion of ((w - c) + (c - c'))^m, exact in exact arithmetic, formed in
-// lane m by a Taylor shift in (c - c')/s' (no binomial coefficients, no division).  Wave
-// w takes the chunks c = w mod kRowWaves in chunk order, and the waves' sums are added in
-// wave order.
+// Row pass, one workgroup per G' row p0 + blockIdx.x.  The row's centre is that of its
+// first far chunk, taken from chunk floor(p nchunks / W) mod nchunks on, which becomes
+// kChunkEval: with W the main kernel's grid width (item i runs on workgroup i mod W), this
+// spreads the rows' evaluations over all of its workgroups.  Each other far chunk is
+// re-expanded about the row's centre c' when its bound there stays far (kChunkFar), and
+// evaluates its own moments otherwise (kChunkSelf); the rest take the direct sum.  About
+// c', phi' = b + a phi with b = (c - c')/(c - e), a = 1 - b, so
+//   nu'_m = sum_{j<=m} C(m,j) b^(m-j) a^j nu_j,   |phi'| <= |b| + |a| P,
+// formed in lane m from a^m nu_m by a Taylor shift in b (kMom - 1 rounds of
+// mu_m += b mu_(m-1), no binomial coefficients).  Wave w takes the chunks c = w mod
+// kRowWaves in chunk order, and the waves' sums are added in wave order.
 constexpr int kRowWaves = 4;
 
 __global__ __launch_bounds__(64 * kRowWaves) void k_gpp_moments_row(int nchunks, int p0, int W,
@@ -481,7 +500,7 @@ __global__ __launch_bounds__(64 * kRowWaves) void k_gpp_moments_row(int nchunks,
   __shared__ double2 part[kRowWaves][kMom];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int p = p0 + blockIdx.x;
-  const double wxmin = wxrange[0], wxmax = wxrange[1];
+  const FarRange fr = far_range(wxrange);
   MomChunk* __restrict__ ch = tab + (size_t)p * nchunks;
   // the first far chunk from chunk `first` on, 64 chunks tested at a time (every wave alike)
   const int first = (int)(((long)p * nchunks / W) % nchunks);
@@ -489,7 +508,7 @@ __global__ __launch_bounds__(64 * kRowWaves) void k_gpp_moments_row(int nchunks,
   for (int k0 = 0; k0 < nchunks && d < 0; k0 += 64) {
     const int k = k0 + lane;
     const int c = first + k < nchunks ? first + k : first + k - nchunks;
-    const bool far = k < nchunks && disc_far(ch[c].cx, ch[c].cy, ch[c].R, wxmin, wxmax);
+    const bool far = k < nchunks && far_ok(ch[c].cx, ch[c].cy, ch[c].P, fr);
     const unsigned long long b = __ballot(far);
     if (b) {
       const int kk = k0 + __builtin_ctzll(b);
@@ -500,39 +519,52 @@ __global__ __launch_bounds__(64 * kRowWaves) void k_gpp_moments_row(int nchunks,
     for (int c = threadIdx.x; c < nchunks; c += 64 * kRowWaves) ch[c].mode = kChunkDirect;
     return;
   }
-  const double pcx = ch[d].cx, pcy = ch[d].cy, ps = ch[d].s, ips = 1.0 / ps;  // exact
+  const double pcx = ch[d].cx, pcy = ch[d].cy;
   const int m = lane < kMom ? lane : kMom - 1;
   double sr = 0.0, si = 0.0;
   for (int c = wave; c < nchunks; c += kRowWaves) {
-    const double cx = ch[c].cx, cy = ch[c].cy, R = ch[c].R;
+    const double cx = ch[c].cx, cy = ch[c].cy, P = ch[c].P;
     int mode = kChunkDirect;
     if (c == d) {
       mode = kChunkEval;
       sr += ch[c].nu[m].x;
       si += ch[c].nu[m].y;
-    } else if (disc_far(cx, cy, R, wxmin, wxmax)) {
-      const double br = (cx - pcx) * ips, bi = (cy - pcy) * ips;  // (c - c')/s'
-      const double Rw = (R + sqrt(fma(cx - pcx, cx - pcx, (cy - pcy) * (cy - pcy)))) * (1.0 + 0x1p-40);
+    } else if (far_ok(cx, cy, P, fr)) {
       mode = kChunkSelf;
-      if (disc_far(pcx, pcy, Rw, wxmin, wxmax)) {
-        // lane j: mu_j = (s/s')^j nu_j (exact: s/s' = 2^ea), then the Taylor shift,
-        // kMom - 1 rounds of mu_m += b mu_(m-1) over m >= k (old values), which leaves
-        // mu_m = sum_{j<=m} C(m,j) b^(m-j) mu_j in lane m
-        int ea = 0;
-        (void)frexp(ch[c].s * ips, &ea);  // s/s' = 2^(ea - 1)
+      // b = (c - c')/(c - e), a = 1 - b
+      const double ur = cx - fr.e, ui = cy;
+      const double q = 1.0 / fma(ur, ur, ui * ui);
+      const double vr = cx - pcx, vi = cy - pcy;
+      const double br = (vr * ur + vi * ui) * q, bi = (vi * ur - vr * ui) * q;
+      const double ar = 1.0 - br, ai = -bi;
+      const double Pw = (sqrt(fma(br, br, bi * bi)) + sqrt(fma(ar, ar, ai * ai)) * P) * (1.0 + 0x1p-40);
+      if (far_ok(pcx, pcy, Pw, fr)) {
+        // lane m: a^m by binary powering, mu_m = a^m nu_m, then the Taylor shift in b
+        double tr = 1.0, ti = 0.0, kr = ar, ki = ai;
+#pragma unroll
+        for (int bit = 1; bit < kMom; bit <<= 1) {
+          if (m & bit) {
+            const double nr = fma(tr, kr, -ti * ki);
+            ti = fma(tr, ki, ti * kr);
+            tr = nr;
+          }
+          const double nr = fma(kr, kr, -ki * ki);
+          ki = 2.0 * kr * ki;
+          kr = nr;
+        }
         const double2 nj = ch[c].nu[m];
-        double ur = ldexp(nj.x, m * (ea - 1)), ui = ldexp(nj.y, m * (ea - 1));
+        double mr = fma(tr, nj.x, -ti * nj.y), mi = fma(tr, nj.y, ti * nj.x);
         for (int k = 1; k < kMom; ++k) {
-          const double qr = __shfl_up(ur, 1, 64), qi = __shfl_up(ui, 1, 64);
+          const double qr = __shfl_up(mr, 1, 64), qi = __shfl_up(mi, 1, 64);
           if (lane >= k && lane < kMom) {
-            const double nr = fma(br, qr, fma(-bi, qi, ur));
-            ui = fma(br, qi, fma(bi, qr, ui));
-            ur = nr;
+            const double nr = fma(br, qr, fma(-bi, qi, mr));
+            mi = fma(br, qi, fma(bi, qr, mi));
+            mr = nr;
           }
         }
-        if (__all(fabs(ur) <= DBL_MAX && fabs(ui) <= DBL_MAX)) {
-          sr += ur;
-          si += ui;
+        if (__all(fabs(mr) <= DBL_MAX && fabs(mi) <= DBL_MAX)) {
+          sr += mr;
+          si += mi;
           mode = kChunkFar;
         }
       }
@@ -553,64 +585,70 @@ __global__ __launch_bounds__(64 * kRowWaves) void k_gpp_moments_row(int nchunks,
   if (threadIdx.x == 0) {
     rows[p].cx = pcx;
     rows[p].cy = pcy;
-    rows[p].s = ps;
   }
 }
 
-// Far-field value of a set of moments (a row's summed far chunks, or one chunk) for a
-// lane's 3*BPL frequencies: acc[b][iw] = T(x[b][iw]).  nu: the moments (LDS).  The 3*BPL reciprocals
-// come from one batched inversion, as in the direct form.
+// Far-field value of a set of moments about centre c (a row's summed far chunks, or one
+// chunk) for a lane's 3*BPL frequencies: acc[b][iw] = T(x[b][iw]).  nu: the moments (LDS).
+// One band at a time (registers); the three reciprocals 1/(x - c) of a band come from one
+// batched inversion, as in the direct form.
 template <int BPL>
-__device__ inline void row_far(const double2* __restrict__ nu, double cx, double cy, double s,
+__device__ inline void row_far(const double2* __restrict__ nu, double cx, double cy, double e,
                                const double (&x)[BPL][3], double (&acc)[BPL][3][2]) {
-  constexpr int K = BPL * 3;
-  double ar[K], ai[K];  // tau = s t
-  {
-    double a[K], d[K], r[K];
+  const double cer = cx - e, cei = cy;
+#pragma unroll 1
+  for (int bb = 0; bb < BPL; ++bb) {
+    double ur[3], ui[3], sr[3], si[3];  // (c - e) t and psi = (x - e) t, t = 1/(x - c)
+    {
+      double a[3], d[3], r[3];
 #pragma unroll
-    for (int k = 0; k < K; ++k) {
-      a[k] = x[k / 3][k % 3] - cx;
-      d[k] = fma(a[k], a[k], cy * cy);
+      for (int k = 0; k < 3; ++k) {
+        a[k] = x[bb][k] - cx;
+        d[k] = fma(a[k], a[k], cy * cy);
+      }
+      batch_inverse(d, r);
+      // 1/(x - c) = 1/((x - cx) - i cy) = ((x - cx) + i cy) / |x - c|^2
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        const double tr = a[k] * r[k], ti = cy * r[k];
+        const double xe = x[bb][k] - e;
+        sr[k] = xe * tr;
+        si[k] = xe * ti;
+        ur[k] = fma(cer, tr, -cei * ti);
+        ui[k] = fma(cer, ti, cei * tr);
+      }
     }
-    batch_inverse(d, r);
-    // 1/(x - c) = 1/((x - cx) - i cy) = ((x - cx) + i cy) / |x - c|^2
+    double hr[3], hi[3];
+    const double2 top = nu[kMom - 1];
 #pragma unroll
-    for (int k = 0; k < K; ++k) {
-      ar[k] = s * (a[k] * r[k]);
-      ai[k] = s * (cy * r[k]);
+    for (int k = 0; k < 3; ++k) {
+      hr[k] = top.x;
+      hi[k] = top.y;
     }
-  }
-  double hr[K], hi[K];
-  const double2 top = nu[kMom - 1];
+#pragma unroll 3
+    for (int m = kMom - 2; m >= 0; --m) {
+      const double2 c = nu[m];
 #pragma unroll
-  for (int k = 0; k < K; ++k) {
-    hr[k] = top.x;
-    hi[k] = top.y;
-  }
-  for (int m = kMom - 2; m >= 0; --m) {
-    const double2 c = nu[m];
-#pragma unroll
-    for (int k = 0; k < K; ++k) {
-      const double nr = fma(hr[k], ar[k], fma(-hi[k], ai[k], c.x));
-      hi[k] = fma(hr[k], ai[k], fma(hi[k], ar[k], c.y));
-      hr[k] = nr;
+      for (int k = 0; k < 3; ++k) {
+        const double nr = fma(hr[k], sr[k], fma(-hi[k], si[k], c.x));
+        hi[k] = fma(hr[k], si[k], fma(hi[k], sr[k], c.y));
+        hr[k] = nr;
+      }
     }
-  }
-  // T = h tau / s
-  const double is = 1.0 / s;  // exact
+    // T = (c - e) t h
 #pragma unroll
-  for (int k = 0; k < K; ++k) {
-    acc[k / 3][k % 3][0] = (hr[k] * ar[k] - hi[k] * ai[k]) * is;
-    acc[k / 3][k % 3][1] = (hr[k] * ai[k] + hi[k] * ar[k]) * is;
+    for (int k = 0; k < 3; ++k) {
+      acc[bb][k][0] = fma(ur[k], hr[k], -ui[k] * hi[k]);
+      acc[bb][k][1] = fma(ur[k], hi[k], ui[k] * hr[k]);
+    }
   }
 }
 
 // Waves per SIMD: FP64 four (128 VGPRs); FP32 eight (64 VGPRs), since FP32 VALU
 // instructions of two waves of a SIMD issue together and the pair rate grows with the
-// number of waves.  FAR: the moment form (FP64): an item of a chunk re-expanded into its
-// row adds nothing, the chunk that evaluates its row adds the row's far-field value
-// (`mrows`), a far chunk outside the row's disc its own (`mom`); the other items take the
-// direct FP64 sum exactly as without FAR.
+// number of waves.  FAR: the direct pass of the moment form (FP64), launched after
+// k_gpp_far_eval over the same items and grid and continuing its totals: it sums the
+// items of kChunkDirect chunks exactly as without FAR and skips the others.
 template <int NT, int BPL, bool F32, bool FAR>
 __global__ __launch_bounds__(NT, F32 ? 8 : 4) void k_gpp_main(const double2* __restrict__ wt,
                                                  const double2* __restrict__ Ieps,
@@ -626,7 +664,6 @@ __global__ __launch_bounds__(NT, F32 ? 8 : 4) void k_gpp_main(const double2* __r
   static_assert(!(F32 && FAR), "the moment form is FP64");
   __shared__ TileEntry tile[NT];
   __shared__ TileEntry32 tile32[F32 ? NT : 1];
-  __shared__ double2 nu_s[FAR ? kMom : 1];
   const int tid = threadIdx.x;
   const int band0 = blockIdx.y * NT * BPL;
   int n1[BPL];
@@ -674,7 +711,7 @@ __global__ __launch_bounds__(NT, F32 ? 8 : 4) void k_gpp_main(const double2* __r
         modes = it < nitems ? mom[it].mode : kChunkDirect;
       }
       mode = __builtin_amdgcn_readlane(modes, k);  // uniform: the item's
-      if (mode == kChunkFar) continue;  // re-expanded into its row's moments, evaluated with the row
+      if (mode != kChunkDirect) continue;  // summed by k_gpp_far_eval
     }
     // sch_store1(n1,p) of this lane's bands (zero for idle lanes)
     double2 sv[BPL];
@@ -688,24 +725,7 @@ __global__ __launch_bounds__(NT, F32 ? 8 : 4) void k_gpp_main(const double2* __r
 #pragma unroll
       for (int iw = 0; iw < 3; ++iw) acc[bb][iw][0] = acc[bb][iw][1] = 0.0;
 
-    bool far = false;
-    if constexpr (FAR) {
-      if (mode == kChunkEval || mode == kChunkSelf) {
-        // uniform: the row's summed moments, or the chunk's own
-        const MomRow* __restrict__ rw = mrows + p;
-        const MomChunk* __restrict__ mc = mom + item;
-        const bool row = mode == kChunkEval;
-        const double2* __restrict__ src = row ? rw->nu : mc->nu;
-        const double cx = row ? rw->cx : mc->cx, cy = row ? rw->cy : mc->cy, s = row ? rw->s : mc->s;
-        __syncthreads();  // the previous item's readers of nu_s are done
-        if (tid < kMom) nu_s[tid] = src[tid];
-        __syncthreads();
-        row_far<BPL>(nu_s, cx, cy, s, x, acc);
-        far = true;
-      }
-    }
-
-    for (int g0 = gb; g0 < (far ? gb : ge); g0 += NT) {
+    for (int g0 = gb; g0 < ge; g0 += NT) {
       const int g = g0 + tid;
       bool ok;
       TileEntry e;
@@ -763,6 +783,80 @@ __global__ __launch_bounds__(NT, F32 ? 8 : 4) void k_gpp_main(const double2* __r
           tot[bb][iw][1] = fma(sv[bb].x, acc[bb][iw][1], fma(sv[bb].y, acc[bb][iw][0], tot[bb][iw][1]));
         }
     }
+  }
+#pragma unroll
+  for (int bb = 0; bb < BPL; ++bb) {
+    if (n1[bb] < nb) {
+      double2* dst = partial + (size_t)blockIdx.x * nb * 3 + (size_t)n1[bb] * 3;
+#pragma unroll
+      for (int iw = 0; iw < 3; ++iw) dst[iw] = make_double2(tot[bb][iw][0], tot[bb][iw][1]);
+    }
+  }
+}
+
+// Far pass of the moment form, on the main kernel's grid and item order (lanes = bands):
+// workgroup b sums, over its items b, b + W, ..., the far-field values of the kChunkEval
+// items (the row's summed moments, `mrows`) and kChunkSelf items (the chunk's own, `mom`),
+// each folded with sch_store1(n1,p), into the per-workgroup totals; carry resumes them
+// from an earlier launch over the items before item0.  The direct pass (k_gpp_main with
+// FAR) continues these totals.
+template <int NT, int BPL>
+__global__ __launch_bounds__(NT, 4) void k_gpp_far_eval(const double2* __restrict__ S, const double* __restrict__ wxb,
+                                                    const double* __restrict__ wxrange, int nb, int nchunks,
+                                                    long item0, long item1, int carry,
+                                                    const MomChunk* __restrict__ mom,
+                                                    const MomRow* __restrict__ mrows, double2* __restrict__ partial) {
+  __shared__ double2 nu_s[kMom];
+  const int tid = threadIdx.x;
+  const int band0 = blockIdx.y * NT * BPL;
+  int n1[BPL];
+  double x[BPL][3];
+  double tot[BPL][3][2];
+#pragma unroll
+  for (int bb = 0; bb < BPL; ++bb) {
+    n1[bb] = band0 + tid + bb * NT;
+    const int nc = n1[bb] < nb ? n1[bb] : nb - 1;  // idle lanes mirror the last band
+    const double2* src = partial + (size_t)blockIdx.x * nb * 3 + (size_t)nc * 3;
+#pragma unroll
+    for (int iw = 0; iw < 3; ++iw) {
+      x[bb][iw] = wxb[nc * 3 + iw];
+      const double2 t0 = carry && n1[bb] < nb ? src[iw] : make_double2(0.0, 0.0);
+      tot[bb][iw][0] = t0.x;
+      tot[bb][iw][1] = t0.y;
+    }
+  }
+  const double e = 0.5 * (wxrange[0] + wxrange[1]);
+  // the chunk modes of the workgroup's next 64 items are loaded at once (lane k: item +
+  // k * gridDim.x) and read lane by lane
+  int modes = kChunkDirect, k = 0;
+  for (long item = item0 + blockIdx.x; item < item1; item += gridDim.x, k = (k + 1) & 63) {
+    if (k == 0) {
+      const long it = item + (long)(tid & 63) * gridDim.x;
+      modes = it < item1 ? mom[it].mode : kChunkDirect;
+    }
+    const int mode = __builtin_amdgcn_readlane(modes, k);  // uniform: the item's
+    if (mode != kChunkEval && mode != kChunkSelf) continue;
+    const int p = (int)(item / nchunks);
+    const bool row = mode == kChunkEval;
+    const MomRow* __restrict__ rw = mrows + p;
+    const MomChunk* __restrict__ mc = mom + item;
+    const double2* __restrict__ src = row ? rw->nu : mc->nu;
+    const double cx = row ? rw->cx : mc->cx, cy = row ? rw->cy : mc->cy;
+    double2 sv[BPL];
+#pragma unroll
+    for (int bb = 0; bb < BPL; ++bb) sv[bb] = n1[bb] < nb ? S[(size_t)p * nb + n1[bb]] : make_double2(0.0, 0.0);
+    __syncthreads();  // the previous item's readers of nu_s are done
+    if (tid < kMom) nu_s[tid] = src[tid];
+    __syncthreads();
+    double acc[BPL][3][2];
+    row_far<BPL>(nu_s, cx, cy, e, x, acc);
+#pragma unroll
+    for (int bb = 0; bb < BPL; ++bb)
+#pragma unroll
+      for (int iw = 0; iw < 3; ++iw) {
+        tot[bb][iw][0] = fma(sv[bb].x, acc[bb][iw][0], fma(-sv[bb].y, acc[bb][iw][1], tot[bb][iw][0]));
+        tot[bb][iw][1] = fma(sv[bb].x, acc[bb][iw][1], fma(sv[bb].y, acc[bb][iw][0], tot[bb][iw][1]));
+      }
   }
 #pragma unroll
   for (int bb = 0; bb < BPL; ++bb) {
@@ -900,17 +994,28 @@ int launch_main(bgw_ctx* ctx, const double2* wt, const double2* I, const double2
         KernelTimer kt(ctx, "gpp_moments");
         const long n = (long)(r1 - r0) * nchunks;
         hipLaunchKernelGGL(k_gpp_moments, dim3((unsigned)((n + kMomWaves - 1) / kMomWaves)), dim3(64 * kMomWaves), 0,
-                           ctx->stream, wt, I, ncouls, chunk, nchunks, (long)r0 * nchunks, (long)r1 * nchunks, mom);
+                           ctx->stream, wt, I, range, ncouls, chunk, nchunks, (long)r0 * nchunks, (long)r1 * nchunks,
+                           mom);
       }
       hipLaunchKernelGGL(k_gpp_moments_row, dim3((unsigned)(r1 - r0)), dim3(64 * kRowWaves), 0, ctx->stream, nchunks,
                          r0, (int)W, range, mom, mrows);
       if ((rc = check_launch(ctx, "k_gpp_moments"))) return rc;
     }
   }
-  KernelTimer kt(ctx, FAR ? "gpp_far" : "gpp_main");
+  if constexpr (FAR) {
+    // the far pass first, then the direct pass continues its totals
+    {
+      KernelTimer kt(ctx, "gpp_far");
+      hipLaunchKernelGGL((k_gpp_far_eval<NT, BPL>), dim3((unsigned)W, btiles), dim3(NT), 0, ctx->stream, S, wxb, range,
+                         nb, nchunks, item0, item1, item0 > 0 ? 1 : 0, mom, mrows, partial_ws);
+    }
+    int rc = check_launch(ctx, "k_gpp_far_eval");
+    if (rc) return rc;
+  }
+  KernelTimer kt(ctx, FAR ? "gpp_far_direct" : "gpp_main");
   hipLaunchKernelGGL((k_gpp_main<NT, BPL, F32, FAR>), dim3((unsigned)W, btiles), dim3(NT), 0, ctx->stream, wt, I, S,
-                     wxb, range, nb, ngpown, ncouls, chunk, nchunks, item0, item1, item0 > 0 ? 1 : 0, mom, mrows,
-                     partial_ws);
+                     wxb, range, nb, ngpown, ncouls, chunk, nchunks, item0, item1, (FAR || item0 > 0) ? 1 : 0, mom,
+                     mrows, partial_ws);
   return check_launch(ctx, "k_gpp_main");
 }
 
