@@ -249,6 +249,32 @@ def adjoint_leg(bgw, torch, dev, stream, ctx, reps=3):
            "gpp_bwd_tflops": bwd_flops / (bk * 1e-3) / 1e12,
            "gpp_bwd_roofline_frac": bwd_flops / (bk * 1e-3) / 1e12 / FP64_PEAK_TFLOPS,
            "dsigma_domega": [repr(complex(z)) for z in o["ds"].cpu().numpy()]}
+    # the moment form of both passes (BGW_FP64_MOMENTS), against the direct results above
+    ref = {k: o[k].cpu().numpy().copy() for k in ("ds", "dw", "dI", "loc")}
+    ctx.set_precision(bgw.BGW_FP64_MOMENTS)
+    fwd()
+    bwd()
+    ctx.synchronize()
+    ctx.reset_timing()
+    ctx.set_timing(True)
+    fm_ms = _timed_calls(torch, stream, fwd, reps)
+    bm_ms = _timed_calls(torch, stream, bwd, reps)
+    ctx.set_timing(False)
+    ctx.set_precision(bgw.BGW_FP64)
+    mk, mn = ctx.kernel_time("gpp_bwd")
+    rk, rn = ctx.kernel_time("gpp_bwd_moments")
+    ctx.synchronize()
+
+    def rdiff(a, b):
+        return float(np.max(np.abs(a - b)) / np.max(np.abs(b)))
+
+    res["moments_form"] = {
+        "forward_all_outputs_ms": fm_ms, "backward_ms": bm_ms, "gpp_bwd_kernel_ms": mk / max(mn, 1),
+        "gpp_bwd_moments_kernel_ms": rk / max(rn, 1), "backward_speedup": b_ms / bm_ms,
+        "max_diff_dL_dwtilde": rdiff(o["dw"].cpu().numpy(), ref["dw"]),
+        "max_diff_dL_dI_eps": rdiff(o["dI"].cpu().numpy(), ref["dI"]),
+        "rel_diff_dsigma_domega": float(np.max(np.abs(o["ds"].cpu().numpy() - ref["ds"]) / np.abs(ref["ds"]))),
+        "note": "max |difference| over the arrays / max |direct value|, against the direct FP64 backward"}
     del T, o
     torch.cuda.empty_cache()
     return res

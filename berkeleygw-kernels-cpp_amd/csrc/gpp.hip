@@ -344,40 +344,6 @@ __device__ inline double rcp_nr(double q) {
   return fma(r, fma(-q, r, 1.0), r);
 }
 
-// Sums of the N doubles v[] over the 64 lanes of a wave, reduce-scatter style: while N
-// is even each exchange halves the values a lane carries (a lane keeps the lower or the
-// upper half by its lane bit and adds its partner's copy of that half), then the rest is
-// summed across the remaining lane bits.  On return v[0..scatter_len(N)) of a lane holds
-// the wave totals of components [base, base + scatter_len(N)); `writer` is set in one lane
-// of each component block.  The order of every addition is fixed, so the sums are bitwise
-// reproducible.
-constexpr int scatter_len(int n, int mask = 32) {
-  return mask == 0 ? n : (n % 2 == 0 ? scatter_len(n / 2, mask / 2) : n);
-}
-
-template <int N, int MASK>
-__device__ __forceinline__ void wave_sum_scatter(double* v, int lane, int& base, bool& writer) {
-  if constexpr (MASK > 0) {
-    if constexpr (N % 2 == 0) {
-      const bool up = (lane & MASK) != 0;
-#pragma unroll
-      for (int j = 0; j < N / 2; ++j) {
-        const double send = up ? v[j] : v[j + N / 2];
-        const double keep = up ? v[j + N / 2] : v[j];
-        v[j] = keep + __shfl_xor(send, MASK, 64);
-      }
-      if (up) base += N / 2;
-      wave_sum_scatter<N / 2, MASK / 2>(v, lane, base, writer);
-    } else {
-      // a + b in one lane and b + a in its partner: the same value (addition commutes)
-#pragma unroll
-      for (int j = 0; j < N; ++j) v[j] += __shfl_xor(v[j], MASK, 64);
-      if (lane & MASK) writer = false;
-      wave_sum_scatter<N, MASK / 2>(v, lane, base, writer);
-    }
-  }
-}
-
 // One wave per work item (p, G-chunk [gb, ge)), four items per workgroup: the chunk's
 // centre from the bounding box of its first kMomSample poles (any centre is valid: P is
 // measured over every pole), its moments and P.  A non-finite pole or weight, or a pole

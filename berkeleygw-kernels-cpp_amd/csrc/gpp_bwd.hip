@@ -14,6 +14,23 @@
 // broadcasts of c(n1,p) and wx(n1,:); the pole terms are recomputed, never stored.
 // Each (p,g) output is owned by one lane, so the G-vector outputs are written
 // directly; dSigma/domega and T(p) go through fixed-order partial reductions.
+//
+// Moment form (BGW_FP64_MOMENTS): the band sums of a pole w are sums over the band
+// frequencies x = wx(n1,iw),
+//   F_iw(w) = sum_n1 c/(x - w),  G = sum_iw F_iw,  H_iw = dF_iw/dw = sum_n1 c/(x - w)^2,
+// the transpose of the forward's pole sum.  With e the middle of the frequency range, c_p
+// a centre of row p's poles and phi(z) = (z - c_p)/(z - e),
+//   1/(x - w) = (c_p - e)/((x - c_p)(w - e)) sum_m (phi(w)/phi(x))^m,
+// so F_iw(w) = (c_p - e) g P_iw(phi(w)), g = 1/(w - e), with the polynomial
+// P_iw(phi) = sum_{m<M} A_iw,m phi^m of the row's target moments
+// A_iw,m = sum_n1 c(n1,p) psi(x)^m / (x - c_p), psi = 1/phi(x) = (x - e)/(x - c_p), and
+// H_iw = (c_p - e) g^2 (-P_iw + (c_p - e) g P_iw').  The truncation error of a pole's
+// F terms is at most rho^M and that of its H terms at most
+// rho^M + M rho^(M-1) (1 + rho)(1 + |phi(w)|) h/D times the sum of their magnitudes
+// (rho = |phi(w)| h/D, h = max |x - e|, D = min |x - c_p|); a pole is "far" when the
+// larger bound is below 2^-54, and a block whose poles are all far takes the expansion
+// (O(M) per pole) instead of the band loop (O(3 nb)).
+#include <cfloat>
 #include <cmath>
 
 #include "bgw_dd.h"
@@ -31,6 +48,112 @@ namespace {
 constexpr int BT = 256;  // threads per block
 constexpr int GPT = 2;   // G-vectors per thread
 constexpr int kMaxBandsLds = 1024;
+
+constexpr int kBwdMom = 16;  // target moments per frequency
+constexpr double kBwdBound = 0x1p-54;
+
+// Row p's centre (the bounding box of its first 256 poles: any centre is valid, each pole's
+// bound is checked), its target moments A[iw][m] and whether they are usable (finite, and
+// the centre away from the frequency range).
+struct __align__(16) BwdRow {
+  double cx, cy;
+  int ok, pad_[3];
+  double2 A[3][kBwdMom];
+};
+
+// One workgroup per G' row: lanes take bands n1 = tid, tid + BT, ...; the moments of each
+// frequency are summed per lane, then over each wave by reduce-scatter and over the waves
+// in wave order (fixed order: bitwise reproducible).
+__global__ __launch_bounds__(BT) void k_gpp_bwd_rowmom(const double2* __restrict__ wt, const double2* __restrict__ C,
+                                                      const double* __restrict__ wxb,
+                                                      const double* __restrict__ wxrange, int nb, int ncouls,
+                                                      BwdRow* __restrict__ rows) {
+  __shared__ double sbox[BT / 64][4];
+  __shared__ double part[BT / 64][2 * kBwdMom];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int p = blockIdx.x;
+  double lr = INFINITY, hr = -INFINITY, li = INFINITY, hi = -INFINITY;
+  if (tid < ncouls) {
+    const double2 w = wt[(size_t)p * ncouls + tid];
+    lr = hr = w.x;
+    li = hi = w.y;
+  }
+#pragma unroll
+  for (int m = 32; m > 0; m >>= 1) {
+    lr = fmin(lr, __shfl_xor(lr, m, 64));
+    hr = fmax(hr, __shfl_xor(hr, m, 64));
+    li = fmin(li, __shfl_xor(li, m, 64));
+    hi = fmax(hi, __shfl_xor(hi, m, 64));
+  }
+  if (lane == 0) {
+    sbox[wave][0] = lr;
+    sbox[wave][1] = hr;
+    sbox[wave][2] = li;
+    sbox[wave][3] = hi;
+  }
+  __syncthreads();
+  lr = sbox[0][0], hr = sbox[0][1], li = sbox[0][2], hi = sbox[0][3];
+#pragma unroll
+  for (int w = 1; w < BT / 64; ++w) {
+    lr = fmin(lr, sbox[w][0]);
+    hr = fmax(hr, sbox[w][1]);
+    li = fmin(li, sbox[w][2]);
+    hi = fmax(hi, sbox[w][3]);
+  }
+  double cx = 0.5 * (lr + hr), cy = 0.5 * (li + hi);
+  if (!(fabs(cx) <= 0x1p+1000 && fabs(cy) <= 0x1p+1000)) cx = cy = 0.0;  // a sample of NaN/Inf poles
+  const double xmin = wxrange[0], xmax = wxrange[1], e = 0.5 * (xmin + xmax);
+  const double dx = cx < xmin ? xmin - cx : (cx > xmax ? cx - xmax : 0.0);
+  const double e0 = xmin - cx, e1 = xmax - cx;
+  // every |x - c|^2 in [2^-150, 2^150] (false for NaN)
+  bool ok = fma(dx, dx, cy * cy) >= 0x1p-150 && fmax(e0 * e0, e1 * e1) + cy * cy <= 0x1p+150;
+  for (int iw = 0; iw < 3; ++iw) {
+    double v[2 * kBwdMom];
+#pragma unroll
+    for (int j = 0; j < 2 * kBwdMom; ++j) v[j] = 0.0;
+    for (int n1 = tid; n1 < nb; n1 += BT) {
+      const double x = wxb[3 * n1 + iw];
+      const double2 cn = C[(size_t)p * nb + n1];
+      const double a = x - cx, r = 1.0 / fma(a, a, cy * cy);
+      const double tr = a * r, ti = cy * r;  // 1/(x - c)
+      const double sr = (x - e) * tr, si = (x - e) * ti;  // psi
+      double qr = fma(cn.x, tr, -cn.y * ti), qi = fma(cn.x, ti, cn.y * tr);  // c/(x - c)
+#pragma unroll
+      for (int m = 0; m < kBwdMom; ++m) {
+        v[2 * m] += qr;
+        v[2 * m + 1] += qi;
+        if (m + 1 < kBwdMom) {
+          const double t = fma(qr, sr, -qi * si);
+          qi = fma(qr, si, qi * sr);
+          qr = t;
+        }
+      }
+    }
+    int base = 0;
+    bool writer = true;
+    wave_sum_scatter<2 * kBwdMom, 32>(v, lane, base, writer);
+    constexpr int NF = scatter_len(2 * kBwdMom);
+    if (writer) {
+#pragma unroll
+      for (int j = 0; j < NF; ++j) part[wave][base + j] = v[j];
+    }
+    __syncthreads();
+    if (tid < 2 * kBwdMom) {
+      double t = part[0][tid];
+#pragma unroll
+      for (int w = 1; w < BT / 64; ++w) t += part[w][tid];
+      ok = ok && fabs(t) <= DBL_MAX;
+      reinterpret_cast<double*>(rows[p].A[iw])[tid] = t;
+    }
+    __syncthreads();
+  }
+  const int all_ok = __syncthreads_and(ok ? 1 : 0);
+  if (tid == 0) {
+    rows[p].cx = cx;
+    rows[p].cy = cy;
+    rows[p].ok = all_ok;
+  }
+}
 
 // holomorphic derivatives of gpp_ssx with respect to I, wtilde and wx (branch-wise)
 __device__ inline void ssx_grad(double wx, double2 wt, double2 I, double2& dI, double2& dW, double2& dX) {
@@ -65,6 +188,7 @@ __device__ inline void ssx_grad(double wx, double2 wt, double2 I, double2& dI, d
 
 // three workgroups per CU (168 VGPRs, three waves per SIMD: 2-3% faster than two at 194
 // VGPRs; the few spill slots are in the screened-exchange loop over the valence bands)
+template <bool FAR>
 __global__ __launch_bounds__(BT, 3) void k_gpp_bwd(const double2* __restrict__ wt, const double2* __restrict__ Ieps,
                                                 const double2* __restrict__ C, const double* __restrict__ wxb,
                                                 const double* __restrict__ wxrange, const int* __restrict__ inv,
@@ -73,10 +197,12 @@ __global__ __launch_bounds__(BT, 3) void k_gpp_bwd(const double2* __restrict__ w
                                                 int nb, int nvband, int ngpown, int ncouls, double occ,
                                                 double2* __restrict__ dL_dwt, double2* __restrict__ dL_dI,
                                                 double4* __restrict__ ds_part /*[blocks][3]*/,
-                                                double4* __restrict__ T_part /*[blocks]*/) {
+                                                double4* __restrict__ T_part /*[blocks]*/,
+                                                const BwdRow* __restrict__ rows) {
   __shared__ double2 sc[kMaxBandsLds];
   __shared__ double sx[kMaxBandsLds * 3];
   __shared__ cdd red[BT / 64];
+  __shared__ double2 sA[FAR ? 3 * kBwdMom : 1];
   const int nchunk = (ncouls + BT * GPT - 1) / (BT * GPT);
   const int p = blockIdx.x / nchunk;
   const int ch = blockIdx.x - p * nchunk;
@@ -104,7 +230,67 @@ __global__ __launch_bounds__(BT, 3) void k_gpp_bwd(const double2* __restrict__ w
     for (int iw = 0; iw < 3; ++iw) H[k][iw][0] = H[k][iw][1] = 0.0;
   }
 
-  for (int b0 = 0; b0 < nb; b0 += kMaxBandsLds) {
+  bool block_far = false;
+  if constexpr (FAR) {
+    // every valid pole of the block far from the band frequencies about the row's centre
+    const BwdRow* __restrict__ rw = rows + p;
+    const double cx = rw->cx, cy = rw->cy;
+    if (threadIdx.x < 3 * kBwdMom) sA[threadIdx.x] = rw->A[threadIdx.x / kBwdMom][threadIdx.x % kBwdMom];
+    const double xmin = wxrange[0], xmax = wxrange[1], e = 0.5 * (xmin + xmax);
+    const double h = fmax(xmax - e, e - xmin) * (1.0 + 0x1p-40);
+    const double dx = cx < xmin ? xmin - cx : (cx > xmax ? cx - xmax : 0.0);
+    const double hD = h / (sqrt(fma(dx, dx, cy * cy)) * (1.0 - 0x1p-40));  // h/D
+    double gr[GPT], gi[GPT], fr[GPT], fi[GPT];
+    bool far = rw->ok != 0;
+#pragma unroll
+    for (int k = 0; k < GPT; ++k) {
+      const double dr = w[k].x - e, di = w[k].y;
+      const double q = 1.0 / fma(dr, dr, di * di);
+      gr[k] = dr * q;
+      gi[k] = -di * q;  // 1/(w - e)
+      const double zr = w[k].x - cx, zi = w[k].y - cy;
+      fr[k] = fma(zr, gr[k], -zi * gi[k]);
+      fi[k] = fma(zr, gi[k], zi * gr[k]);  // phi(w)
+      const double af = sqrt(fma(fr[k], fr[k], fi[k] * fi[k])) * (1.0 + 0x1p-40);
+      const double rho = af * hD;
+      const double r2 = rho * rho, r4 = r2 * r2, r8 = r4 * r4, r15 = r8 * r4 * r2 * rho;
+      const double bound = fma(r15, rho, kBwdMom * r15 * (1.0 + rho) * (1.0 + af) * hD);
+      far = far && (!valid[k] || bound <= kBwdBound);  // false for NaN
+    }
+    block_far = __syncthreads_and(far ? 1 : 0) != 0;
+    if (block_far) {
+      // G = sum_iw F_iw, F_iw = (c - e) g P_iw(phi), H_iw = (c - e) g^2 (-P_iw + (c - e) g P_iw')
+      const double cer = cx - e, cei = cy;
+#pragma unroll
+      for (int k = 0; k < GPT; ++k) {
+        const double cgr = fma(cer, gr[k], -cei * gi[k]), cgi = fma(cer, gi[k], cei * gr[k]);
+#pragma unroll
+        for (int iw = 0; iw < 3; ++iw) {
+          const double2* __restrict__ A = sA + iw * kBwdMom;
+          double pr = A[kBwdMom - 1].x, pi = A[kBwdMom - 1].y, dr = 0.0, di = 0.0;
+#pragma unroll 3
+          for (int m = kBwdMom - 2; m >= 0; --m) {
+            const double ndr = fma(dr, fr[k], fma(-di, fi[k], pr));
+            di = fma(dr, fi[k], fma(di, fr[k], pi));
+            dr = ndr;
+            const double2 a = A[m];
+            const double npr = fma(pr, fr[k], fma(-pi, fi[k], a.x));
+            pi = fma(pr, fi[k], fma(pi, fr[k], a.y));
+            pr = npr;
+          }
+          // F = cg P;  H = cg g (cg P' - P)
+          G[k][0] += fma(cgr, pr, -cgi * pi);
+          G[k][1] += fma(cgr, pi, cgi * pr);
+          const double ur = fma(cgr, dr, -cgi * di) - pr, ui = fma(cgr, di, cgi * dr) - pi;
+          const double vr = fma(cgr, gr[k], -cgi * gi[k]), vi = fma(cgr, gi[k], cgi * gr[k]);
+          H[k][iw][0] = fma(vr, ur, -vi * ui);
+          H[k][iw][1] = fma(vr, ui, vi * ur);
+        }
+      }
+    }
+  }
+
+  for (int b0 = 0; b0 < (block_far ? 0 : nb); b0 += kMaxBandsLds) {
     const int nbt = min(kMaxBandsLds, nb - b0);
     __syncthreads();
     for (int i = threadIdx.x; i < nbt; i += BT) {
@@ -256,11 +442,25 @@ int gpp_backward_device(bgw_ctx* ctx, size_t nb_, size_t nvband_, size_t ngpown_
   void *pds, *pT;
   if ((rc = pool_get(ctx, "ws_bwd_ds", blocks * 3 * 32, &pds))) return rc;
   if ((rc = pool_get(ctx, "ws_bwd_T", blocks * 32, &pT))) return rc;
-  {
+  if (ctx->precision == BGW_FP64_MOMENTS) {
+    void* prow;
+    if ((rc = pool_get(ctx, "ws_bwd_rows", (size_t)ngpown * sizeof(BwdRow), &prow))) return rc;
+    BwdRow* rows = static_cast<BwdRow*>(prow);
+    {
+      KernelTimer kt(ctx, "gpp_bwd_moments");
+      hipLaunchKernelGGL(k_gpp_bwd_rowmom, dim3((unsigned)ngpown), dim3(BT), 0, s, wt, C, wxb, range, nb, ncouls,
+                         rows);
+    }
     KernelTimer kt(ctx, "gpp_bwd");
-    hipLaunchKernelGGL(k_gpp_bwd, dim3((unsigned)blocks), dim3(BT), 0, s, wt, I, C, wxb, range, inv, indinv, am, an,
-                       vcoul, nb, nvband, ngpown, ncouls, occ, reinterpret_cast<double2*>(dwt),
-                       reinterpret_cast<double2*>(dI), static_cast<double4*>(pds), static_cast<double4*>(pT));
+    hipLaunchKernelGGL(k_gpp_bwd<true>, dim3((unsigned)blocks), dim3(BT), 0, s, wt, I, C, wxb, range, inv, indinv, am,
+                       an, vcoul, nb, nvband, ngpown, ncouls, occ, reinterpret_cast<double2*>(dwt),
+                       reinterpret_cast<double2*>(dI), static_cast<double4*>(pds), static_cast<double4*>(pT), rows);
+  } else {
+    KernelTimer kt(ctx, "gpp_bwd");
+    hipLaunchKernelGGL(k_gpp_bwd<false>, dim3((unsigned)blocks), dim3(BT), 0, s, wt, I, C, wxb, range, inv, indinv,
+                       am, an, vcoul, nb, nvband, ngpown, ncouls, occ, reinterpret_cast<double2*>(dwt),
+                       reinterpret_cast<double2*>(dI), static_cast<double4*>(pds), static_cast<double4*>(pT),
+                       nullptr);
   }
   if ((rc = check_launch(ctx, "k_gpp_bwd"))) return rc;
   if (dwt) {

@@ -206,3 +206,72 @@ def test_config1_full_vs_direct_and_oracle(ctxm):
                wx=d["wx"][:36].copy())
     o = O.gpp_forward(sub, want=("achtemp", "acht_n1_loc"))
     assert rel_err(r["acht_n1_loc"][:12], o["acht_n1_loc"]) < TOL
+
+
+# ------------------------------------------------------------------ adjoint in the moment form
+def _bwd_direct(ctx, d, occ):
+    ctx.set_precision(bgw.BGW_FP64)
+    try:
+        return ctx.gpp_backward(d, occ=occ)
+    finally:
+        ctx.set_precision(bgw.BGW_FP64_MOMENTS)
+
+
+def _grad_close(g, dw, dI, ds, tol=1e-12):
+    assert np.max(np.abs(g["dL_dwtilde"] - dw)) <= tol * np.max(np.abs(dw))
+    assert np.max(np.abs(g["dL_dI_eps"] - dI)) <= tol * np.max(np.abs(dI))
+    assert rel_err(g["dsigma_domega"], ds) < tol
+
+
+@pytest.mark.parametrize("nb,ncouls,ngpown", [(70, 300, 20), (1100, 600, 4), (33, 1500, 7)])
+def test_adjoint_vs_oracle(ctxm, nb, ncouls, ngpown):
+    """The band sums of every pole from the row's 16 target moments (1100 bands: more than
+    the direct loop's LDS chunk; 1500 G-vectors: three blocks per row, the last one short)."""
+    d = bgw.gpp_inputs(nb, 3, ncouls, ngpown, 555 + nb)
+    g = ctxm.gpp_backward(d, occ=0.8)
+    e = np.arange(ngpown * ncouls, dtype=np.int64)
+    dw, dI = O.gpp_grad_entries(d, e, occ=0.8)
+    _grad_close(g, dw, dI, O.gpp_dsigma_domega(d, occ=0.8))
+    f = _bwd_direct(ctxm, d, 0.8)
+    assert not np.array_equal(g["dL_dI_eps"], f["dL_dI_eps"])  # the moments were used
+
+
+def test_adjoint_near_poles_fall_back_bitwise(ctxm):
+    # a pole near a band frequency in every block: the direct band loop everywhere
+    d = _near_poles(bgw.gpp_inputs(64, 2, 2000, 6, 97), 499)
+    g = ctxm.gpp_backward(d, occ=1.0)
+    f = _bwd_direct(ctxm, d, 1.0)
+    for k in g:
+        assert np.array_equal(g[k], f[k]), k
+
+
+def test_adjoint_mixed_blocks_vs_oracle(ctxm):
+    # near poles only in the first 512-G block of each row
+    d = bgw.gpp_inputs(48, 2, 1800, 5, 98)
+    wx = d["wx"].reshape(48, 3)
+    wt = d["wtilde"].reshape(5, 1800)
+    wt[:, 300] = wx[9, 1] - 1e-3 - 2e-4j
+    d["wtilde"] = wt.reshape(-1)
+    g = ctxm.gpp_backward(d, occ=1.0)
+    e = np.arange(5 * 1800, dtype=np.int64)
+    dw, dI = O.gpp_grad_entries(d, e, occ=1.0)
+    _grad_close(g, dw, dI, O.gpp_dsigma_domega(d, occ=1.0))
+
+
+def test_adjoint_bitwise_reproducible(ctxm):
+    d = bgw.gpp_inputs(40, 2, 700, 9, 99)
+    a = ctxm.gpp_backward(d)
+    b = ctxm.gpp_backward(d)
+    for k in a:
+        assert np.array_equal(a[k], b[k])
+
+
+@pytest.mark.slow
+def test_adjoint_config3_vs_direct(ctxm):
+    """The adjoint workload at full size (256 bands, ngpown 819, ncouls 16384): the moment
+    form against the direct FP64 backward (itself checked against the oracle's analytic
+    gradient and finite differences) on the same inputs."""
+    d = bgw.gpp_config_inputs(3)
+    g = ctxm.gpp_backward(d, occ=1.0)
+    f = _bwd_direct(ctxm, d, 1.0)
+    _grad_close(g, f["dL_dwtilde"], f["dL_dI_eps"], f["dsigma_domega"])
