@@ -41,7 +41,8 @@ struct bgw_ctx {
   hipStream_t stream = nullptr;
   int precision = BGW_FP64;
   std::string err;
-  int* d_flag = nullptr;  // device word: bad-index report of *_device calls
+  int* d_flag = nullptr;  // device words: [0] bad-index report of *_device calls,
+                          // [1] the GPP moment form's "some chunk takes the direct sum"
   // named grow-only pools: workspace and host-API staging of inputs/outputs
   std::map<std::string, bgw::DevBuf> pool;
   // timing

@@ -355,8 +355,10 @@ __global__ __launch_bounds__(64 * kMomWaves) void k_gpp_moments(const double2* _
                                                                 const double2* __restrict__ Ieps,
                                                                 const double* __restrict__ wxrange, int ncouls,
                                                                 int chunk, int nchunks, long item0, long item1,
-                                                                MomChunk* __restrict__ tab) {
+                                                                MomChunk* __restrict__ tab, int* __restrict__ any_direct,
+                                                                int reset) {
   const int lane = threadIdx.x & 63;
+  if (reset && blockIdx.x == 0 && threadIdx.x == 0) *any_direct = 0;  // before the row passes of this call
   const long item = item0 + (long)blockIdx.x * kMomWaves + (threadIdx.x >> 6);
   if (item >= item1) return;  // the whole wave: the kernel has no workgroup barrier
   const int p = (int)(item / nchunks);
@@ -462,7 +464,8 @@ constexpr int kRowWaves = 4;
 __global__ __launch_bounds__(64 * kRowWaves) void k_gpp_moments_row(int nchunks, int p0, int W,
                                                                    const double* __restrict__ wxrange,
                                                                    MomChunk* __restrict__ tab,
-                                                                   MomRow* __restrict__ rows) {
+                                                                   MomRow* __restrict__ rows,
+                                                                   int* __restrict__ any_direct) {
   __shared__ double2 part[kRowWaves][kMom];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int p = p0 + blockIdx.x;
@@ -483,6 +486,7 @@ __global__ __launch_bounds__(64 * kRowWaves) void k_gpp_moments_row(int nchunks,
   }
   if (d < 0) {  // no far chunk: every chunk takes the direct sum
     for (int c = threadIdx.x; c < nchunks; c += 64 * kRowWaves) ch[c].mode = kChunkDirect;
+    if (threadIdx.x == 0) *any_direct = 1;
     return;
   }
   const double pcx = ch[d].cx, pcy = ch[d].cy;
@@ -535,7 +539,10 @@ __global__ __launch_bounds__(64 * kRowWaves) void k_gpp_moments_row(int nchunks,
         }
       }
     }
-    if (lane == 0) ch[c].mode = mode;
+    if (lane == 0) {
+      ch[c].mode = mode;
+      if (mode == kChunkDirect) *any_direct = 1;
+    }
   }
   if (lane < kMom) part[wave][lane] = make_double2(sr, si);
   __syncthreads();
@@ -614,7 +621,8 @@ __device__ inline void row_far(const double2* __restrict__ nu, double cx, double
 // instructions of two waves of a SIMD issue together and the pair rate grows with the
 // number of waves.  FAR: the direct pass of the moment form (FP64), launched after
 // k_gpp_far_eval over the same items and grid and continuing its totals: it sums the
-// items of kChunkDirect chunks exactly as without FAR and skips the others.
+// items of kChunkDirect chunks exactly as without FAR and skips the others (all of them
+// when the row passes found no such chunk).
 template <int NT, int BPL, bool F32, bool FAR>
 __global__ __launch_bounds__(NT, F32 ? 8 : 4) void k_gpp_main(const double2* __restrict__ wt,
                                                  const double2* __restrict__ Ieps,
@@ -624,10 +632,14 @@ __global__ __launch_bounds__(NT, F32 ? 8 : 4) void k_gpp_main(const double2* __r
                                                  int ngpown, int ncouls, int chunk, int nchunks,
                                                  long item0, long item1, int carry,
                                                  const MomChunk* __restrict__ mom,
-                                                 const MomRow* __restrict__ mrows,
+                                                 const int* __restrict__ any_direct,
                                                  double2* __restrict__ partial) {
   static_assert(!F32 || BPL == 1, "the FP32-bulk form runs one band per lane");
   static_assert(!(F32 && FAR), "the moment form is FP64");
+  // no chunk of the call takes the direct sum: the far pass's totals stand as they are
+  if constexpr (FAR) {
+    if (*any_direct == 0) return;
+  }
   __shared__ TileEntry tile[NT];
   __shared__ TileEntry32 tile32[F32 ? NT : 1];
   const int tid = threadIdx.x;
@@ -835,27 +847,31 @@ __global__ __launch_bounds__(NT, 4) void k_gpp_far_eval(const double2* __restric
 }
 
 // ------------------------------------------------------------------ reductions
-// acht_n1_loc[n1][iw] = sum_w partial[w][n1][iw] (double-double).  A block owns 16
-// consecutive (n1,iw) entries; its 16 row groups each sum a strided set of w,
-// then the groups are added in fixed order, so the result is bitwise stable.
-__global__ __launch_bounds__(256) void k_reduce_bands(const double2* __restrict__ partial, int W, int nb,
-                                                      double2* __restrict__ acht_n1_loc,
-                                                      double4* __restrict__ band_dd) {
-  __shared__ cdd sh[16][16];
-  const int e = threadIdx.x & 15, grp = threadIdx.x >> 4;
-  const int k = blockIdx.x * 16 + e;
+// acht_n1_loc[n1][iw] = sum_w partial[w][n1][iw] (double-double).  A block owns 4
+// consecutive (n1,iw) entries; its 64 row groups each sum a strided set of w, then a
+// fixed tree over the groups adds them, so the result is bitwise stable.
+constexpr int kRedEntries = 4, kRedGroups = 64;
+
+__global__ __launch_bounds__(kRedEntries * kRedGroups) void k_reduce_bands(const double2* __restrict__ partial, int W,
+                                                                         int nb, double2* __restrict__ acht_n1_loc,
+                                                                         double4* __restrict__ band_dd) {
+  __shared__ cdd sh[kRedGroups][kRedEntries];
+  const int e = threadIdx.x % kRedEntries, grp = threadIdx.x / kRedEntries;
+  const int k = blockIdx.x * kRedEntries + e;
   const int n = nb * 3;
   cdd a = cdd_zero();
   if (k < n)
-    for (int w = grp; w < W; w += 16) {
+    for (int w = grp; w < W; w += kRedGroups) {
       const double2 v = partial[(size_t)w * n + k];
       a = cdd_add(a, v.x, v.y);
     }
   sh[grp][e] = a;
-  __syncthreads();
+  for (int h = kRedGroups / 2; h > 0; h >>= 1) {
+    __syncthreads();
+    if (grp < h) sh[grp][e] = cdd_add(sh[grp][e], sh[grp + h][e]);
+  }
   if (grp == 0 && k < n) {
-    cdd t = sh[0][e];
-    for (int q = 1; q < 16; ++q) t = cdd_add(t, sh[q][e]);
+    const cdd t = sh[0][e];
     if (acht_n1_loc) acht_n1_loc[k] = make_double2(t.re.hi + t.re.lo, t.im.hi + t.im.lo);
     cdd_store(&band_dd[k], t);
   }
@@ -961,10 +977,10 @@ int launch_main(bgw_ctx* ctx, const double2* wt, const double2* I, const double2
         const long n = (long)(r1 - r0) * nchunks;
         hipLaunchKernelGGL(k_gpp_moments, dim3((unsigned)((n + kMomWaves - 1) / kMomWaves)), dim3(64 * kMomWaves), 0,
                            ctx->stream, wt, I, range, ncouls, chunk, nchunks, (long)r0 * nchunks, (long)r1 * nchunks,
-                           mom);
+                           mom, ctx->d_flag + 1, r0 == 0 ? 1 : 0);
       }
       hipLaunchKernelGGL(k_gpp_moments_row, dim3((unsigned)(r1 - r0)), dim3(64 * kRowWaves), 0, ctx->stream, nchunks,
-                         r0, (int)W, range, mom, mrows);
+                         r0, (int)W, range, mom, mrows, ctx->d_flag + 1);
       if ((rc = check_launch(ctx, "k_gpp_moments"))) return rc;
     }
   }
@@ -981,7 +997,7 @@ int launch_main(bgw_ctx* ctx, const double2* wt, const double2* I, const double2
   KernelTimer kt(ctx, FAR ? "gpp_far_direct" : "gpp_main");
   hipLaunchKernelGGL((k_gpp_main<NT, BPL, F32, FAR>), dim3((unsigned)W, btiles), dim3(NT), 0, ctx->stream, wt, I, S,
                      wxb, range, nb, ngpown, ncouls, chunk, nchunks, item0, item1, (FAR || item0 > 0) ? 1 : 0, mom,
-                     mrows, partial_ws);
+                     ctx->d_flag + 1, partial_ws);
   return check_launch(ctx, "k_gpp_main");
 }
 
@@ -1119,8 +1135,8 @@ int fwd_end(bgw_ctx* ctx, const Fwd& f, int nb, int nvband, int ngpown, int ncou
             const double* vcoul, const double* wxb, double occ, double* achtemp, double* acht_n1_loc,
             double* achstemp, double* asxtemp) {
   hipStream_t s = ctx->stream;
-  hipLaunchKernelGGL(k_reduce_bands, dim3((nb * 3 + 15) / 16), dim3(256), 0, s, f.part, f.W, nb,
-                     reinterpret_cast<double2*>(acht_n1_loc), f.band);
+  hipLaunchKernelGGL(k_reduce_bands, dim3((nb * 3 + kRedEntries - 1) / kRedEntries), dim3(kRedEntries * kRedGroups), 0,
+                     s, f.part, f.W, nb, reinterpret_cast<double2*>(acht_n1_loc), f.band);
   hipLaunchKernelGGL(k_reduce_achtemp, dim3(1), dim3(256), 0, s, f.band, nb, reinterpret_cast<double2*>(achtemp));
   int rc;
   if ((rc = check_launch(ctx, "k_reduce"))) return rc;
