@@ -563,13 +563,14 @@ __global__ __launch_bounds__(64 * kRowWaves) void k_gpp_moments_row(int nchunks,
 
 // Far-field value of a set of moments about centre c (a row's summed far chunks, or one
 // chunk) for a lane's 3*BPL frequencies: acc[b][iw] = T(x[b][iw]).  nu: the moments (LDS).
-// One band at a time (registers); the three reciprocals 1/(x - c) of a band come from one
-// batched inversion, as in the direct form.
+// The three reciprocals 1/(x - c) of a band come from one batched inversion, as in the
+// direct form; the bands' Horner chains are unrolled together (33 against 40 us for the
+// far pass at the benchmark workload, with a few spill slots).
 template <int BPL>
 __device__ inline void row_far(const double2* __restrict__ nu, double cx, double cy, double e,
                                const double (&x)[BPL][3], double (&acc)[BPL][3][2]) {
   const double cer = cx - e, cei = cy;
-#pragma unroll 1
+#pragma unroll
   for (int bb = 0; bb < BPL; ++bb) {
     double ur[3], ui[3], sr[3], si[3];  // (c - e) t and psi = (x - e) t, t = 1/(x - c)
     {
