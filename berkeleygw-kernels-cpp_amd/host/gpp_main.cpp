@@ -6,7 +6,7 @@
 // selects the seeded benchmark distributions.  The one-sided golden check of the
 // reference (GPP/gpp.cpp:117-131) is replaced by a two-sided check against the
 // exact closed form of the constant inputs.  Extra flags: --random SEED,
-// --precision fp64|fp32, --all (also prints acht_n1_loc sum, achstemp, asxtemp),
+// --precision fp64|fp32|moments, --all (also prints acht_n1_loc sum, achstemp, asxtemp),
 // --dump FILE (writes the generated inputs as an input set, see bgw.h) and
 // --input FILE (reads an input set instead of generating inputs; no size arguments).
 #include <chrono>
@@ -31,8 +31,8 @@ using clk = std::chrono::steady_clock;
 
 static void usage() {
   std::printf("The correct form of input is :\n ./gpp.ex <number_bands> <number_valence_bands> "
-              "<number_plane_waves> <nodes_per_mpi_group> [--random SEED] [--precision fp64|fp32] [--all] "
-              "[--dump FILE]\n ./gpp.ex --input FILE [--precision fp64|fp32] [--all]\n");
+              "<number_plane_waves> <nodes_per_mpi_group> [--random SEED] [--precision fp64|fp32|moments] "
+              "[--all] [--dump FILE]\n ./gpp.ex --input FILE [--precision fp64|fp32|moments] [--all]\n");
 }
 
 static void print_extras(int number_bands, int nvband, int ngpown, int ncouls, const int* inv, const int* indinv,
@@ -104,11 +104,11 @@ static int run(int argc, char** argv) {
       }
     } else if (a == "--precision" && i + 1 < argc) {
       const std::string p = argv[++i];
-      if (p != "fp32" && p != "fp64") {
+      if (p != "fp32" && p != "fp64" && p != "moments") {
         usage();
         return 1;
       }
-      precision = p == "fp32" ? BGW_FP32_BULK : BGW_FP64;
+      precision = p == "fp32" ? BGW_FP32_BULK : (p == "moments" ? BGW_FP64_MOMENTS : BGW_FP64);
     } else if (a == "--all") {
       all = true;
     } else if (a == "-h" || a == "--help") {
@@ -241,7 +241,7 @@ static int run(int argc, char** argv) {
     // exact value for the constant inputs: nb*ngpown*ncouls * 0.125i / (wx - 0.5 - 0.5i)
     const cplx cf = (double)number_bands * ngpown * ncouls * cplx(0.0, 0.125) / (wx3(0) - cplx(0.5, 0.5));
     const double rel = std::abs(achtemp(0) - cf) / std::abs(cf);
-    const double tol = precision == BGW_FP64 ? 1e-12 : 1e-6;
+    const double tol = precision == BGW_FP32_BULK ? 1e-6 : 1e-12;
     if (rel < tol)
       std::printf("\n!!!! SUCCESS - !!!! Correctness test passed :-D :-D  (rel err %.3e vs exact)\n\n", rel);
     else

@@ -32,7 +32,7 @@ def _pair(out, label):
     return complex(float(m.group(1)), float(m.group(2)))
 
 
-@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+@pytest.mark.parametrize("precision", ["fp64", "fp32", "moments"])
 def test_gpp_driver_reference_constants(precision):
     rc, out = _run("gpp.ex", 64, 2, 4096, 8, "--precision", precision)
     assert rc == 0
@@ -42,9 +42,10 @@ def test_gpp_driver_reference_constants(precision):
     assert abs(_pair(out, "Final achtemp\n") - cf) <= 1e-6 * abs(cf)
 
 
-def test_gpp_driver_random_inputs_vs_oracle():
+@pytest.mark.parametrize("form", [[], ["--precision", "moments"]])
+def test_gpp_driver_random_inputs_vs_oracle(form):
     nb, nvband, ncouls, nodes = 24, 2, 800, 100  # ngpown = 8
-    rc, out = _run("gpp.ex", nb, nvband, ncouls, nodes, "--random", 4321, "--all")
+    rc, out = _run("gpp.ex", nb, nvband, ncouls, nodes, "--random", 4321, "--all", *form)
     assert rc == 0
     d = bgw.gpp_inputs(nb, nvband, ncouls, ncouls // nodes, 4321)
     o = O.gpp_forward(d)
