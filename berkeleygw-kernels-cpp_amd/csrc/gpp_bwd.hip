@@ -158,7 +158,9 @@ __global__ __launch_bounds__(BT) void k_gpp_bwd_rowmom(const double2* __restrict
 // holomorphic derivatives of gpp_ssx with respect to I, wtilde and wx (branch-wise)
 __device__ inline void ssx_grad(double wx, double2 wt, double2 I, double2& dI, double2& dW, double2& dX) {
   dI = dW = dX = make_double2(0.0, 0.0);
-  if (gpp_ssx_cut(wx, gpp_ssx_raw(wx, wt, I), I)) return;  // cut terms are constant zero
+  // cut terms are constant zero; the cut applies only for wx < 0 (tested first, so the
+  // value it needs is formed only then)
+  if (wx < 0.0 && gpp_ssx_cut(wx, gpp_ssx_raw(wx, wt, I), I)) return;
   const double2 wt2 = cmul(wt, wt);
   const double2 wdiff = make_double2(wx - wt.x, -wt.y);
   const double wdiffr = cnorm2(wdiff);
