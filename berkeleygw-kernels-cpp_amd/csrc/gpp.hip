@@ -304,11 +304,18 @@ constexpr int kChunkDirect = 0;  // the direct sum
 constexpr int kChunkFar = 1;     // re-expanded into its row's moments, evaluated with the row
 constexpr int kChunkEval = 2;    // far, and evaluates its row's moments
 constexpr int kChunkSelf = 3;    // far, but evaluated from its own moments (outside the row's bound)
+constexpr int kModeMask = 7;
+constexpr int kNearFlag = 8;     // mode bit: the chunk's near poles go through the direct pass
+constexpr int kNearCap = 128;    // near poles listed per chunk; more: the whole chunk takes the direct sum
 
+// Moments are summed over a chunk's far poles, |phi(w)| <= kFarRho D/h about the chunk's
+// centre; the others ("near": close to a band frequency, or non-finite) are listed, up to
+// kNearCap of them, and summed directly.
 struct __align__(16) MomChunk {
   double cx, cy;  // centre c
-  double P;       // bound of |phi(w)| over the chunk's poles; +inf: not usable
-  int mode, pad_;
+  double P;       // bound of |phi(w)| over the chunk's far poles; +inf: not usable
+  int mode;       // set by the row pass
+  int nnear;      // listed near poles
   double2 nu[kMom];
 };
 struct __align__(16) MomRow {
@@ -344,19 +351,38 @@ __device__ inline double rcp_nr(double q) {
   return fma(r, fma(-q, r, 1.0), r);
 }
 
-// One wave per work item (p, G-chunk [gb, ge)), four items per workgroup: the chunk's
-// centre from the bounding box of its first kMomSample poles (any centre is valid: P is
-// measured over every pole), its moments and P.  A non-finite pole or weight, or a pole
-// at e, makes a moment non-finite (NaN and Inf survive every sum and product), which
-// marks the chunk as not usable.
+// One wave per work item (p, G-chunk [gb, ge)), four items per workgroup.  The chunk's
+// centre is the middle of the bounding box of sample poles (its first kMomSample) that lie
+// more than 2h from e, on the side of e that holds more of them (all sample poles when none
+// does): a few poles near the band frequencies then neither move the centre nor, being
+// listed as near, spoil the moments of the rest.  A non-finite weight of a far pole makes a
+// moment non-finite (NaN and Inf survive every sum and product), which marks the chunk as
+// not usable.
 constexpr int kMomWaves = 4;
+
+__device__ inline void box_add(double (&b)[4], double2 w) {
+  b[0] = fmin(b[0], w.x);
+  b[1] = fmax(b[1], w.x);
+  b[2] = fmin(b[2], w.y);
+  b[3] = fmax(b[3], w.y);
+}
+__device__ inline void box_wave(double (&b)[4]) {
+#pragma unroll
+  for (int m = 32; m > 0; m >>= 1) {
+    b[0] = fmin(b[0], __shfl_xor(b[0], m, 64));
+    b[1] = fmax(b[1], __shfl_xor(b[1], m, 64));
+    b[2] = fmin(b[2], __shfl_xor(b[2], m, 64));
+    b[3] = fmax(b[3], __shfl_xor(b[3], m, 64));
+  }
+}
 
 __global__ __launch_bounds__(64 * kMomWaves) void k_gpp_moments(const double2* __restrict__ wt,
                                                                 const double2* __restrict__ Ieps,
                                                                 const double* __restrict__ wxrange, int ncouls,
                                                                 int chunk, int nchunks, long item0, long item1,
-                                                                MomChunk* __restrict__ tab, int* __restrict__ any_direct,
-                                                                int reset) {
+                                                                MomChunk* __restrict__ tab,
+                                                                unsigned short* __restrict__ near_idx,
+                                                                int* __restrict__ any_direct, int reset) {
   const int lane = threadIdx.x & 63;
   if (reset && blockIdx.x == 0 && threadIdx.x == 0) *any_direct = 0;  // before the row passes of this call
   const long item = item0 + (long)blockIdx.x * kMomWaves + (threadIdx.x >> 6);
@@ -366,39 +392,66 @@ __global__ __launch_bounds__(64 * kMomWaves) void k_gpp_moments(const double2* _
   const int ge = min(ncouls, gb + chunk);
   const double2* __restrict__ wrow = wt + (size_t)p * ncouls;
   const double2* __restrict__ irow = Ieps + (size_t)p * ncouls;
-  const double e = 0.5 * (wxrange[0] + wxrange[1]);
+  const FarRange fr = far_range(wxrange);
+  const double e = fr.e;
 
-  double lr = INFINITY, hr = -INFINITY, li = INFINITY, hi = -INFINITY;
+  double ball[4] = {INFINITY, -INFINITY, INFINITY, -INFINITY};
+  double bl[4] = {INFINITY, -INFINITY, INFINITY, -INFINITY}, br[4] = {INFINITY, -INFINITY, INFINITY, -INFINITY};
+  int nl = 0, nr = 0;
+  const double clear2 = 4.0 * fr.h * fr.h;
 #pragma unroll
   for (int k = 0; k < kMomSample / 64; ++k) {
     const int g = gb + lane + 64 * k;
     if (g < ge) {
       const double2 w = wrow[g];
-      lr = fmin(lr, w.x);
-      hr = fmax(hr, w.x);
-      li = fmin(li, w.y);
-      hi = fmax(hi, w.y);
+      box_add(ball, w);
+      const double dr = w.x - e;
+      if (fma(dr, dr, w.y * w.y) > clear2) {
+        if (w.x < e) {
+          box_add(bl, w);
+          ++nl;
+        } else {
+          box_add(br, w);
+          ++nr;
+        }
+      }
     }
   }
+  box_wave(ball);
+  box_wave(bl);
+  box_wave(br);
 #pragma unroll
   for (int m = 32; m > 0; m >>= 1) {
-    lr = fmin(lr, __shfl_xor(lr, m, 64));
-    hr = fmax(hr, __shfl_xor(hr, m, 64));
-    li = fmin(li, __shfl_xor(li, m, 64));
-    hi = fmax(hi, __shfl_xor(hi, m, 64));
+    nl += __shfl_xor(nl, m, 64);
+    nr += __shfl_xor(nr, m, 64);
   }
-  double cx = 0.5 * (lr + hr), cy = 0.5 * (li + hi);
+  const double* b = nl + nr == 0 ? ball : (nl >= nr ? bl : br);
+  double cx = 0.5 * (b[0] + b[1]), cy = 0.5 * (b[2] + b[3]);
   if (!(fabs(cx) <= 0x1p+1000 && fabs(cy) <= 0x1p+1000)) cx = cy = 0.0;  // a sample of NaN/Inf poles
+  // far poles: |phi| <= kFarRho D/h, a 2^-30 margin inside the row pass's test of the same
+  // centre (far_ok), so the bound P formed below passes it; 0 when no pole can be far
+  double t2 = 0.0;
+  {
+    const double dx = cx < fr.xmin ? fr.xmin - cx : (cx > fr.xmax ? cx - fr.xmax : 0.0);
+    const double D = sqrt(fma(dx, dx, cy * cy)) * (1.0 - 0x1p-40);
+    const double t = kFarRho * D / fr.h * (1.0 - 0x1p-30);
+    if (t > 0.0) t2 = t * t;  // false for NaN; +inf for a single band frequency (h = 0)
+  }
 
   double v[2 * kMom];
 #pragma unroll
   for (int j = 0; j < 2 * kMom; ++j) v[j] = 0.0;
-  double f2 = 0.0;  // max |phi|^2
-  // one pole ahead: the loads of the next pole are in flight while this one is summed
-  // (the last index is re-read, in bounds, past the end)
-  int g = gb + lane;
-  double2 wn = wrow[min(g, ge - 1)], In = irow[min(g, ge - 1)];
-  for (; g < ge; g += 64) {
+  double f2 = 0.0;  // max |phi|^2 over the far poles
+  int nn = 0;       // near poles so far (every lane holds the same count)
+  unsigned short* __restrict__ list = near_idx + (size_t)item * kNearCap;
+  // rounds of 64 poles, the same number in every lane (lanes past the end carry a zero
+  // weight); one round ahead: the next round's loads are in flight while this one is summed
+  // (indices past the end re-read the last pole, in bounds)
+  const int rounds = (ge - gb + 63) / 64;
+  double2 wn = wrow[min(gb + lane, ge - 1)], In = irow[min(gb + lane, ge - 1)];
+  for (int k = 0; k < rounds; ++k) {
+    const int g = gb + lane + 64 * k;
+    const bool valid = g < ge;
     const double2 w = wn, I = In;
     const int gn = min(g + 64, ge - 1);
     wn = wrow[gn];
@@ -410,16 +463,26 @@ __global__ __launch_bounds__(64 * kMomWaves) void k_gpp_moments(const double2* _
     const double r = rcp_nr(fma(dr, dr, di * di));
     const double gr = dr * r, gi = -di * r;              // 1/(w - e)
     const double zr = w.x - cx, zi = w.y - cy;           // w - c
-    const double fr = fma(zr, gr, -zi * gi), fi = fma(zr, gi, zi * gr);  // phi
-    f2 = fmax(f2, fma(fr, fr, fi * fi));
-    double pr = fma(I.x, gr, -I.y * gi), pi = fma(I.x, gi, I.y * gr);    // I/(w - e)
+    const double fr_ = fma(zr, gr, -zi * gi), fi = fma(zr, gi, zi * gr);  // phi
+    const double q = fma(fr_, fr_, fi * fi);
+    const bool far = q <= t2;  // false for NaN
+    const bool near = valid && !far;
+    f2 = (valid && far) ? fmax(f2, q) : f2;
+    const double wgt = (valid && far) ? 1.0 : 0.0;
+    double pr = wgt * fma(I.x, gr, -I.y * gi), pi = wgt * fma(I.x, gi, I.y * gr);  // I/(w - e)
+    const unsigned long long bal = __ballot(near);
+    if (near) {
+      const int pos = nn + __builtin_popcountll(bal & ((1ull << lane) - 1));
+      if (pos < kNearCap) list[pos] = (unsigned short)(g - gb);
+    }
+    nn += __builtin_popcountll(bal);
 #pragma unroll
     for (int m = 0; m < kMom; ++m) {
       v[2 * m] += pr;
       v[2 * m + 1] += pi;
       if (m + 1 < kMom) {
-        const double t = fma(pr, fr, -pi * fi);
-        pi = fma(pr, fi, pi * fr);
+        const double t = fma(pr, fr_, -pi * fi);
+        pi = fma(pr, fi, pi * fr_);
         pr = t;
       }
     }
@@ -433,7 +496,7 @@ __global__ __launch_bounds__(64 * kMomWaves) void k_gpp_moments(const double2* _
   for (int j = 0; j < NF; ++j) fin &= fabs(v[j]) <= DBL_MAX;
 #pragma unroll
   for (int m = 32; m > 0; m >>= 1) f2 = fmax(f2, __shfl_xor(f2, m, 64));
-  const bool ok = __all(fin);
+  const bool ok = __all(fin) && nn <= kNearCap;
   MomChunk* __restrict__ me = tab + item;
   double* __restrict__ out = reinterpret_cast<double*>(me->nu);
   if (writer) {
@@ -443,8 +506,10 @@ __global__ __launch_bounds__(64 * kMomWaves) void k_gpp_moments(const double2* _
   if (lane == 0) {
     me->cx = cx;
     me->cy = cy;
-    // sqrt(f2) is within a few tens of ulp of max |phi|: the factor makes P a bound
+    // sqrt(f2) is within a few tens of ulp of max |phi| over the far poles: the factor
+    // makes P a bound
     me->P = ok ? sqrt(f2) * (1.0 + 0x1p-40) : INFINITY;
+    me->nnear = ok ? nn : 0;
   }
 }
 
@@ -540,8 +605,10 @@ __global__ __launch_bounds__(64 * kRowWaves) void k_gpp_moments_row(int nchunks,
       }
     }
     if (lane == 0) {
+      // a far chunk's near poles, if any, go through the direct pass
+      if (mode != kChunkDirect && ch[c].nnear > 0) mode |= kNearFlag;
       ch[c].mode = mode;
-      if (mode == kChunkDirect) *any_direct = 1;
+      if (mode == kChunkDirect || (mode & kNearFlag)) *any_direct = 1;
     }
   }
   if (lane < kMom) part[wave][lane] = make_double2(sr, si);
@@ -633,6 +700,7 @@ __global__ __launch_bounds__(NT, F32 ? 8 : 4) void k_gpp_main(const double2* __r
                                                  int ngpown, int ncouls, int chunk, int nchunks,
                                                  long item0, long item1, int carry,
                                                  const MomChunk* __restrict__ mom,
+                                                 const unsigned short* __restrict__ near_idx,
                                                  const int* __restrict__ any_direct,
                                                  double2* __restrict__ partial) {
   static_assert(!F32 || BPL == 1, "the FP32-bulk form runs one band per lane");
@@ -690,7 +758,8 @@ __global__ __launch_bounds__(NT, F32 ? 8 : 4) void k_gpp_main(const double2* __r
         modes = it < nitems ? mom[it].mode : kChunkDirect;
       }
       mode = __builtin_amdgcn_readlane(modes, k);  // uniform: the item's
-      if (mode != kChunkDirect) continue;  // summed by k_gpp_far_eval
+      // far poles are summed by k_gpp_far_eval; a far chunk's near poles, listed, here
+      if ((mode & kModeMask) != kChunkDirect && !(mode & kNearFlag)) continue;
     }
     // sch_store1(n1,p) of this lane's bands (zero for idle lanes)
     double2 sv[BPL];
@@ -704,13 +773,26 @@ __global__ __launch_bounds__(NT, F32 ? 8 : 4) void k_gpp_main(const double2* __r
 #pragma unroll
       for (int iw = 0; iw < 3; ++iw) acc[bb][iw][0] = acc[bb][iw][1] = 0.0;
 
-    for (int g0 = gb; g0 < ge; g0 += NT) {
-      const int g = g0 + tid;
+    // the chunk's G-vectors gb + j, j < cnt, or (FAR, a far chunk) its listed near poles
+    int cnt = ge - gb;
+    const unsigned short* __restrict__ list = nullptr;
+    if constexpr (FAR) {
+      if ((mode & kModeMask) != kChunkDirect) {
+        list = near_idx + (size_t)item * kNearCap;
+        cnt = mom[item].nnear;
+      }
+    }
+    for (int t0 = 0; t0 < cnt; t0 += NT) {
+      const int j = t0 + tid;
+      int g = j < cnt ? gb + j : -1;
+      if constexpr (FAR) {
+        if (list && j < cnt) g = gb + list[j];
+      }
       bool ok;
       TileEntry e;
-      const double2 wv = g < ge ? wt[rowoff + g] : make_double2(0.0, 1.0);
-      const double2 Iv = g < ge ? Ieps[rowoff + g] : make_double2(0.0, 0.0);
-      // padding entries (g >= ge) contribute exactly zero, denominators wx^2+1
+      const double2 wv = g >= 0 ? wt[rowoff + g] : make_double2(0.0, 1.0);
+      const double2 Iv = g >= 0 ? Ieps[rowoff + g] : make_double2(0.0, 0.0);
+      // padding entries (g < 0) contribute exactly zero, denominators wx^2+1
       e = make_entry(wv, Iv, wxmin, wxmax, ok);
       __syncthreads();  // previous tile fully consumed
       tile[tid] = e;
@@ -720,7 +802,7 @@ __global__ __launch_bounds__(NT, F32 ? 8 : 4) void k_gpp_main(const double2* __r
         tile32[tid] = entry32(e);
       }
       const int unsafe = __syncthreads_or(!ok);
-      const int n = min(NT, ge - g0);
+      const int n = min(NT, cnt - t0);
       if constexpr (F32) {
         // FP32-bulk form: FP32 tiles far from every pole, FP64 (IEEE division) otherwise
         if (!unsafe && !__syncthreads_or(!ok32)) {
@@ -813,7 +895,7 @@ __global__ __launch_bounds__(NT, 4) void k_gpp_far_eval(const double2* __restric
       const long it = item + (long)(tid & 63) * gridDim.x;
       modes = it < item1 ? mom[it].mode : kChunkDirect;
     }
-    const int mode = __builtin_amdgcn_readlane(modes, k);  // uniform: the item's
+    const int mode = __builtin_amdgcn_readlane(modes, k) & kModeMask;  // uniform: the item's
     if (mode != kChunkEval && mode != kChunkSelf) continue;
     const int p = (int)(item / nchunks);
     const bool row = mode == kChunkEval;
@@ -960,16 +1042,19 @@ int launch_main(bgw_ctx* ctx, const double2* wt, const double2* I, const double2
   if (item1 < 0) item1 = items;
   MomChunk* mom = nullptr;
   MomRow* mrows = nullptr;
+  unsigned short* near = nullptr;
   if constexpr (FAR) {
     // Moments of every chunk of the rows that items [item0, item1) reach and that an
     // earlier launch of this call has not covered (whole rows: a row's pass needs all of
     // its chunks), the row pass over them, then the evaluation of the items.
-    void *pm, *pr;
+    void *pm, *pr, *pn;
     int rc = pool_get(ctx, "ws_moments", (size_t)items * sizeof(MomChunk), &pm);
     if (!rc) rc = pool_get(ctx, "ws_moment_rows", (size_t)ngpown * sizeof(MomRow), &pr);
+    if (!rc) rc = pool_get(ctx, "ws_near_idx", (size_t)items * kNearCap * sizeof(unsigned short), &pn);
     if (rc) return rc;
     mom = static_cast<MomChunk*>(pm);
     mrows = static_cast<MomRow*>(pr);
+    near = static_cast<unsigned short*>(pn);
     const int r0 = item0 > 0 ? (int)((item0 - 1) / nchunks) + 1 : 0;
     const int r1 = (int)((item1 - 1) / nchunks) + 1;
     if (r1 > r0) {
@@ -978,7 +1063,7 @@ int launch_main(bgw_ctx* ctx, const double2* wt, const double2* I, const double2
         const long n = (long)(r1 - r0) * nchunks;
         hipLaunchKernelGGL(k_gpp_moments, dim3((unsigned)((n + kMomWaves - 1) / kMomWaves)), dim3(64 * kMomWaves), 0,
                            ctx->stream, wt, I, range, ncouls, chunk, nchunks, (long)r0 * nchunks, (long)r1 * nchunks,
-                           mom, ctx->d_flag + 1, r0 == 0 ? 1 : 0);
+                           mom, near, ctx->d_flag + 1, r0 == 0 ? 1 : 0);
       }
       hipLaunchKernelGGL(k_gpp_moments_row, dim3((unsigned)(r1 - r0)), dim3(64 * kRowWaves), 0, ctx->stream, nchunks,
                          r0, (int)W, range, mom, mrows, ctx->d_flag + 1);
@@ -998,7 +1083,7 @@ int launch_main(bgw_ctx* ctx, const double2* wt, const double2* I, const double2
   KernelTimer kt(ctx, FAR ? "gpp_far_direct" : "gpp_main");
   hipLaunchKernelGGL((k_gpp_main<NT, BPL, F32, FAR>), dim3((unsigned)W, btiles), dim3(NT), 0, ctx->stream, wt, I, S,
                      wxb, range, nb, ngpown, ncouls, chunk, nchunks, item0, item1, (FAR || item0 > 0) ? 1 : 0, mom,
-                     ctx->d_flag + 1, partial_ws);
+                     near, ctx->d_flag + 1, partial_ws);
   return check_launch(ctx, "k_gpp_main");
 }
 
@@ -1067,8 +1152,9 @@ int run_main(bgw_ctx* ctx, MainCfg cfg, const double2* wt, const double2* I, con
 // Bytes of the moments table of a forward call (the moment form only).
 size_t moments_bytes(bgw_ctx* ctx, int ngpown, int ncouls) {
   if (ctx->precision != BGW_FP64_MOMENTS) return 0;
-  return (((size_t)ngpown * ((ncouls + kGChunk - 1) / kGChunk) * sizeof(MomChunk) + 255) & ~size_t(255)) +
-         (size_t)ngpown * sizeof(MomRow);
+  const size_t items = (size_t)ngpown * ((ncouls + kGChunk - 1) / kGChunk);
+  return ((items * sizeof(MomChunk) + 255) & ~size_t(255)) + (((size_t)ngpown * sizeof(MomRow) + 255) & ~size_t(255)) +
+         items * kNearCap * sizeof(unsigned short);
 }
 
 }  // namespace

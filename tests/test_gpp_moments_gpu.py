@@ -1,10 +1,10 @@
-"""Moment form of the GPP forward (bgw_set_precision(BGW_FP64_MOMENTS)): each work item's
-G-sum sum_g I_eps(p,g)/(wx - wtilde(p,g)) is evaluated from 28 moments of its poles
-about the chunk's centre when the poles lie within 0.26 x the distance from the centre to
-every band frequency (the far chunks of a row re-expanded about one centre and evaluated
-together), and by the direct FP64 sum otherwise.  The truncation is below 2^-54
-of the sum of the terms' magnitudes, so the FP64 tolerance applies: 1e-12 relative error
-per complex output against the oracle."""
+"""Moment form of the GPP forward and adjoint (bgw_set_precision(BGW_FP64_MOMENTS)).  Each
+work item's G-sum sum_g I_eps(p,g)/(wx - wtilde(p,g)) is evaluated from 16 moments of its
+far poles, those with |phi(w)| h/D <= 0.095 about the chunk's centre; the far chunks of a
+row are re-expanded about one centre and evaluated together.  A chunk's near poles, up to
+128, are listed and summed directly, and a chunk with more takes the direct FP64 sum as a
+whole.  The truncation is below 2^-54 of the sum of the terms' magnitudes, so the FP64
+tolerance applies: 1e-12 relative error per complex output against the oracle."""
 import numpy as np
 import pytest
 
@@ -92,15 +92,30 @@ def test_exact_fixtures(ctxm, case):
 
 
 def test_every_chunk_near_falls_back_bitwise(ctxm):
-    """A pole near a band frequency in every chunk: every item takes the direct sum, in
-    the same launch shape and order as the FP64 form, so the results are equal bit for bit."""
-    d = _near_poles(bgw.gpp_inputs(300, 2, 5000, 12, 91), 997)
+    """Every 7th pole near a band frequency: more near poles in every chunk than its list
+    holds (128), so every item takes the direct sum, in the same launch shape and order as
+    the FP64 form, and the results are equal bit for bit."""
+    d = _near_poles(bgw.gpp_inputs(300, 2, 5000, 12, 91), 7)
     r = ctxm.gpp_solver(d, outputs=("achtemp", "acht_n1_loc"))
     f = _direct(ctxm, d, outputs=("achtemp", "acht_n1_loc"))
     assert np.array_equal(r["achtemp"], f["achtemp"])
     assert np.array_equal(r["acht_n1_loc"], f["acht_n1_loc"])
     o = O.gpp_forward(d, want=("achtemp", "acht_n1_loc"))
     assert rel_err(r["acht_n1_loc"], o["acht_n1_loc"]) < TOL
+
+
+@pytest.mark.parametrize("every", [997, 61])
+def test_sparse_near_poles_listed_vs_oracle(ctxm, every):
+    """A few poles near a band frequency in every chunk (2 or 34 of 2048): they are listed
+    and summed directly, the rest of the chunk from its moments; the sample's centre
+    ignores them."""
+    d = _near_poles(bgw.gpp_inputs(96, 2, 5000, 12, 90 + every), every)
+    r = ctxm.gpp_solver(d, outputs=("achtemp", "acht_n1_loc"))
+    o = O.gpp_forward(d, want=("achtemp", "acht_n1_loc"))
+    assert rel_err(r["achtemp"], o["achtemp"]) < TOL
+    assert rel_err(r["acht_n1_loc"], o["acht_n1_loc"]) < TOL
+    f = _direct(ctxm, d, outputs=("achtemp", "acht_n1_loc"))
+    assert not np.array_equal(r["acht_n1_loc"], f["acht_n1_loc"])  # the moments were used
 
 
 def test_mixed_near_and_far_chunks_vs_oracle(ctxm):
