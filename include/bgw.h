@@ -53,11 +53,12 @@ extern "C" {
                              FP64 pole denominators and FP64 compensated reductions   */
 #define BGW_FP64_MOMENTS 2  /* GPP achtemp / acht_n1_loc sum: the pole sum
                              sum_g I_eps(p,g)/(wx - wtilde(p,g)) of each 2048-G chunk from
-                             16 moments of the Moebius variable (w - c)/(w - e), e the middle
-                             of the band-frequency range, FP64, used for a chunk when
-                             rho = max|phi(w)| h/D <= 0.095 (h the range's half-width, D the
-                             distance from c to it): truncation below 2^-54 x the sum of the
-                             terms' magnitudes; other chunks take the direct FP64 sum.  The
+                             16 moments of the Moebius variable phi = (w - c)/(w - e), e the
+                             middle of the band-frequency range, FP64, over the poles with
+                             |phi| h/D <= 0.095 (h the range's half-width, D the distance from
+                             c to it): truncation below 2^-54 x the sum of the terms'
+                             magnitudes; up to 128 other poles per chunk summed directly, a
+                             chunk with more by the direct FP64 sum as a whole.  The
                              adjoint's band sums likewise, from 16 moments over a row's band
                              frequencies per pole (blocks with a pole near a band frequency
                              take the band loop).  Every other kernel runs as in BGW_FP64. */
