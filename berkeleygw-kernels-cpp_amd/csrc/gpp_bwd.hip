@@ -28,7 +28,7 @@
 // F terms is at most rho^M and that of its H terms at most
 // rho^M + M rho^(M-1) (1 + rho)(1 + |phi(w)|) h/D times the sum of their magnitudes
 // (rho = |phi(w)| h/D, h = max |x - e|, D = min |x - c_p|); a pole is "far" when the
-// larger bound is below 2^-54, and a block whose poles are all far takes the expansion
+// larger bound is below 2^-54, and a wave whose poles are all far takes the expansion
 // (O(M) per pole) instead of the band loop (O(3 nb)).
 #include <cfloat>
 #include <cmath>
@@ -52,9 +52,9 @@ constexpr int kMaxBandsLds = 1024;
 constexpr int kBwdMom = 16;  // target moments per frequency
 constexpr double kBwdBound = 0x1p-54;
 
-// Row p's centre (the bounding box of its first 256 poles: any centre is valid, each pole's
-// bound is checked), its target moments A[iw][m] and whether they are usable (finite, and
-// the centre away from the frequency range).
+// Row p's centre (from the bounding box of sample poles among its first 256: any centre is
+// valid, each pole's bound is checked), its target moments A[iw][m] and whether they are
+// usable (finite, and the centre away from the frequency range).
 struct __align__(16) BwdRow {
   double cx, cy;
   int ok, pad_[3];
@@ -68,41 +68,68 @@ __global__ __launch_bounds__(BT) void k_gpp_bwd_rowmom(const double2* __restrict
                                                       const double* __restrict__ wxb,
                                                       const double* __restrict__ wxrange, int nb, int ncouls,
                                                       BwdRow* __restrict__ rows) {
-  __shared__ double sbox[BT / 64][4];
+  __shared__ double sbox[BT / 64][3][5];
   __shared__ double part[BT / 64][2 * kBwdMom];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int p = blockIdx.x;
-  double lr = INFINITY, hr = -INFINITY, li = INFINITY, hi = -INFINITY;
+  // the bounding box of the sample poles more than 2h from e, on the side of e that holds
+  // more of them (all sample poles when none is): a few poles near the band frequencies do
+  // not move the centre
+  const double xmin = wxrange[0], xmax = wxrange[1], e = 0.5 * (xmin + xmax);
+  const double h = fmax(xmax - e, e - xmin);
+  // boxes: [0] all, [1] left of e, [2] right of e; (min re, max re, min im, max im, count)
+  double bx[3][4];
+  double cnt[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+  for (int q = 0; q < 3; ++q) bx[q][0] = bx[q][2] = INFINITY, bx[q][1] = bx[q][3] = -INFINITY;
   if (tid < ncouls) {
     const double2 w = wt[(size_t)p * ncouls + tid];
-    lr = hr = w.x;
-    li = hi = w.y;
+    const double dr = w.x - e;
+    const int side = fma(dr, dr, w.y * w.y) > 4.0 * h * h ? (w.x < e ? 1 : 2) : 0;
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      if (q == 0 || q == side) {
+        bx[q][0] = bx[q][1] = w.x;
+        bx[q][2] = bx[q][3] = w.y;
+        cnt[q] = 1.0;
+      }
+    }
   }
 #pragma unroll
-  for (int m = 32; m > 0; m >>= 1) {
-    lr = fmin(lr, __shfl_xor(lr, m, 64));
-    hr = fmax(hr, __shfl_xor(hr, m, 64));
-    li = fmin(li, __shfl_xor(li, m, 64));
-    hi = fmax(hi, __shfl_xor(hi, m, 64));
-  }
-  if (lane == 0) {
-    sbox[wave][0] = lr;
-    sbox[wave][1] = hr;
-    sbox[wave][2] = li;
-    sbox[wave][3] = hi;
+  for (int q = 0; q < 3; ++q) {
+#pragma unroll
+    for (int m = 32; m > 0; m >>= 1) {
+      bx[q][0] = fmin(bx[q][0], __shfl_xor(bx[q][0], m, 64));
+      bx[q][1] = fmax(bx[q][1], __shfl_xor(bx[q][1], m, 64));
+      bx[q][2] = fmin(bx[q][2], __shfl_xor(bx[q][2], m, 64));
+      bx[q][3] = fmax(bx[q][3], __shfl_xor(bx[q][3], m, 64));
+      cnt[q] += __shfl_xor(cnt[q], m, 64);
+    }
+    if (lane == 0) {
+      sbox[wave][q][0] = bx[q][0];
+      sbox[wave][q][1] = bx[q][1];
+      sbox[wave][q][2] = bx[q][2];
+      sbox[wave][q][3] = bx[q][3];
+      sbox[wave][q][4] = cnt[q];
+    }
   }
   __syncthreads();
-  lr = sbox[0][0], hr = sbox[0][1], li = sbox[0][2], hi = sbox[0][3];
 #pragma unroll
-  for (int w = 1; w < BT / 64; ++w) {
-    lr = fmin(lr, sbox[w][0]);
-    hr = fmax(hr, sbox[w][1]);
-    li = fmin(li, sbox[w][2]);
-    hi = fmax(hi, sbox[w][3]);
+  for (int q = 0; q < 3; ++q) {
+    bx[q][0] = sbox[0][q][0], bx[q][1] = sbox[0][q][1], bx[q][2] = sbox[0][q][2], bx[q][3] = sbox[0][q][3];
+    cnt[q] = sbox[0][q][4];
+#pragma unroll
+    for (int w = 1; w < BT / 64; ++w) {
+      bx[q][0] = fmin(bx[q][0], sbox[w][q][0]);
+      bx[q][1] = fmax(bx[q][1], sbox[w][q][1]);
+      bx[q][2] = fmin(bx[q][2], sbox[w][q][2]);
+      bx[q][3] = fmax(bx[q][3], sbox[w][q][3]);
+      cnt[q] += sbox[w][q][4];
+    }
   }
-  double cx = 0.5 * (lr + hr), cy = 0.5 * (li + hi);
+  const int pick = cnt[1] + cnt[2] == 0.0 ? 0 : (cnt[1] >= cnt[2] ? 1 : 2);
+  double cx = 0.5 * (bx[pick][0] + bx[pick][1]), cy = 0.5 * (bx[pick][2] + bx[pick][3]);
   if (!(fabs(cx) <= 0x1p+1000 && fabs(cy) <= 0x1p+1000)) cx = cy = 0.0;  // a sample of NaN/Inf poles
-  const double xmin = wxrange[0], xmax = wxrange[1], e = 0.5 * (xmin + xmax);
   const double dx = cx < xmin ? xmin - cx : (cx > xmax ? cx - xmax : 0.0);
   const double e0 = xmin - cx, e1 = xmax - cx;
   // every |x - c|^2 in [2^-150, 2^150] (false for NaN)
@@ -232,9 +259,11 @@ __global__ __launch_bounds__(BT, 3) void k_gpp_bwd(const double2* __restrict__ w
     for (int iw = 0; iw < 3; ++iw) H[k][iw][0] = H[k][iw][1] = 0.0;
   }
 
-  bool block_far = false;
+  // FAR: a wave whose poles are all far takes the expansion; the band loop runs (its LDS
+  // staging for the whole block) when any wave of the block needs it
+  bool wave_far = false, need_loop = true;
   if constexpr (FAR) {
-    // every valid pole of the block far from the band frequencies about the row's centre
+    // the wave's valid poles far from the band frequencies about the row's centre
     const BwdRow* __restrict__ rw = rows + p;
     const double cx = rw->cx, cy = rw->cy;
     if (threadIdx.x < 3 * kBwdMom) sA[threadIdx.x] = rw->A[threadIdx.x / kBwdMom][threadIdx.x % kBwdMom];
@@ -259,8 +288,9 @@ __global__ __launch_bounds__(BT, 3) void k_gpp_bwd(const double2* __restrict__ w
       const double bound = fma(r15, rho, kBwdMom * r15 * (1.0 + rho) * (1.0 + af) * hD);
       far = far && (!valid[k] || bound <= kBwdBound);  // false for NaN
     }
-    block_far = __syncthreads_and(far ? 1 : 0) != 0;
-    if (block_far) {
+    wave_far = __all(far) != 0;
+    need_loop = __syncthreads_or(wave_far ? 0 : 1) != 0;  // also orders the sA writes before their reads
+    if (wave_far) {
       // G = sum_iw F_iw, F_iw = (c - e) g P_iw(phi), H_iw = (c - e) g^2 (-P_iw + (c - e) g P_iw')
       const double cer = cx - e, cei = cy;
 #pragma unroll
@@ -292,7 +322,7 @@ __global__ __launch_bounds__(BT, 3) void k_gpp_bwd(const double2* __restrict__ w
     }
   }
 
-  for (int b0 = 0; b0 < (block_far ? 0 : nb); b0 += kMaxBandsLds) {
+  for (int b0 = 0; b0 < (need_loop ? nb : 0); b0 += kMaxBandsLds) {
     const int nbt = min(kMaxBandsLds, nb - b0);
     __syncthreads();
     for (int i = threadIdx.x; i < nbt; i += BT) {
@@ -302,7 +332,7 @@ __global__ __launch_bounds__(BT, 3) void k_gpp_bwd(const double2* __restrict__ w
       sx[3 * i + 2] = wxb[3 * (b0 + i) + 2];
     }
     __syncthreads();
-    for (int j = 0; j < nbt; ++j) {
+    for (int j = 0; j < (wave_far ? 0 : nbt); ++j) {
       const double2 c = sc[j];
       const double x0 = sx[3 * j], x1 = sx[3 * j + 1], x2 = sx[3 * j + 2];
 #pragma unroll

@@ -252,16 +252,28 @@ def test_adjoint_vs_oracle(ctxm, nb, ncouls, ngpown):
 
 
 def test_adjoint_near_poles_fall_back_bitwise(ctxm):
-    # a pole near a band frequency in every block: the direct band loop everywhere
-    d = _near_poles(bgw.gpp_inputs(64, 2, 2000, 6, 97), 499)
+    # a pole near a band frequency in every wave of 64 poles: the band loop everywhere
+    d = _near_poles(bgw.gpp_inputs(64, 2, 2000, 6, 97), 61)
     g = ctxm.gpp_backward(d, occ=1.0)
     f = _bwd_direct(ctxm, d, 1.0)
     for k in g:
         assert np.array_equal(g[k], f[k]), k
 
 
+def test_adjoint_sparse_near_poles_vs_oracle(ctxm):
+    # one pole in 499 near a band frequency: its wave takes the band loop, the others the
+    # expansion
+    d = _near_poles(bgw.gpp_inputs(64, 2, 2000, 6, 96), 499)
+    g = ctxm.gpp_backward(d, occ=1.0)
+    e = np.arange(6 * 2000, dtype=np.int64)
+    dw, dI = O.gpp_grad_entries(d, e, occ=1.0)
+    _grad_close(g, dw, dI, O.gpp_dsigma_domega(d, occ=1.0))
+    f = _bwd_direct(ctxm, d, 1.0)
+    assert not np.array_equal(g["dL_dI_eps"], f["dL_dI_eps"])
+
+
 def test_adjoint_mixed_blocks_vs_oracle(ctxm):
-    # near poles only in the first 512-G block of each row
+    # a near pole in one wave of the first 512-G block of each row
     d = bgw.gpp_inputs(48, 2, 1800, 5, 98)
     wx = d["wx"].reshape(48, 3)
     wt = d["wtilde"].reshape(5, 1800)
