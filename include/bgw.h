@@ -60,8 +60,9 @@ extern "C" {
                              magnitudes; up to 128 other poles per chunk summed directly, a
                              chunk with more by the direct FP64 sum as a whole.  The
                              adjoint's band sums likewise, from 16 moments over a row's band
-                             frequencies per pole (blocks with a pole near a band frequency
-                             take the band loop).  Every other kernel runs as in BGW_FP64. */
+                             frequencies per pole (waves of 64 poles with a pole near a band
+                             frequency take the band loop).  Every other kernel runs as in
+                             BGW_FP64.                                                    */
 
 typedef struct bgw_ctx bgw_ctx;
 
