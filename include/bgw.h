@@ -140,7 +140,7 @@ int bgw_gpp_solver_device(bgw_ctx* ctx, size_t number_bands, size_t nvband, size
  * that forbid hipMalloc.  It calls no allocating routine: all workspace comes from a
  * 64 MiB device array of the library (one per GPU, for the life of the process), and a
  * problem whose workspace does not fit returns BGW_EINVAL without launching anything (the
- * benchmark workload, 512 bands x 1638 G', needs 37-49 MiB, depending on the precision).  Every call on a GPU
+ * benchmark workload, 512 bands x 1638 G', needs 37-51 MiB, depending on the arithmetic form).  Every call on a GPU
  * shares that GPU's array, so calls that overlap in time on one GPU (two streams, or
  * two contexts on one device) race; calls queued on one stream do not.  Results equal
  * bgw_gpp_solver_device's bit for bit.  Bad indices are clamped and reported by the
