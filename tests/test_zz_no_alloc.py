@@ -68,3 +68,23 @@ def test_noalloc_rejects_what_does_not_fit():
             ctx.gpp_solver_device_noalloc(8, 4, 16, *([None] * 8), None, None)
         assert e.value.code == bgw.BGW_EINVAL
         ctx.synchronize()
+
+
+@pytest.mark.slow
+@pytest.mark.parametrize("precision", [bgw.BGW_FP64, bgw.BGW_FP32_BULK, bgw.BGW_FP64_MOMENTS])
+def test_noalloc_benchmark_workload(precision):
+    """The benchmark workload (512 bands, ngpown 1638, ncouls 32768) fits the 64 MiB scratch
+    array in every arithmetic form (37-51 MiB) and matches the device entry point bit for bit."""
+    torch = pytest.importorskip("torch")
+    d = bgw.gpp_config_inputs(1)
+    with bgw.Context(0) as ctx:
+        ctx.set_precision(precision)
+        T = _device_inputs(torch, d)
+        out = _outputs(torch, d)
+        ref = _run(torch, ctx, d, T, out, noalloc=False)
+        out[0].fill_(np.nan)
+        out[1].fill_(np.nan)
+        torch.cuda.synchronize()
+        got = _run(torch, ctx, d, T, out, noalloc=True)
+        assert np.array_equal(got[0], ref[0])
+        assert np.array_equal(got[1], ref[1])
