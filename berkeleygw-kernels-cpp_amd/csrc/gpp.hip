@@ -14,6 +14,13 @@
 // Partial sums stay in registers per lane and per work item, then go to
 // per-workgroup partials [W][nb][3]; a fixed-order double-double pass produces
 // acht_n1_loc and achtemp (run-to-run bitwise reproducible).
+//
+// Moment form (BGW_FP64_MOMENTS): the G-sum of a work item is a sum of poles
+// I/(wx - wtilde), so k_gpp_moments reduces each chunk's far poles to 16 moments of a
+// Moebius expansion, k_gpp_moments_row re-expands a row's far chunks about one centre,
+// k_gpp_far_eval evaluates them at every band frequency on the main kernel's grid, and
+// k_gpp_main<..., FAR> sums the rest directly (whole chunks, or a chunk's listed near poles)
+// into the same per-workgroup partials.
 #include <algorithm>
 #include <cfloat>
 #include <cmath>
