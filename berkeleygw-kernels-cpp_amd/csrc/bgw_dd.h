@@ -102,4 +102,39 @@ __device__ inline cdd cdd_block_reduce(cdd v, cdd* sh) {
   return r;
 }
 
+// Wave sums of N double-doubles v[] (N a power of two, at most 64), reduce-scatter style:
+// each of the first log2(N) exchanges halves the values a lane carries (the lane keeps the
+// lower or upper half by its lane bit and adds its partner's copy of that half), the
+// remaining lane bits sum the one value left.  On return v[0] of a lane is the wave total
+// of component `comp`; `writer` is set in one lane per component.  That takes
+// N - 1 + 6 - log2(N) additions per lane, against 6 N for N butterflies; every addition has
+// a fixed operand order (lower lane first), so the sums are bitwise reproducible.
+template <int N, int MASK = 32>
+__device__ __forceinline__ void dd_wave_sum_scatter(dd* v, int lane, int& comp, bool& writer) {
+  static_assert(N >= 1 && (N & (N - 1)) == 0 && (MASK == 0 ? N == 1 : N <= 2 * MASK), "N: a power of two up to 64");
+  if constexpr (MASK > 0) {
+    const bool up = (lane & MASK) != 0;
+    if constexpr (N > 1) {
+#pragma unroll
+      for (int j = 0; j < N / 2; ++j) {
+        const dd send = up ? v[j] : v[j + N / 2];
+        const dd keep = up ? v[j + N / 2] : v[j];
+        dd o;
+        o.hi = __shfl_xor(send.hi, MASK, 64);
+        o.lo = __shfl_xor(send.lo, MASK, 64);
+        v[j] = up ? dd_add(o, keep) : dd_add(keep, o);
+      }
+      if (up) comp += N / 2;
+      dd_wave_sum_scatter<N / 2, MASK / 2>(v, lane, comp, writer);
+    } else {
+      dd o;
+      o.hi = __shfl_xor(v[0].hi, MASK, 64);
+      o.lo = __shfl_xor(v[0].lo, MASK, 64);
+      v[0] = up ? dd_add(o, v[0]) : dd_add(v[0], o);
+      if (up) writer = false;
+      dd_wave_sum_scatter<1, MASK / 2>(v, lane, comp, writer);
+    }
+  }
+}
+
 }  // namespace bgw

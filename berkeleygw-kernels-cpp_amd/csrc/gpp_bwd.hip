@@ -20,16 +20,18 @@
 //   F_iw(w) = sum_n1 c/(x - w),  G = sum_iw F_iw,  H_iw = dF_iw/dw = sum_n1 c/(x - w)^2,
 // the transpose of the forward's pole sum.  With e the middle of the frequency range, c_p
 // a centre of row p's poles and phi(z) = (z - c_p)/(z - e),
-//   1/(x - w) = (c_p - e)/((x - c_p)(w - e)) sum_m (phi(w)/phi(x))^m,
-// so F_iw(w) = (c_p - e) g P_iw(phi(w)), g = 1/(w - e), with the polynomial
-// P_iw(phi) = sum_{m<M} A_iw,m phi^m of the row's target moments
-// A_iw,m = sum_n1 c(n1,p) psi(x)^m / (x - c_p), psi = 1/phi(x) = (x - e)/(x - c_p), and
-// H_iw = (c_p - e) g^2 (-P_iw + (c_p - e) g P_iw').  The truncation error of a pole's
-// F terms is at most rho^M and that of its H terms at most
-// rho^M + M rho^(M-1) (1 + rho)(1 + |phi(w)|) h/D times the sum of their magnitudes
-// (rho = |phi(w)| h/D, h = max |x - e|, D = min |x - c_p|); a pole is "far" when the
-// larger bound is below 2^-54, and a wave whose poles are all far takes the expansion
-// (O(M) per pole) instead of the band loop (O(3 nb)).
+//   1/(x - w) = (c_p - e)/((x - c_p)(w - e)) sum_m (phi(w)/phi(x))^m
+// and, squared, 1/(x - w)^2 = ((c_p - e)/((x - c_p)(w - e)))^2 sum_m (m + 1) (phi(w)/phi(x))^m.
+// So with g = 1/(w - e) and psi = 1/phi(x) = (x - e)/(x - c_p),
+//   G(w) = (c_p - e) g P_A(phi(w)),       A_m = sum_iw sum_n1 c(n1,p) psi^m / (x - c_p),
+//   H_iw(w) = (c_p - e)^2 g^2 P_Q_iw(phi), Q_iw,m = (m + 1) sum_n1 c(n1,p) psi^m / (x - c_p)^2,
+// with P_A, P_Q_iw the polynomials of the row's target moments.  The tail of a term beyond
+// m = M - 1 is exactly q^M (resp. q^M ((M + 1) - M q)) times the term, q = phi(w)/phi(x),
+// so the truncation error of a pole's G terms is at most rho^M and that of its H terms at
+// most rho^M (M + 1 + M rho) times the sum of their magnitudes (rho = |phi(w)| h/D,
+// h = max |x - e|, D = min |x - c_p|); a pole is "far" when the larger bound is below 2^-54,
+// and a wave whose poles are all far takes the expansion (four Horner chains of M terms per
+// pole) instead of the band loop (O(3 nb)).
 #include <cfloat>
 #include <cmath>
 
@@ -53,17 +55,20 @@ constexpr int kBwdMom = 16;  // target moments per frequency
 constexpr double kBwdBound = 0x1p-54;
 
 // Row p's centre (from the bounding box of sample poles among its first 256: any centre is
-// valid, each pole's bound is checked), its target moments A[iw][m] and whether they are
-// usable (finite, and the centre away from the frequency range).
+// valid, each pole's bound is checked), its target moments and whether they are usable
+// (finite, and the centre away from the frequency range):
+//   A[m]     = sum_iw sum_n1 c psi^m / (x - c)      (the frequencies' F summed)
+//   Q[iw][m] = (m + 1) sum_n1 c psi^m / (x - c)^2   (H_iw).
 struct __align__(16) BwdRow {
   double cx, cy;
   int ok, pad_[3];
-  double2 A[3][kBwdMom];
+  double2 A[kBwdMom];
+  double2 Q[3][kBwdMom];
 };
 
-// One workgroup per G' row: lanes take bands n1 = tid, tid + BT, ...; the moments of each
-// frequency are summed per lane, then over each wave by reduce-scatter and over the waves
-// in wave order (fixed order: bitwise reproducible).
+// One workgroup per G' row: lanes take bands n1 = tid, tid + BT, ...; the moments are
+// summed per lane, then over each wave by reduce-scatter and over the waves in wave order
+// (fixed order: bitwise reproducible).
 __global__ __launch_bounds__(BT) void k_gpp_bwd_rowmom(const double2* __restrict__ wt, const double2* __restrict__ C,
                                                       const double* __restrict__ wxb,
                                                       const double* __restrict__ wxrange, int nb, int ncouls,
@@ -134,28 +139,9 @@ __global__ __launch_bounds__(BT) void k_gpp_bwd_rowmom(const double2* __restrict
   const double e0 = xmin - cx, e1 = xmax - cx;
   // every |x - c|^2 in [2^-150, 2^150] (false for NaN)
   bool ok = fma(dx, dx, cy * cy) >= 0x1p-150 && fmax(e0 * e0, e1 * e1) + cy * cy <= 0x1p+150;
-  for (int iw = 0; iw < 3; ++iw) {
-    double v[2 * kBwdMom];
-#pragma unroll
-    for (int j = 0; j < 2 * kBwdMom; ++j) v[j] = 0.0;
-    for (int n1 = tid; n1 < nb; n1 += BT) {
-      const double x = wxb[3 * n1 + iw];
-      const double2 cn = C[(size_t)p * nb + n1];
-      const double a = x - cx, r = 1.0 / fma(a, a, cy * cy);
-      const double tr = a * r, ti = cy * r;  // 1/(x - c)
-      const double sr = (x - e) * tr, si = (x - e) * ti;  // psi
-      double qr = fma(cn.x, tr, -cn.y * ti), qi = fma(cn.x, ti, cn.y * tr);  // c/(x - c)
-#pragma unroll
-      for (int m = 0; m < kBwdMom; ++m) {
-        v[2 * m] += qr;
-        v[2 * m + 1] += qi;
-        if (m + 1 < kBwdMom) {
-          const double t = fma(qr, sr, -qi * si);
-          qi = fma(qr, si, qi * sr);
-          qr = t;
-        }
-      }
-    }
+  // one reduce-scatter of 2 kBwdMom doubles over the block, in fixed order; the total of
+  // component j lands in tot[j] for threads j < 2 kBwdMom
+  auto block_sum = [&](double (&v)[2 * kBwdMom], double* tot) {
     int base = 0;
     bool writer = true;
     wave_sum_scatter<2 * kBwdMom, 32>(v, lane, base, writer);
@@ -170,10 +156,50 @@ __global__ __launch_bounds__(BT) void k_gpp_bwd_rowmom(const double2* __restrict
 #pragma unroll
       for (int w = 1; w < BT / 64; ++w) t += part[w][tid];
       ok = ok && fabs(t) <= DBL_MAX;
-      reinterpret_cast<double*>(rows[p].A[iw])[tid] = t;
+      tot[tid] = t;
     }
     __syncthreads();
+  };
+  double va[2 * kBwdMom];  // A: every frequency's first-order terms
+#pragma unroll
+  for (int j = 0; j < 2 * kBwdMom; ++j) va[j] = 0.0;
+  for (int iw = 0; iw < 3; ++iw) {
+    double v[2 * kBwdMom];
+#pragma unroll
+    for (int j = 0; j < 2 * kBwdMom; ++j) v[j] = 0.0;
+    for (int n1 = tid; n1 < nb; n1 += BT) {
+      const double x = wxb[3 * n1 + iw];
+      const double2 cn = C[(size_t)p * nb + n1];
+      const double a = x - cx, r = 1.0 / fma(a, a, cy * cy);
+      const double tr = a * r, ti = cy * r;  // 1/(x - c)
+      const double sr = (x - e) * tr, si = (x - e) * ti;  // psi
+      double qr = fma(cn.x, tr, -cn.y * ti), qi = fma(cn.x, ti, cn.y * tr);  // c/(x - c)
+      double ur = fma(qr, tr, -qi * ti), ui = fma(qr, ti, qi * tr);          // c/(x - c)^2
+#pragma unroll
+      for (int m = 0; m < kBwdMom; ++m) {
+        va[2 * m] += qr;
+        va[2 * m + 1] += qi;
+        v[2 * m] += ur;
+        v[2 * m + 1] += ui;
+        if (m + 1 < kBwdMom) {
+          const double t = fma(qr, sr, -qi * si);
+          qi = fma(qr, si, qi * sr);
+          qr = t;
+          const double t2 = fma(ur, sr, -ui * si);
+          ui = fma(ur, si, ui * sr);
+          ur = t2;
+        }
+      }
+    }
+    double* Q = reinterpret_cast<double*>(rows[p].Q[iw]);
+    block_sum(v, Q);
+    if (tid < 2 * kBwdMom) {
+      const double q = Q[tid] * (double)(tid / 2 + 1);  // the factor m + 1
+      Q[tid] = q;
+      ok = ok && fabs(q) <= DBL_MAX;
+    }
   }
+  block_sum(va, reinterpret_cast<double*>(rows[p].A));
   const int all_ok = __syncthreads_and(ok ? 1 : 0);
   if (tid == 0) {
     rows[p].cx = cx;
@@ -182,13 +208,19 @@ __global__ __launch_bounds__(BT) void k_gpp_bwd_rowmom(const double2* __restrict
   }
 }
 
-// holomorphic derivatives of gpp_ssx with respect to I, wtilde and wx (branch-wise)
-__device__ inline void ssx_grad(double wx, double2 wt, double2 I, double2& dI, double2& dW, double2& dX) {
-  dI = dW = dX = make_double2(0.0, 0.0);
+// Holomorphic derivatives of gpp_ssx with respect to I, wtilde and wx (branch-wise) for one
+// frequency wx of a band.  The branch is decided by the forward's own operations.  Regular
+// branch (ic = 1/(wx^2 - wt^2)): dI = wt^2 ic, dW = 2 wx^2 wt I ic^2, dX = -2 wx wt^2 I ic^2;
+// its dI and dW are returned as the frequency-dependent factors ic and 2 wx^2 ic^2, summed
+// over the band's frequencies into s1 and s2 (the caller applies wt^2 and wt I once per band);
+// the near-pole branch's dI and dW are summed into oI and oW as they are.  wt2 = wt^2 and
+// w2I = wt^2 I.
+__device__ inline void ssx_grad(double wx, double2 wt, double2 wt2, double2 w2I, double2 I, double2& s1, double2& s2,
+                                double2& oI, double2& oW, double2& dX) {
+  dX = make_double2(0.0, 0.0);
   // cut terms are constant zero; the cut applies only for wx < 0 (tested first, so the
   // value it needs is formed only then)
   if (wx < 0.0 && gpp_ssx_cut(wx, gpp_ssx_raw(wx, wt, I), I)) return;
-  const double2 wt2 = cmul(wt, wt);
   const double2 wdiff = make_double2(wx - wt.x, -wt.y);
   const double wdiffr = cnorm2(wdiff);
   const double2 delw = cscale(cmul(wt, cconj(wdiff)), 1.0 / wdiffr);  // = wt / wdiff
@@ -199,9 +231,9 @@ __device__ inline void ssx_grad(double wx, double2 wt, double2 I, double2& dI, d
     const double2 cden = make_double2(wx * wx - wt2.x, -wt2.y);
     const double2 ic = inv(cden);
     const double2 ic2 = cmul(ic, ic);
-    dI = cmul(wt2, ic);
-    dW = cscale(cmul(cmul(wt, I), ic2), 2.0 * wx * wx);
-    dX = cscale(cmul(cmul(wt2, I), ic2), -2.0 * wx);
+    s1 = cadd(s1, ic);
+    s2 = cadd(s2, cscale(ic2, 2.0 * wx * wx));
+    dX = cscale(cmul(w2I, ic2), -2.0 * wx);
   } else if (delwr > kTo1) {
     const double2 dp = make_double2(delw.x + 0.5, delw.y);
     const double2 idp = inv(dp);
@@ -209,8 +241,8 @@ __device__ inline void ssx_grad(double wx, double2 wt, double2 I, double2& dI, d
     const double2 dq = cscale(cmul(idp, idp), 0.5);
     const double2 iwd = inv(wdiff);
     const double2 iwd2 = cmul(iwd, iwd);
-    dI = cscale(q, -0.25);
-    dW = cscale(cmul(cmul(I, dq), iwd2), -0.25 * wx);
+    oI = cadd(oI, cscale(q, -0.25));
+    oW = cadd(oW, cscale(cmul(cmul(I, dq), iwd2), -0.25 * wx));
     dX = cscale(cmul(cmul(cmul(I, dq), iwd2), wt), 0.25);
   }
 }
@@ -230,8 +262,8 @@ __global__ __launch_bounds__(BT, 3) void k_gpp_bwd(const double2* __restrict__ w
                                                 const BwdRow* __restrict__ rows) {
   __shared__ double2 sc[kMaxBandsLds];
   __shared__ double sx[kMaxBandsLds * 3];
-  __shared__ cdd red[BT / 64];
-  __shared__ double2 sA[FAR ? 3 * kBwdMom : 1];
+  __shared__ dd red[BT / 64][8];
+  __shared__ double2 sA[FAR ? 4 * kBwdMom : 1];  // the row's A, then Q[0..2]
   const int nchunk = (ncouls + BT * GPT - 1) / (BT * GPT);
   const int p = blockIdx.x / nchunk;
   const int ch = blockIdx.x - p * nchunk;
@@ -266,7 +298,8 @@ __global__ __launch_bounds__(BT, 3) void k_gpp_bwd(const double2* __restrict__ w
     // the wave's valid poles far from the band frequencies about the row's centre
     const BwdRow* __restrict__ rw = rows + p;
     const double cx = rw->cx, cy = rw->cy;
-    if (threadIdx.x < 3 * kBwdMom) sA[threadIdx.x] = rw->A[threadIdx.x / kBwdMom][threadIdx.x % kBwdMom];
+    if (threadIdx.x < 4 * kBwdMom)
+      sA[threadIdx.x] = threadIdx.x < kBwdMom ? rw->A[threadIdx.x] : rw->Q[threadIdx.x / kBwdMom - 1][threadIdx.x % kBwdMom];
     const double xmin = wxrange[0], xmax = wxrange[1], e = 0.5 * (xmin + xmax);
     const double h = fmax(xmax - e, e - xmin) * (1.0 + 0x1p-40);
     const double dx = cx < xmin ? xmin - cx : (cx > xmax ? cx - xmax : 0.0);
@@ -282,41 +315,39 @@ __global__ __launch_bounds__(BT, 3) void k_gpp_bwd(const double2* __restrict__ w
       const double zr = w[k].x - cx, zi = w[k].y - cy;
       fr[k] = fma(zr, gr[k], -zi * gi[k]);
       fi[k] = fma(zr, gi[k], zi * gr[k]);  // phi(w)
-      const double af = sqrt(fma(fr[k], fr[k], fi[k] * fi[k])) * (1.0 + 0x1p-40);
-      const double rho = af * hD;
-      const double r2 = rho * rho, r4 = r2 * r2, r8 = r4 * r4, r15 = r8 * r4 * r2 * rho;
-      const double bound = fma(r15, rho, kBwdMom * r15 * (1.0 + rho) * (1.0 + af) * hD);
+      const double rho = sqrt(fma(fr[k], fr[k], fi[k] * fi[k])) * (1.0 + 0x1p-40) * hD;
+      const double r2 = rho * rho, r4 = r2 * r2, r8 = r4 * r4, r16 = r8 * r8;
+      // the second-order tail; it bounds the first-order one (rho^16) too
+      const double bound = r16 * fma(kBwdMom, rho, kBwdMom + 1.0);
       far = far && (!valid[k] || bound <= kBwdBound);  // false for NaN
     }
     wave_far = __all(far) != 0;
     need_loop = __syncthreads_or(wave_far ? 0 : 1) != 0;  // also orders the sA writes before their reads
     if (wave_far) {
-      // G = sum_iw F_iw, F_iw = (c - e) g P_iw(phi), H_iw = (c - e) g^2 (-P_iw + (c - e) g P_iw')
+      // G = (c - e) g P_A(phi), H_iw = (c - e)^2 g^2 P_Q_iw(phi)
       const double cer = cx - e, cei = cy;
 #pragma unroll
       for (int k = 0; k < GPT; ++k) {
-        const double cgr = fma(cer, gr[k], -cei * gi[k]), cgi = fma(cer, gi[k], cei * gr[k]);
+        const double cgr = fma(cer, gr[k], -cei * gi[k]), cgi = fma(cer, gi[k], cei * gr[k]);  // (c - e) g
+        const double c2r = fma(cgr, cgr, -cgi * cgi), c2i = 2.0 * cgr * cgi;                  // its square
 #pragma unroll
-        for (int iw = 0; iw < 3; ++iw) {
-          const double2* __restrict__ A = sA + iw * kBwdMom;
-          double pr = A[kBwdMom - 1].x, pi = A[kBwdMom - 1].y, dr = 0.0, di = 0.0;
+        for (int j = 0; j < 4; ++j) {  // j = 0: P_A; j = 1..3: P_Q of frequency j - 1
+          const double2* __restrict__ cf = sA + j * kBwdMom;
+          double pr = cf[kBwdMom - 1].x, pi = cf[kBwdMom - 1].y;
 #pragma unroll 3
           for (int m = kBwdMom - 2; m >= 0; --m) {
-            const double ndr = fma(dr, fr[k], fma(-di, fi[k], pr));
-            di = fma(dr, fi[k], fma(di, fr[k], pi));
-            dr = ndr;
-            const double2 a = A[m];
+            const double2 a = cf[m];
             const double npr = fma(pr, fr[k], fma(-pi, fi[k], a.x));
             pi = fma(pr, fi[k], fma(pi, fr[k], a.y));
             pr = npr;
           }
-          // F = cg P;  H = cg g (cg P' - P)
-          G[k][0] += fma(cgr, pr, -cgi * pi);
-          G[k][1] += fma(cgr, pi, cgi * pr);
-          const double ur = fma(cgr, dr, -cgi * di) - pr, ui = fma(cgr, di, cgi * dr) - pi;
-          const double vr = fma(cgr, gr[k], -cgi * gi[k]), vi = fma(cgr, gi[k], cgi * gr[k]);
-          H[k][iw][0] = fma(vr, ur, -vi * ui);
-          H[k][iw][1] = fma(vr, ui, vi * ur);
+          if (j == 0) {
+            G[k][0] = fma(cgr, pr, -cgi * pi);
+            G[k][1] = fma(cgr, pi, cgi * pr);
+          } else {
+            H[k][j - 1][0] = fma(c2r, pr, -c2i * pi);
+            H[k][j - 1][1] = fma(c2r, pi, c2i * pr);
+          }
         }
       }
     }
@@ -391,18 +422,22 @@ __global__ __launch_bounds__(BT, 3) void k_gpp_bwd(const double2* __restrict__ w
       if (!valid[k]) continue;
       const int g = ch * BT * GPT + k * BT + threadIdx.x;
       const double2 ma = cmul(m, aqsn[(size_t)n1 * ncouls + g]);
+      const double2 wt2 = cmul(w[k], w[k]), w2I = cmul(wt2, I[k]);
+      double2 s1 = make_double2(0.0, 0.0), s2 = s1, oI = s1, oW = s1;
 #pragma unroll
       for (int iw = 0; iw < 3; ++iw) {
-        double2 gI, gW, gX;
-        ssx_grad(wxb[3 * n1 + iw], w[k], I[k], gI, gW, gX);
-        const double2 tI = cmul(ma, gI), tW = cmul(ma, gW), tX = cmul(ma, gX);
-        dIs[k][0] += tI.x;
-        dIs[k][1] += tI.y;
-        dWs[k][0] += tW.x;
-        dWs[k][1] += tW.y;
+        double2 gX;
+        ssx_grad(wxb[3 * n1 + iw], w[k], wt2, w2I, I[k], s1, s2, oI, oW, gX);
+        const double2 tX = cmul(ma, gX);
         dsx[iw][0] += tX.x;
         dsx[iw][1] += tX.y;
       }
+      const double2 tI = cmul(ma, cadd(cmul(wt2, s1), oI));
+      const double2 tW = cmul(ma, cadd(cmul(cmul(w[k], I[k]), s2), oW));
+      dIs[k][0] += tI.x;
+      dIs[k][1] += tI.y;
+      dWs[k][0] += tW.x;
+      dWs[k][1] += tW.y;
     }
   }
 
@@ -427,15 +462,30 @@ __global__ __launch_bounds__(BT, 3) void k_gpp_bwd(const double2* __restrict__ w
     const double2 t = cmul(I[k], Gc);
     Tp = cdd_add(Tp, t.x, t.y);
   }
+  // the block's eight double-double totals (re and im of dSigma/domega[0..2] and of T):
+  // one reduce-scatter over each wave, then the waves in wave order
+  dd v[8];
 #pragma unroll
   for (int iw = 0; iw < 3; ++iw) {
     ds[iw] = cdd_add(ds[iw], dsx[iw][0], dsx[iw][1]);
-    cdd r = cdd_block_reduce(ds[iw], red);
-    if (threadIdx.x == 0) cdd_store(&ds_part[(size_t)blockIdx.x * 3 + iw], r);
-    __syncthreads();
+    v[2 * iw] = ds[iw].re;
+    v[2 * iw + 1] = ds[iw].im;
   }
-  cdd r = cdd_block_reduce(Tp, red);
-  if (threadIdx.x == 0) cdd_store(&T_part[blockIdx.x], r);
+  v[6] = Tp.re;
+  v[7] = Tp.im;
+  int comp = 0;
+  bool writer = true;
+  dd_wave_sum_scatter<8>(v, threadIdx.x & 63, comp, writer);
+  if (writer) red[threadIdx.x >> 6][comp] = v[0];
+  __syncthreads();
+  if (threadIdx.x < 8) {
+    dd t = red[0][threadIdx.x];
+#pragma unroll
+    for (int w = 1; w < BT / 64; ++w) t = dd_add(t, red[w][threadIdx.x]);
+    // stored as cdd partials (re.hi, re.lo, im.hi, im.lo): component 2k is the real part
+    double4* dst = threadIdx.x < 6 ? &ds_part[(size_t)blockIdx.x * 3 + threadIdx.x / 2] : &T_part[blockIdx.x];
+    reinterpret_cast<double2*>(dst)[threadIdx.x & 1] = make_double2(t.hi, t.lo);
+  }
 }
 
 // dL/dwtilde(p, igp(p)) += conj(T(p)), T(p) = sum over the G-chunks of p (fixed order).
