@@ -223,11 +223,12 @@ __device__ inline void ssx_grad(double wx, double2 wt, double2 wt2, double2 w2I,
   if (wx < 0.0 && gpp_ssx_cut(wx, gpp_ssx_raw(wx, wt, I), I)) return;
   const double2 wdiff = make_double2(wx - wt.x, -wt.y);
   const double wdiffr = cnorm2(wdiff);
-  const double2 delw = cscale(cmul(wt, cconj(wdiff)), 1.0 / wdiffr);  // = wt / wdiff
-  const double delwr = cnorm2(delw);
+  const SsxBranch br = gpp_ssx_branch(wt, wdiff, wdiffr);
+  const double2 delw = br.delw;  // = wt / wdiff, formed when the branch is not regular
+  const double delwr = br.delwr;
   // 1/z = conj(z)/|z|^2
   auto inv = [](double2 z) { return cscale(cconj(z), 1.0 / cnorm2(z)); };
-  if (wdiffr > kLimitTwo && delwr < kLimitOne) {
+  if (br.regular) {
     const double2 cden = make_double2(wx * wx - wt2.x, -wt2.y);
     const double2 ic = inv(cden);
     const double2 ic2 = cmul(ic, ic);

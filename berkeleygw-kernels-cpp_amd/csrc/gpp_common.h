@@ -94,15 +94,41 @@ __device__ inline SsxG gpp_ssx_g(double2 wt, double2 I) {
   return s;
 }
 
+// The branch of the ssx term: regular when wdiffr > limittwo and |delw|^2 < limitone, with
+// delw = wt conj(wdiff) / wdiffr and wdiffr = |wdiff|^2.  Most terms are decided without the
+// division: |wt conj(wdiff)|^2 < limitone wdiffr^2 (1 - 2^-40) implies that |delw|^2 formed
+// with the division (a few roundings, each within 2^-53) is below limitone too.  Otherwise
+// (near the threshold, an overflow, NaN) delw is formed and the test made exactly, so every
+// term takes the branch the exact test gives; delw is valid whenever regular is false.
+struct SsxBranch {
+  bool regular;
+  double2 delw;
+  double delwr;
+};
+__device__ inline SsxBranch gpp_ssx_branch(double2 wt, double2 wdiff, double wdiffr) {
+  SsxBranch b;
+  const double2 a = cmul(wt, cconj(wdiff));
+  b.regular = wdiffr > kLimitTwo && cnorm2(a) < kLimitOne * (wdiffr * wdiffr) * (1.0 - 0x1p-40);
+  b.delw = make_double2(0.0, 0.0);
+  b.delwr = 0.0;
+  if (!b.regular) {
+    const double rden = 1.0 / wdiffr;
+    b.delw = cscale(a, rden);
+    b.delwr = cnorm2(b.delw);
+    b.regular = wdiffr > kLimitTwo && b.delwr < kLimitOne;
+  }
+  return b;
+}
+
 __device__ inline double2 gpp_ssx_raw(double wx, const SsxG& g) {
   const double2 wt = g.wt, wt2 = g.wt2, Omega2 = g.Omega2;
   const double2 wdiff = make_double2(wx - wt.x, -wt.y);
   const double wdiffr = cnorm2(wdiff);
-  const double rden = 1.0 / wdiffr;
-  const double2 delw = cscale(cmul(wt, cconj(wdiff)), rden);
-  const double delwr = cnorm2(delw);
+  const SsxBranch br = gpp_ssx_branch(wt, wdiff, wdiffr);
+  const double2 delw = br.delw;
+  const double delwr = br.delwr;
   double2 ssx;
-  if (wdiffr > kLimitTwo && delwr < kLimitOne) {
+  if (br.regular) {
     const double2 cden = make_double2(wx * wx - wt2.x, -wt2.y);
     const double rd = 1.0 / cnorm2(cden);
     ssx = cscale(cmul(Omega2, cconj(cden)), rd);
