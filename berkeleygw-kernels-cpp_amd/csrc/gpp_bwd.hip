@@ -327,28 +327,42 @@ __global__ __launch_bounds__(BT, 3) void k_gpp_bwd(const double2* __restrict__ w
     if (wave_far) {
       // G = (c - e) g P_A(phi), H_iw = (c - e)^2 g^2 P_Q_iw(phi)
       const double cer = cx - e, cei = cy;
+      // the 4 GPT Horner chains (P_A and the three P_Q_iw of each pole) advance together, so
+      // consecutive FMAs are independent
+      double pr[GPT][4], pi[GPT][4];
+#pragma unroll
+      for (int k = 0; k < GPT; ++k) {
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          pr[k][j] = sA[j * kBwdMom + kBwdMom - 1].x;
+          pi[k][j] = sA[j * kBwdMom + kBwdMom - 1].y;
+        }
+      }
+#pragma unroll 3
+      for (int m = kBwdMom - 2; m >= 0; --m) {
+        double2 a[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) a[j] = sA[j * kBwdMom + m];
+#pragma unroll
+        for (int k = 0; k < GPT; ++k) {
+#pragma unroll
+          for (int j = 0; j < 4; ++j) {
+            const double npr = fma(pr[k][j], fr[k], fma(-pi[k][j], fi[k], a[j].x));
+            pi[k][j] = fma(pr[k][j], fi[k], fma(pi[k][j], fr[k], a[j].y));
+            pr[k][j] = npr;
+          }
+        }
+      }
 #pragma unroll
       for (int k = 0; k < GPT; ++k) {
         const double cgr = fma(cer, gr[k], -cei * gi[k]), cgi = fma(cer, gi[k], cei * gr[k]);  // (c - e) g
         const double c2r = fma(cgr, cgr, -cgi * cgi), c2i = 2.0 * cgr * cgi;                  // its square
+        G[k][0] = fma(cgr, pr[k][0], -cgi * pi[k][0]);
+        G[k][1] = fma(cgr, pi[k][0], cgi * pr[k][0]);
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {  // j = 0: P_A; j = 1..3: P_Q of frequency j - 1
-          const double2* __restrict__ cf = sA + j * kBwdMom;
-          double pr = cf[kBwdMom - 1].x, pi = cf[kBwdMom - 1].y;
-#pragma unroll 3
-          for (int m = kBwdMom - 2; m >= 0; --m) {
-            const double2 a = cf[m];
-            const double npr = fma(pr, fr[k], fma(-pi, fi[k], a.x));
-            pi = fma(pr, fi[k], fma(pi, fr[k], a.y));
-            pr = npr;
-          }
-          if (j == 0) {
-            G[k][0] = fma(cgr, pr, -cgi * pi);
-            G[k][1] = fma(cgr, pi, cgi * pr);
-          } else {
-            H[k][j - 1][0] = fma(c2r, pr, -c2i * pi);
-            H[k][j - 1][1] = fma(c2r, pi, c2i * pr);
-          }
+        for (int iw = 0; iw < 3; ++iw) {
+          H[k][iw][0] = fma(c2r, pr[k][iw + 1], -c2i * pi[k][iw + 1]);
+          H[k][iw][1] = fma(c2r, pi[k][iw + 1], c2i * pr[k][iw + 1]);
         }
       }
     }
