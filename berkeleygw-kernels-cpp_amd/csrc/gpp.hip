@@ -1011,9 +1011,11 @@ __global__ __launch_bounds__(NT) void k_gpp_asx(const int* __restrict__ inv, con
         const double2 am = aqsm[(size_t)n1 * ncouls + igp];
         const double2 m = make_double2(am.x * occ * vc, -am.y * occ * vc);
         const double2 mn = cmul(m, aqsn[(size_t)n1 * ncouls + g]);
+        double2 v[3];
+        gpp_ssx3(wxb + n1 * 3, G, v);
 #pragma unroll
         for (int iw = 0; iw < 3; ++iw) {
-          const double2 t = cmul(mn, gpp_ssx(wxb[n1 * 3 + iw], G));
+          const double2 t = cmul(mn, v[iw]);
           s[iw][0] += t.x;
           s[iw][1] += t.y;
         }

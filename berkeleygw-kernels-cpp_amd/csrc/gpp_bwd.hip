@@ -215,6 +215,15 @@ __global__ __launch_bounds__(BT) void k_gpp_bwd_rowmom(const double2* __restrict
 // over the band's frequencies into s1 and s2 (the caller applies wt^2 and wt I once per band);
 // the near-pole branch's dI and dW are summed into oI and oW as they are.  wt2 = wt^2 and
 // w2I = wt^2 I.
+__device__ inline void ssx_grad_regular(double wx, double2 wt2, double2 w2I, double2& s1, double2& s2, double2& dX) {
+  const double2 cden = make_double2(wx * wx - wt2.x, -wt2.y);
+  const double2 ic = cscale(cconj(cden), 1.0 / cnorm2(cden));  // 1/cden
+  const double2 ic2 = cmul(ic, ic);
+  s1 = cadd(s1, ic);
+  s2 = cadd(s2, cscale(ic2, 2.0 * wx * wx));
+  dX = cscale(cmul(w2I, ic2), -2.0 * wx);
+}
+
 __device__ inline void ssx_grad(double wx, double2 wt, double2 wt2, double2 w2I, double2 I, double2& s1, double2& s2,
                                 double2& oI, double2& oW, double2& dX) {
   dX = make_double2(0.0, 0.0);
@@ -229,12 +238,7 @@ __device__ inline void ssx_grad(double wx, double2 wt, double2 wt2, double2 w2I,
   // 1/z = conj(z)/|z|^2
   auto inv = [](double2 z) { return cscale(cconj(z), 1.0 / cnorm2(z)); };
   if (br.regular) {
-    const double2 cden = make_double2(wx * wx - wt2.x, -wt2.y);
-    const double2 ic = inv(cden);
-    const double2 ic2 = cmul(ic, ic);
-    s1 = cadd(s1, ic);
-    s2 = cadd(s2, cscale(ic2, 2.0 * wx * wx));
-    dX = cscale(cmul(w2I, ic2), -2.0 * wx);
+    ssx_grad_regular(wx, wt2, w2I, s1, s2, dX);
   } else if (delwr > kTo1) {
     const double2 dp = make_double2(delw.x + 0.5, delw.y);
     const double2 idp = inv(dp);
@@ -438,12 +442,26 @@ __global__ __launch_bounds__(BT, 3) void k_gpp_bwd(const double2* __restrict__ w
       const int g = ch * BT * GPT + k * BT + threadIdx.x;
       const double2 ma = cmul(m, aqsn[(size_t)n1 * ncouls + g]);
       const double2 wt2 = cmul(w[k], w[k]), w2I = cmul(wt2, I[k]);
-      double2 s1 = make_double2(0.0, 0.0), s2 = s1, oI = s1, oW = s1;
+      double2 s1 = make_double2(0.0, 0.0), s2 = s1, oI = s1, oW = s1, gX[3];
+      // the band's three frequencies in straight-line code when all are >= 0 (no cut-off)
+      // and surely regular, so their divisions overlap; otherwise branch by branch
+      bool reg = true;
 #pragma unroll
       for (int iw = 0; iw < 3; ++iw) {
-        double2 gX;
-        ssx_grad(wxb[3 * n1 + iw], w[k], wt2, w2I, I[k], s1, s2, oI, oW, gX);
-        const double2 tX = cmul(ma, gX);
+        const double wx = wxb[3 * n1 + iw];
+        const double2 wdiff = make_double2(wx - w[k].x, -w[k].y);
+        reg = reg && wx >= 0.0 && gpp_ssx_surely_regular(w[k], wdiff, cnorm2(wdiff));
+      }
+      if (reg) {
+#pragma unroll
+        for (int iw = 0; iw < 3; ++iw) ssx_grad_regular(wxb[3 * n1 + iw], wt2, w2I, s1, s2, gX[iw]);
+      } else {
+#pragma unroll
+        for (int iw = 0; iw < 3; ++iw) ssx_grad(wxb[3 * n1 + iw], w[k], wt2, w2I, I[k], s1, s2, oI, oW, gX[iw]);
+      }
+#pragma unroll
+      for (int iw = 0; iw < 3; ++iw) {
+        const double2 tX = cmul(ma, gX[iw]);
         dsx[iw][0] += tX.x;
         dsx[iw][1] += tX.y;
       }

@@ -100,6 +100,13 @@ __device__ inline SsxG gpp_ssx_g(double2 wt, double2 I) {
 // with the division (a few roundings, each within 2^-53) is below limitone too.  Otherwise
 // (near the threshold, an overflow, NaN) delw is formed and the test made exactly, so every
 // term takes the branch the exact test gives; delw is valid whenever regular is false.
+// gpp_ssx_surely_regular is the division-free test: true only where the exact test gives
+// the regular branch.
+__device__ inline bool gpp_ssx_surely_regular(double2 wt, double2 wdiff, double wdiffr) {
+  const double2 a = cmul(wt, cconj(wdiff));
+  return wdiffr > kLimitTwo && cnorm2(a) < kLimitOne * (wdiffr * wdiffr) * (1.0 - 0x1p-40);
+}
+
 struct SsxBranch {
   bool regular;
   double2 delw;
@@ -107,13 +114,12 @@ struct SsxBranch {
 };
 __device__ inline SsxBranch gpp_ssx_branch(double2 wt, double2 wdiff, double wdiffr) {
   SsxBranch b;
-  const double2 a = cmul(wt, cconj(wdiff));
-  b.regular = wdiffr > kLimitTwo && cnorm2(a) < kLimitOne * (wdiffr * wdiffr) * (1.0 - 0x1p-40);
+  b.regular = gpp_ssx_surely_regular(wt, wdiff, wdiffr);
   b.delw = make_double2(0.0, 0.0);
   b.delwr = 0.0;
   if (!b.regular) {
     const double rden = 1.0 / wdiffr;
-    b.delw = cscale(a, rden);
+    b.delw = cscale(cmul(wt, cconj(wdiff)), rden);
     b.delwr = cnorm2(b.delw);
     b.regular = wdiffr > kLimitTwo && b.delwr < kLimitOne;
   }
@@ -159,6 +165,30 @@ __device__ inline double2 gpp_ssx(double wx, const SsxG& g) {
 }
 
 __device__ inline double2 gpp_ssx(double wx, double2 wt, double2 I) { return gpp_ssx(wx, gpp_ssx_g(wt, I)); }
+
+// ssx of one G-vector at a band's three frequencies.  When every frequency is >= 0 (no
+// cut-off) and surely regular, the three regular values are formed in straight-line code,
+// so their divisions overlap instead of each waiting behind its own branch; otherwise each
+// frequency takes gpp_ssx.  Either way each value is the one of its branch.
+__device__ inline void gpp_ssx3(const double* __restrict__ wx, const SsxG& g, double2 (&out)[3]) {
+  bool reg = true;
+#pragma unroll
+  for (int iw = 0; iw < 3; ++iw) {
+    const double2 wdiff = make_double2(wx[iw] - g.wt.x, -g.wt.y);
+    reg = reg && wx[iw] >= 0.0 && gpp_ssx_surely_regular(g.wt, wdiff, cnorm2(wdiff));
+  }
+  if (reg) {
+#pragma unroll
+    for (int iw = 0; iw < 3; ++iw) {
+      const double2 cden = make_double2(wx[iw] * wx[iw] - g.wt2.x, -g.wt2.y);
+      const double rd = 1.0 / cnorm2(cden);
+      out[iw] = cscale(cmul(g.Omega2, cconj(cden)), rd);
+    }
+  } else {
+#pragma unroll
+    for (int iw = 0; iw < 3; ++iw) out[iw] = gpp_ssx(wx[iw], g);
+  }
+}
 
 // MFMA trace contraction (contract.hip):
 //   out = sum_{r} sum_p W(r,p) sum_g A(rows[r], g) * B(p, g),
