@@ -43,6 +43,33 @@ def test_near_pole_branches_vs_oracle(ctx):
     assert rel_err(g["dsigma_domega"], O.gpp_dsigma_domega(d, occ=1.0)) < 1e-12
 
 
+def test_ssx_mixed_branches_within_a_band_vs_oracle(ctx):
+    # The screened-exchange terms of a band's three frequencies are formed together when all
+    # three are surely regular and >= 0; these inputs break that for some G-vectors: 40 poles
+    # next to valence band 0's middle frequency (|wdiff|^2 = 5e-4 < limittwo there: the
+    # near-pole branch, the other two frequencies regular), and valence band 1 with a negative
+    # first frequency (the cut-off test).  Forward asxtemp and the gradients against the oracle.
+    d = bgw.gpp_inputs(24, 2, 256, 8, 31)
+    ngpown, ncouls = d["ngpown"], d["ncouls"]
+    wx = d["wx"].reshape(-1, 3).copy()
+    wx[1, 0] = -0.3
+    d["wx"] = wx.reshape(-1).copy()
+    wt = d["wtilde"].copy()
+    idx = np.random.default_rng(5).choice(ngpown * ncouls, 40, replace=False)
+    wt[idx] = wx[0, 1] + 0.02 - 0.01j
+    d["wtilde"] = wt
+    r = ctx.gpp_solver(d, occ=1.0)
+    o = O.gpp_forward(d, occ=1.0)
+    assert rel_err(r["asxtemp"], o["asxtemp"]) < 1e-12
+    assert rel_err(r["achtemp"], o["achtemp"]) < 1e-12
+    g = ctx.gpp_backward(d, occ=1.0)
+    e = np.arange(ngpown * ncouls, dtype=np.int64)
+    dw, dI = O.gpp_grad_entries(d, e, occ=1.0)
+    assert np.max(np.abs(g["dL_dwtilde"] - dw)) <= 1e-12 * np.max(np.abs(dw))
+    assert np.max(np.abs(g["dL_dI_eps"] - dI)) <= 1e-12 * np.max(np.abs(dI))
+    assert rel_err(g["dsigma_domega"], O.gpp_dsigma_domega(d, occ=1.0)) < 1e-12
+
+
 def test_more_bands_than_one_lds_chunk_vs_oracle(ctx):
     # 1100 bands: the band loop stages c(n1,p) and wx in LDS chunks of 1024
     d = bgw.gpp_inputs(1100, 3, 300, 4, 1100)
